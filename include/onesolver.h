@@ -1,0 +1,83 @@
+/*
+ * onesolver.h -- C ABI of the MI355X simulated-annealing QUBO library
+ * (libonesolver.so).
+ *
+ * The reference (oneSolver) is a header-only C++ template library with no FFI
+ * layer; each entry point below names the reference interface it replaces.
+ * Conventions:
+ *   - every call returns OS_OK (0) or a negative OS_ERR_* code; the message of
+ *     the last failing call on the calling thread is os_last_error();
+ *   - host arrays are caller-owned and only read/written during the call
+ *     (copied to and from device memory inside);
+ *   - a context owns one GPU, one HIP stream and its device buffers; calls on
+ *     one context are synchronous and must not overlap; separate contexts
+ *     are independent;
+ *   - QUBO matrices are dense row-major symmetric N x N ("flat" in the
+ *     reference: include/helpers/qubo_helpers.hpp:26-44) with the linear
+ *     terms on the diagonal; energy E(x) = sum_{i<=j} Q_ij x_i x_j
+ *     (include/simulated_annealing/annealing.hpp:31-40).
+ */
+#ifndef ONESOLVER_H_
+#define ONESOLVER_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OS_OK 0
+#define OS_ERR_INVALID (-1)     /* invalid argument (reference: std::invalid_argument) */
+#define OS_ERR_HIP (-2)         /* HIP runtime error (reference: sycl::exception) */
+#define OS_ERR_UNSUPPORTED (-3) /* size or option outside what the kernels support */
+#define OS_ERR_INTERNAL (-4)
+
+typedef struct os_ctx os_ctx;
+
+/* Library version as major*10000 + minor*100 + patch. */
+int os_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *os_last_error(void);
+
+/* Number of visible GPUs.  Replaces the SYCL device enumeration behind
+ * devices::construct_device_selector (include/helpers/devices.hpp:27-42). */
+int os_device_count(int *count);
+
+/* Create / destroy a context on GPU `device`.  Replaces the sycl::queue of
+ * app/one-solver-anneal.cpp:139-147. */
+int os_ctx_create(int device, os_ctx **out);
+int os_ctx_destroy(os_ctx *ctx);
+
+/* Device name ("Using device: ..." line, app/one-solver-anneal.cpp:149-151). */
+int os_ctx_device_name(os_ctx *ctx, char *buf, size_t len);
+
+/* Reference-semantics simulated annealing.  Replaces sa::anneal
+ * (include/simulated_annealing/annealing.hpp:55-140):
+ *   - num_tries independent replicas, each starting from uniform random bits;
+ *   - num_iter * sweeps_per_beta attempts per replica; attempt t proposes a
+ *     uniformly random site and uses beta[t / sweeps_per_beta];
+ *   - accept iff E_new < E_cur || exp((E_cur - E_new) / beta) > u, u~U[0,1)
+ *     (beta acts as a temperature, as in the reference);
+ *   - the best state of each replica is tracked after every attempt.
+ * q_sym: N x N FP64.  beta: num_iter values.  The RNG is Philox4x32-10 keyed
+ * by `seed` (the reference's fixed seed is 1234).
+ * Outputs (each may be NULL): best_states [num_tries x n] (0/1 bytes),
+ * best_energies [num_tries] (FP64, recomputed from the returned states),
+ * best_state [n] and best_energy: the replica with the lowest energy, first
+ * index on ties (annealing.hpp:134-139). */
+int os_anneal_reference(os_ctx *ctx, int n, const double *q_sym, const double *beta,
+                        int num_iter, int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                        uint8_t *best_states, double *best_energies, uint8_t *best_state,
+                        double *best_energy);
+
+/* FP64 QUBO energies of `count` states (0/1 bytes, count x n).  Replaces
+ * sa::energy (include/simulated_annealing/annealing.hpp:31-40). */
+int os_energy_fp64(os_ctx *ctx, int n, const double *q_sym, uint32_t count,
+                   const uint8_t *states, double *energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ONESOLVER_H_ */

@@ -1,0 +1,228 @@
+/*! file one-solver-anneal.cpp
+ *
+ * Drop-in for oneSolver's one-solver-anneal (app/one-solver-anneal.cpp:41-178):
+ * same flags, defaults, validation messages, stdout log lines, exit codes and
+ * output file.  `--device-type gpu` runs on the MI355X through
+ * libonesolver.so; `host` uses the calling thread and `cpu` all host cores.
+ *
+ * Options are parsed like Boost.Program_options' command-line style:
+ * "--name value" or "--name=value", unambiguous prefixes accepted, each
+ * option at most once, no positional arguments.  Parse errors print
+ * "error: <message>" and exit 1; validation failures exit with -1 (255).
+ *
+ * Extensions (optional, reference-compatible defaults): --seed (the
+ * reference's fixed engine seed, 1234) and --sweeps-per-beta (the library
+ * parameter of sa::anneal that the reference CLI leaves at 1).
+ */
+#include <cstdlib>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "onesolver/helpers/devices.hpp"
+#include "onesolver/model/qubo.hpp"
+#include "onesolver/model/solution.hpp"
+#include "onesolver/simulated_annealing/annealing.hpp"
+#include "onesolver/simulated_annealing/schedules.hpp"
+
+namespace {
+
+const char *kHelp =
+    "Allowed options:\n"
+    "  --help                                produce help message\n"
+    "  --input arg                           input file\n"
+    "  --output arg                          output file\n"
+    "  --num-iter arg (=100)                 number of iterations of the algorithm\n"
+    "  --num-tries arg (=100)                number of trajectories to try\n"
+    "  --schedule-type arg (=geometric)      type of beta schedule tu use, either \n"
+    "                                        linear or geometric\n"
+    "  --beta-min arg (=0.10000000000000001) minimum value of beta in the annealing \n"
+    "                                        schedule (default 0.1)\n"
+    "  --beta-max arg (=1)                   maximum value of beta in the annealing \n"
+    "                                        schedule (default 1.0)\n"
+    "  --device-type arg (=host)             device type to use (cpu, gpu or host)\n"
+    "\n"
+    "Extensions:\n"
+    "  --seed arg (=1234)                    seed of the Philox4x32-10 streams\n"
+    "  --sweeps-per-beta arg (=1)            attempts per beta value\n";
+
+struct Options {
+  std::map<std::string, std::string> values;
+  bool help = false;
+};
+
+const std::vector<std::string> kNames = {"help",     "input",       "output",
+                                         "num-iter", "num-tries",   "schedule-type",
+                                         "beta-min", "beta-max",    "device-type",
+                                         "seed",     "sweeps-per-beta"};
+
+std::string resolve(const std::string &given) {
+  for (const auto &n : kNames)
+    if (n == given) return n;
+  std::vector<std::string> hits;
+  for (const auto &n : kNames)
+    if (n.compare(0, given.size(), given) == 0) hits.push_back(n);
+  if (hits.empty()) throw std::runtime_error("unrecognised option '--" + given + "'");
+  if (hits.size() > 1) {
+    std::string m = "option '--" + given + "' is ambiguous and matches ";
+    for (size_t i = 0; i < hits.size(); ++i)
+      m += (i ? (i + 1 == hits.size() ? ", and " : ", ") : "") + ("'--" + hits[i] + "'");
+    throw std::runtime_error(m);
+  }
+  return hits[0];
+}
+
+Options parse(int argc, char *argv[]) {
+  Options o;
+  for (int i = 1; i < argc; ++i) {
+    std::string tok = argv[i];
+    if (tok.size() < 3 || tok.compare(0, 2, "--") != 0) {
+      if (tok.size() >= 2 && tok[0] == '-') throw std::runtime_error("unrecognised option '" + tok + "'");
+      throw std::runtime_error("too many positional options have been specified on the command line");
+    }
+    std::string body = tok.substr(2), val;
+    bool has_val = false;
+    const size_t eq = body.find('=');
+    if (eq != std::string::npos) {
+      val = body.substr(eq + 1);
+      body = body.substr(0, eq);
+      has_val = true;
+    }
+    const std::string name = resolve(body);
+    if (name == "help") {
+      if (has_val) throw std::runtime_error("option '--help' does not take any arguments");
+      o.help = true;
+      continue;
+    }
+    if (!has_val) {
+      if (i + 1 >= argc)
+        throw std::runtime_error("the required argument for option '--" + name + "' is missing");
+      val = argv[++i];
+    }
+    if (o.values.count(name))
+      throw std::runtime_error("option '--" + name + "' cannot be specified more than once");
+    o.values[name] = val;
+  }
+  return o;
+}
+
+template <typename T>
+T convert(const Options &o, const std::string &name, T def) {
+  auto it = o.values.find(name);
+  if (it == o.values.end()) return def;
+  std::istringstream is(it->second);
+  T v;
+  if (!(is >> v) || !is.eof() || (std::is_unsigned<T>::value && it->second.find('-') != std::string::npos))
+    throw std::runtime_error("the argument ('" + it->second + "') for option '--" + name +
+                             "' is invalid");
+  return v;
+}
+
+template <>
+std::string convert<std::string>(const Options &o, const std::string &name, std::string def) {
+  auto it = o.values.find(name);
+  return it == o.values.end() ? def : it->second;
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  std::string input_file, output_file, schedule_type, device_type;
+  unsigned num_iter, num_tries;
+  double beta_min, beta_max;
+  try {
+    Options opt = parse(argc, argv);
+    if (opt.help) {
+      std::cout << kHelp << std::endl;
+      return 0;
+    }
+    num_iter = convert<unsigned>(opt, "num-iter", 100u);
+    num_tries = convert<unsigned>(opt, "num-tries", 100u);
+    schedule_type = convert<std::string>(opt, "schedule-type", "geometric");
+    beta_min = convert<double>(opt, "beta-min", 0.1);
+    beta_max = convert<double>(opt, "beta-max", 1.0);
+    device_type = convert<std::string>(opt, "device-type", "host");
+    const uint64_t seed = convert<uint64_t>(opt, "seed", 1234u);
+    const unsigned sweeps_per_beta = convert<unsigned>(opt, "sweeps-per-beta", 1u);
+
+    if (!opt.values.count("input")) {
+      std::cerr << "No input file provided." << std::endl;
+      return -1;
+    }
+    input_file = opt.values["input"];
+    if (!opt.values.count("output")) {
+      std::cerr << "No output file provided." << std::endl;
+      return -1;
+    }
+    output_file = opt.values["output"];
+    if (device_type != "cpu" && device_type != "gpu" && device_type != "host") {
+      std::cerr << "Unknown device type: " << device_type << std::endl;
+      return -1;
+    }
+    if (schedule_type != "linear" && schedule_type != "geometric") {
+      std::cerr << "Unknown beta schedule: " << schedule_type << std::endl;
+      return -1;
+    }
+    if (beta_max < 0 || beta_min < 0) {
+      std::cerr << "Invalid schedule, both ends of beta range need to be positive" << std::endl;
+      return -1;
+    }
+    if (beta_min >= beta_max) {
+      std::cerr << "Invalid schedule, initial beta is not lesser than final beta" << std::endl;
+      return -1;
+    }
+    if (sweeps_per_beta < 1) {
+      std::cerr << "Invalid sweeps per beta, need at least 1" << std::endl;
+      return -1;
+    }
+
+    std::cout << "Reading input from: " << input_file << std::endl;
+    std::cout << "Output will be saved to: " << output_file << std::endl;
+    std::cout << "Schedule type: " << schedule_type << std::endl;
+    std::cout << "Beta range: [" << beta_min << ", " << beta_max << "]" << std::endl;
+    std::cout << "Number of iterations: " << num_iter << std::endl;
+    std::cout << "Number of tries: " << num_tries << std::endl;
+
+    std::ifstream qubo_file(input_file);
+    if (!qubo_file) {
+      std::cerr << "can not open input file: " << input_file << std::endl;
+      return -1;
+    }
+    qubo_file.unsetf(std::ios::skipws);
+    auto instance = qubo::QUBOModel<int, double>::load(qubo_file);
+
+    std::unique_ptr<devices::Device> device;
+    try {
+      device.reset(new devices::Device(*devices::construct_device_selector(device_type)));
+    } catch (std::runtime_error &) {
+      std::cerr << "No devices of given type could be initialized." << std::endl;
+      return 1;
+    }
+    std::cout << "Using device: " << device->name() << std::endl;
+
+    std::vector<double> beta_schedule(num_iter);
+    if (num_iter > 0) {
+      if (schedule_type == "linear")
+        construct_linear_beta_schedule(beta_schedule, beta_min, beta_max, num_iter);
+      else
+        construct_geometric_beta_schedule(beta_schedule, beta_min, beta_max, num_iter);
+    }
+
+    auto solution = sa::anneal(instance, *device, beta_schedule, (int)num_iter, num_tries,
+                               (int)sweeps_per_beta, seed);
+
+    std::ofstream results_file(output_file);
+    solution.save(results_file);
+    results_file.close();
+  } catch (std::exception &e) {
+    std::cerr << "error: " << e.what() << "\n";
+    return 1;
+  } catch (...) {
+    std::cerr << "Exception of unknown type!\n";
+  }
+  return 0;
+}

@@ -1,0 +1,117 @@
+// anneal_ref.hip -- reference-semantics simulated annealing on the GPU.
+//
+// Restates the chain of sa::anneal (include/simulated_annealing/annealing.hpp:85-126)
+// with O(N) local fields instead of the O(N^2) energy recompute of :104:
+//   - one wavefront per replica; its 64 lanes share the replica's fields
+//     h_j = Q_jj + sum_{l!=j} Q_jl x_l (FP64, in LDS when they fit);
+//   - attempt t: Philox draw -> site s = floor(w0*N/2^32), u = w1*2^-32;
+//     dE = (1-2x_s) h_s; accept iff dE < 0 || exp(-dE/beta) > u
+//     (annealing.hpp:106-107 with E_cur - E_new = -dE);
+//   - an accepted flip updates all fields with row s of Q (coalesced,
+//     Q is symmetric), E += dE;
+//   - best state copied whenever E < E_best (annealing.hpp:115-121).
+#include "os_kernels.h"
+
+namespace os {
+
+template <bool kLdsFields>
+__global__ __launch_bounds__(64) void ref_anneal_kernel(int n, const double *__restrict__ q,
+                                                        const double *__restrict__ beta,
+                                                        uint64_t total_attempts, int spb,
+                                                        uint32_t k0, uint32_t k1,
+                                                        double *__restrict__ gfields,
+                                                        uint32_t *__restrict__ best_bits, int W,
+                                                        double *__restrict__ best_run) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t replica = blockIdx.x;
+  const int lane = threadIdx.x;
+  double *h = kLdsFields ? reinterpret_cast<double *>(smem) : gfields + (size_t)replica * n;
+  uint8_t *x = smem + (kLdsFields ? sizeof(double) * (size_t)n : 0);
+  uint8_t *best = x + n;
+
+  for (int j = lane; j < n; j += 64) {
+    const uint4 o = philox4x32_10(make_uint4((uint32_t)(j >> 7), replica, 0u, TAG_REF_INIT), k0, k1);
+    const uint8_t b = (uint8_t)((pick(o, (j >> 5) & 3) >> (j & 31)) & 1u);
+    x[j] = b;
+    best[j] = b;
+    h[j] = q[(size_t)j * n + j];
+  }
+  __syncthreads();
+  // h_j += sum_{l != j, x_l = 1} Q_lj (row l of the symmetric Q, coalesced)
+  for (int l = 0; l < n; ++l) {
+    if (x[l]) {
+      const double *row = q + (size_t)l * n;
+      for (int j = lane; j < n; j += 64)
+        if (j != l) h[j] += row[j];
+    }
+  }
+  __syncthreads();
+  double part = 0.0;
+  for (int j = lane; j < n; j += 64)
+    if (x[j]) part += 0.5 * (h[j] + q[(size_t)j * n + j]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  double e_cur = part;
+  double e_best = part;
+
+  for (uint64_t t = 0; t < total_attempts; ++t) {
+    const double b = beta[t / (uint64_t)spb];
+    const uint4 o =
+        philox4x32_10(make_uint4((uint32_t)t, replica, (uint32_t)(t >> 32), TAG_REF_STEP), k0, k1);
+    const int s = (int)__umulhi(o.x, (uint32_t)n);
+    const double u = (double)o.y * 0x1p-32;
+    const uint8_t xs = x[s];
+    const double hs = h[s];
+    const double de = xs ? -hs : hs;
+    const bool accept = (de < 0.0) || (exp(-de / b) > u);
+    if (accept) {
+      const double d = xs ? -1.0 : 1.0;
+      const double *row = q + (size_t)s * n;
+      for (int j = lane; j < n; j += 64)
+        if (j != s) h[j] = fma(row[j], d, h[j]);
+      if (lane == 0) x[s] = (uint8_t)(1 - xs);
+      e_cur += de;
+    }
+    __syncthreads();
+    if (e_cur < e_best) {
+      e_best = e_cur;
+      for (int j = lane; j < n; j += 64) best[j] = x[j];
+    }
+  }
+  __syncthreads();
+  uint32_t *dst = best_bits + (size_t)replica * W;
+  for (int w = lane; w < W; w += 64) {
+    uint32_t v = 0;
+    for (int bb = 0; bb < 32; ++bb) {
+      const int j = w * 32 + bb;
+      if (j < n && best[j]) v |= 1u << bb;
+    }
+    dst[w] = v;
+  }
+  if (lane == 0) best_run[replica] = e_best;
+}
+
+int ref_lds_limit() { return 4096; }
+
+size_t ref_lds_bytes(int n) {
+  return (n <= ref_lds_limit() ? sizeof(double) * (size_t)n : 0) + 2 * (size_t)n;
+}
+
+void launch_ref_anneal(hipStream_t st, int n, const double *q, const double *beta, int num_iter,
+                       int spb, uint32_t num_tries, uint64_t seed, double *gfields,
+                       uint32_t *best_bits, double *best_run) {
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const uint64_t total = (uint64_t)num_iter * (uint64_t)spb;
+  const int W = (n + 31) / 32;
+  const size_t lds = (ref_lds_bytes(n) + 15) & ~(size_t)15;
+  if (n <= ref_lds_limit()) {
+    ref_anneal_kernel<true><<<num_tries, 64, lds, st>>>(n, q, beta, total, spb, k0, k1, nullptr,
+                                                         best_bits, W, best_run);
+  } else {
+    ref_anneal_kernel<false><<<num_tries, 64, lds, st>>>(n, q, beta, total, spb, k0, k1, gfields,
+                                                          best_bits, W, best_run);
+  }
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace os

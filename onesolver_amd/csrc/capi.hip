@@ -1,0 +1,214 @@
+// capi.hip -- extern "C" entry points of libonesolver.so (declared in include/onesolver.h).
+//
+// Each entry point validates its arguments, runs on the context's stream and
+// converts every failure into a negative return code plus a thread-local
+// message, mirroring the reference's exception types (std::invalid_argument
+// for bad input, sycl::exception for device failures).
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/onesolver.h"
+#include "os_ctx.h"
+#include "os_kernels.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return OS_OK;
+  } catch (const os::Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "out of host memory";
+    return OS_ERR_INTERNAL;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return OS_ERR_INTERNAL;
+  }
+}
+}  // namespace
+
+namespace os {
+
+void DevBuf::ensure(size_t bytes) {
+  if (bytes <= cap) return;
+  release();
+  if (bytes == 0) return;
+  OS_HIP_CHECK(hipMalloc(&ptr, bytes));
+  cap = bytes;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+}
+
+Ctx::Ctx(int dev) : device(dev) {
+  int count = 0;
+  OS_HIP_CHECK(hipGetDeviceCount(&count));
+  OS_REQUIRE(dev >= 0 && dev < count, "device index out of range");
+  OS_HIP_CHECK(hipSetDevice(dev));
+  hipDeviceProp_t prop;
+  OS_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  name = prop.name;
+  arch = prop.gcnArchName;
+  num_cus = prop.multiProcessorCount;
+  OS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  OS_HIP_CHECK(hipEventCreate(&ev_start));
+  OS_HIP_CHECK(hipEventCreate(&ev_stop));
+}
+
+Ctx::~Ctx() {
+  (void)hipSetDevice(device);
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (auto *b : {&b_q, &b_beta, &b_bits, &b_run, &b_energy, &b_scratch, &b_fields, &b_bytes})
+    b->release();
+  if (ev_start) (void)hipEventDestroy(ev_start);
+  if (ev_stop) (void)hipEventDestroy(ev_stop);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Ctx::activate() const { OS_HIP_CHECK(hipSetDevice(device)); }
+
+void Ctx::sync() const { OS_HIP_CHECK(hipStreamSynchronize(stream)); }
+
+}  // namespace os
+
+using os::Ctx;
+
+extern "C" {
+
+int os_version(void) { return 10000; }
+
+const char *os_last_error(void) { return g_last_error.c_str(); }
+
+int os_device_count(int *count) {
+  return guarded([&] {
+    OS_REQUIRE(count != nullptr, "count is null");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e == hipErrorNoDevice) c = 0;
+    else OS_HIP_CHECK(e);
+    *count = c;
+  });
+}
+
+int os_ctx_create(int device, os_ctx **out) {
+  return guarded([&] {
+    OS_REQUIRE(out != nullptr, "out is null");
+    *out = reinterpret_cast<os_ctx *>(new Ctx(device));
+  });
+}
+
+int os_ctx_destroy(os_ctx *ctx) {
+  return guarded([&] { delete reinterpret_cast<Ctx *>(ctx); });
+}
+
+int os_ctx_device_name(os_ctx *ctx, char *buf, size_t len) {
+  return guarded([&] {
+    OS_REQUIRE(ctx && buf && len > 0, "invalid argument");
+    const std::string &nm = reinterpret_cast<Ctx *>(ctx)->name;
+    const size_t k = std::min(len - 1, nm.size());
+    std::memcpy(buf, nm.data(), k);
+    buf[k] = 0;
+  });
+}
+
+int os_energy_fp64(os_ctx *ctx_, int n, const double *q_sym, uint32_t count,
+                   const uint8_t *states, double *energies) {
+  return guarded([&] {
+    OS_REQUIRE(ctx_ != nullptr, "context is null");
+    OS_REQUIRE(n >= 1, "n must be positive");
+    OS_REQUIRE(q_sym && energies && (states || count == 0), "null array");
+    if (count == 0) return;
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    const int W = (n + 31) / 32;
+    const size_t qbytes = sizeof(double) * (size_t)n * n;
+    c.b_q.ensure(qbytes);
+    c.b_bytes.ensure((size_t)count * n);
+    c.b_bits.ensure(sizeof(uint32_t) * (size_t)count * W);
+    c.b_energy.ensure(sizeof(double) * count);
+    c.b_scratch.ensure(os::energy_scratch_bytes(n, count));
+    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_sym, qbytes, hipMemcpyHostToDevice, c.stream));
+    OS_HIP_CHECK(hipMemcpyAsync(c.b_bytes.ptr, states, (size_t)count * n, hipMemcpyHostToDevice,
+                                c.stream));
+    os::launch_pack_bytes(c.stream, c.b_bytes.as<uint8_t>(), n, count, c.b_bits.as<uint32_t>());
+    os::launch_energy_fp64(c.stream, n, c.b_q.as<double>(), count, c.b_bits.as<uint32_t>(),
+                           c.b_energy.as<double>(), c.b_scratch.as<double>(), nullptr, 0, nullptr);
+    OS_HIP_CHECK(hipMemcpyAsync(energies, c.b_energy.ptr, sizeof(double) * count,
+                                hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *beta,
+                        int num_iter, int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                        uint8_t *best_states, double *best_energies, uint8_t *best_state,
+                        double *best_energy) {
+  return guarded([&] {
+    OS_REQUIRE(ctx_ != nullptr, "context is null");
+    OS_REQUIRE(n >= 1, "n must be positive");
+    OS_REQUIRE(q_sym != nullptr, "q_sym is null");
+    OS_REQUIRE(num_iter >= 0, "num_iter must be non-negative");
+    OS_REQUIRE(num_iter == 0 || beta != nullptr, "beta is null");
+    OS_REQUIRE(sweeps_per_beta >= 1, "sweeps_per_beta must be positive");
+    OS_REQUIRE(num_tries >= 1, "num_tries must be positive");
+    if (n > 32768)
+      throw os::Error(OS_ERR_UNSUPPORTED, "reference-mode annealing supports n <= 32768");
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    const int W = (n + 31) / 32;
+    const size_t qbytes = sizeof(double) * (size_t)n * n;
+    c.b_q.ensure(qbytes);
+    c.b_beta.ensure(sizeof(double) * std::max(num_iter, 1));
+    c.b_bits.ensure(sizeof(uint32_t) * (size_t)num_tries * W);
+    c.b_run.ensure(sizeof(double) * num_tries);
+    c.b_energy.ensure(sizeof(double) * num_tries);
+    c.b_scratch.ensure(os::energy_scratch_bytes(n, num_tries));
+    if (n > os::ref_lds_limit()) c.b_fields.ensure(sizeof(double) * (size_t)num_tries * n);
+    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_sym, qbytes, hipMemcpyHostToDevice, c.stream));
+    if (num_iter > 0)
+      OS_HIP_CHECK(hipMemcpyAsync(c.b_beta.ptr, beta, sizeof(double) * num_iter,
+                                  hipMemcpyHostToDevice, c.stream));
+    os::launch_ref_anneal(c.stream, n, c.b_q.as<double>(), c.b_beta.as<double>(), num_iter,
+                          sweeps_per_beta, num_tries, seed,
+                          n > os::ref_lds_limit() ? c.b_fields.as<double>() : nullptr,
+                          c.b_bits.as<uint32_t>(), c.b_run.as<double>());
+    os::launch_energy_fp64(c.stream, n, c.b_q.as<double>(), (int)num_tries,
+                           c.b_bits.as<uint32_t>(), c.b_energy.as<double>(),
+                           c.b_scratch.as<double>(), nullptr, 0, nullptr);
+    std::vector<double> e(num_tries);
+    OS_HIP_CHECK(hipMemcpyAsync(e.data(), c.b_energy.ptr, sizeof(double) * num_tries,
+                                hipMemcpyDeviceToHost, c.stream));
+    std::vector<uint8_t> bytes;
+    if (best_states) {
+      c.b_bytes.ensure((size_t)num_tries * n);
+      os::launch_unpack_bits(c.stream, c.b_bits.as<uint32_t>(), n, num_tries,
+                             c.b_bytes.as<uint8_t>());
+      OS_HIP_CHECK(hipMemcpyAsync(best_states, c.b_bytes.ptr, (size_t)num_tries * n,
+                                  hipMemcpyDeviceToHost, c.stream));
+    }
+    c.sync();
+    // argmin over replicas, first index on ties (annealing.hpp:134-139)
+    const size_t best_idx = (size_t)(std::min_element(e.begin(), e.end()) - e.begin());
+    if (best_energies) std::memcpy(best_energies, e.data(), sizeof(double) * num_tries);
+    if (best_energy) *best_energy = e[best_idx];
+    if (best_state) {
+      std::vector<uint32_t> w(W);
+      OS_HIP_CHECK(hipMemcpy(w.data(), c.b_bits.as<uint32_t>() + best_idx * W,
+                             sizeof(uint32_t) * W, hipMemcpyDeviceToHost));
+      for (int j = 0; j < n; ++j) best_state[j] = (uint8_t)((w[j >> 5] >> (j & 31)) & 1u);
+    }
+  });
+}
+
+}  // extern "C"

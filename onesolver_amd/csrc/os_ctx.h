@@ -1,0 +1,41 @@
+// os_ctx.h -- the context behind the C ABI's os_ctx handle.
+#pragma once
+
+#include <string>
+
+#include "os_common.h"
+
+namespace os {
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  void ensure(size_t bytes);
+  void release();
+  template <typename T>
+  T *as() const {
+    return static_cast<T *>(ptr);
+  }
+};
+
+struct Ctx {
+  int device = 0;
+  std::string name;
+  std::string arch;
+  int num_cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr;
+  hipEvent_t ev_stop = nullptr;
+  // scratch buffers of the one-shot entry points
+  DevBuf b_q, b_beta, b_bits, b_run, b_energy, b_scratch, b_fields, b_bytes;
+
+  explicit Ctx(int dev);
+  ~Ctx();
+  Ctx(const Ctx &) = delete;
+  Ctx &operator=(const Ctx &) = delete;
+  void activate() const;
+  void sync() const;
+};
+
+}  // namespace os

@@ -1,0 +1,199 @@
+"""ctypes binding of the CPU oracle (``oracle/liboracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()``
+(as the checker) and ``bench.py``'s cpu_baseline leg; the product never imports
+it.  Every function mirrors one in ``oracle/oracle.h``, which cites the
+reference file:line each one restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.or_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.or_neg_ln_u.argtypes = [ctypes.c_uint32]
+        L.or_neg_ln_u.restype = ctypes.c_float
+        L.or_neg_ln_u_many.argtypes = [_u32p, ctypes.c_int64, _f32p]
+        L.or_flatten.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _f64p, ctypes.c_int,
+                                 _i32p, _i32p, _f64p, _f64p]
+        L.or_energy.argtypes = [_f64p, _u8p, ctypes.c_int]
+        L.or_energy.restype = ctypes.c_double
+        L.or_energy_many.argtypes = [_f64p, _u8p, ctypes.c_int, ctypes.c_int64, _f64p]
+        L.or_linear_schedule.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _f64p]
+        L.or_geometric_schedule.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_uint32, _f64p]
+        L.or_exhaustive_literal.argtypes = [ctypes.c_int, _f64p, _u8p, _f64p]
+        L.or_exhaustive.argtypes = [ctypes.c_int, _f64p, _u8p, _f64p, ctypes.c_int]
+        anneal_args = [ctypes.c_int, _f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                       ctypes.c_uint64, ctypes.c_uint32, _u8p, _f64p, ctypes.c_int]
+        L.or_anneal_literal.argtypes = anneal_args
+        L.or_anneal_fields.argtypes = anneal_args
+        L.or_sweep_dense.argtypes = [ctypes.c_int, _f64p, _f32p, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_uint64, ctypes.c_uint32, _u8p, _u8p, _u8p, _f64p,
+                                     ctypes.c_int]
+        L.or_sweep_csr.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, ctypes.c_int,
+                                   ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, _u8p, _u8p,
+                                   _u8p, _f64p, ctypes.c_int]
+        L.or_warm_init.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, ctypes.c_uint64,
+                                   ctypes.c_uint32, _u8p]
+        L.or_sweep_init.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                    ctypes.c_uint32, _u8p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().or_philox4x32_10(_p(c, _u32p), _p(k, _u32p), _p(o, _u32p))
+    return o
+
+
+def neg_ln_u(w: int) -> float:
+    return float(lib().or_neg_ln_u(ctypes.c_uint32(w)))
+
+
+def neg_ln_u_many(w):
+    ww = np.ascontiguousarray(w, dtype=np.uint32)
+    out = np.zeros(len(ww), dtype=np.float32)
+    lib().or_neg_ln_u_many(_p(ww, _u32p), len(ww), _p(out, _f32p))
+    return out
+
+
+def flatten(n, linear: dict, quadratic: dict):
+    li = np.array(list(linear.keys()), dtype=np.int32)
+    lv = np.array(list(linear.values()), dtype=np.float64)
+    qi = np.array([k[0] for k in quadratic], dtype=np.int32)
+    qj = np.array([k[1] for k in quadratic], dtype=np.int32)
+    qv = np.array(list(quadratic.values()), dtype=np.float64)
+    out = np.zeros(n * n, dtype=np.float64)
+    lib().or_flatten(n, len(li), _p(li, _i32p), _p(lv, _f64p), len(qi), _p(qi, _i32p),
+                     _p(qj, _i32p), _p(qv, _f64p), _p(out, _f64p))
+    return out
+
+
+def energy(flat, state):
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    st = np.ascontiguousarray(state, dtype=np.uint8)
+    n = st.shape[-1]
+    if st.ndim == 1:
+        return float(lib().or_energy(_p(flat, _f64p), _p(st, _u8p), n))
+    st = st.reshape(-1, n)
+    out = np.zeros(st.shape[0], dtype=np.float64)
+    lib().or_energy_many(_p(flat, _f64p), _p(st, _u8p), n, st.shape[0], _p(out, _f64p))
+    return out
+
+
+def linear_schedule(beta_min, beta_max, n):
+    out = np.zeros(n, dtype=np.float64)
+    lib().or_linear_schedule(beta_min, beta_max, n, _p(out, _f64p))
+    return out
+
+
+def geometric_schedule(beta_min, beta_max, n):
+    out = np.zeros(max(n, 1), dtype=np.float64)
+    lib().or_geometric_schedule(beta_min, beta_max, n, _p(out, _f64p))
+    return out[:n]
+
+
+def exhaustive(flat, n, literal=False, threads=0):
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    st = np.zeros(n, dtype=np.uint8)
+    e = np.zeros(1, dtype=np.float64)
+    if literal:
+        rc = lib().or_exhaustive_literal(n, _p(flat, _f64p), _p(st, _u8p), _p(e, _f64p))
+    else:
+        rc = lib().or_exhaustive(n, _p(flat, _f64p), _p(st, _u8p), _p(e, _f64p), threads)
+    if rc != 0:
+        raise ValueError("exhaustive: unsupported size")
+    return st, float(e[0])
+
+
+def anneal(flat, n, beta, num_tries, sweeps_per_beta=1, seed=1234, first_replica=0,
+           literal=False, threads=0):
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    bs = np.zeros((num_tries, n), dtype=np.uint8)
+    be = np.zeros(num_tries, dtype=np.float64)
+    fn = lib().or_anneal_literal if literal else lib().or_anneal_fields
+    fn(n, _p(flat, _f64p), _p(beta, _f64p), len(beta), sweeps_per_beta, num_tries, seed,
+       first_replica, _p(bs, _u8p), _p(be, _f64p), threads)
+    return bs, be
+
+
+def sweep_dense(flat, n, tscale, num_replicas, seed=1, instance=0, init=None, threads=0):
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    fin = np.zeros((num_replicas, n), dtype=np.uint8)
+    bs = np.zeros((num_replicas, n), dtype=np.uint8)
+    be = np.zeros(num_replicas, dtype=np.float64)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint8)
+    lib().or_sweep_dense(n, _p(flat, _f64p), _p(ts, _f32p), len(ts), num_replicas, seed, instance,
+                         _p(ini, _u8p), _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), threads)
+    return fin, bs, be
+
+
+def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0, init=None,
+              threads=0):
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    cl = np.ascontiguousarray(col, dtype=np.int32)
+    vl = np.ascontiguousarray(val, dtype=np.float32)
+    dg = np.ascontiguousarray(diag, dtype=np.float32)
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    n = len(dg)
+    fin = np.zeros((num_replicas, n), dtype=np.uint8)
+    bs = np.zeros((num_replicas, n), dtype=np.uint8)
+    be = np.zeros(num_replicas, dtype=np.float64)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint8)
+    lib().or_sweep_csr(n, _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f32p), _p(dg, _f32p),
+                       _p(ts, _f32p), len(ts), num_replicas, seed, instance, _p(ini, _u8p),
+                       _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), threads)
+    return fin, bs, be
+
+
+def warm_init(p, num_replicas, seed=1, instance=0):
+    pp = np.ascontiguousarray(p, dtype=np.float32)
+    st = np.zeros((num_replicas, len(pp)), dtype=np.uint8)
+    lib().or_warm_init(len(pp), _p(pp, _f32p), num_replicas, seed, instance, _p(st, _u8p))
+    return st
+
+
+def sweep_init(n, num_replicas, seed=1, instance=0):
+    st = np.zeros((num_replicas, n), dtype=np.uint8)
+    lib().or_sweep_init(n, num_replicas, seed, instance, _p(st, _u8p))
+    return st
+
+
+def csr_energy(row_ptr, col, val, diag, states):
+    """FP64 energy of CSR instances: sum_j x_j diag_j + sum_{j<l} w_jl x_j x_l."""
+    st = np.asarray(states, dtype=np.float64)
+    n = len(diag)
+    rows = np.repeat(np.arange(n), np.diff(row_ptr))
+    upper = col > rows
+    st2 = st.reshape(-1, n)
+    e = st2 @ np.asarray(diag, dtype=np.float64)
+    e += (st2[:, rows[upper]] * st2[:, col[upper]]) @ np.asarray(val, dtype=np.float64)[upper]
+    return e if st.ndim > 1 else float(e[0])

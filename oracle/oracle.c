@@ -1,0 +1,605 @@
+/*
+ * oracle.c -- CPU restatement of the oneSolver simulated-annealing QUBO path.
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() (as the
+ * checker) and bench.py's cpu_baseline leg.  The product never links it.
+ * Compiled with -ffp-contract=off so that every floating-point expression is
+ * evaluated exactly as written (explicit fma() where the specification
+ * says fma).
+ *
+ * Reference files restated (paths relative to the oneSolver tree):
+ *   include/simulated_annealing/annealing.hpp      sa::energy, sa::anneal
+ *   include/simulated_annealing/device_rng.hpp     bit / bit_index / uniform
+ *   include/helpers/qubo_helpers.hpp               flatten_qubo
+ *   include/exhaustive/exhaustive.hpp              exhaustive::solve
+ *   app/one-solver-anneal.cpp                      beta schedules
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11).  Multipliers and Weyl key increments are the    */
+/* published constants.                                                     */
+/* ------------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int round = 0; round < 10; ++round) {
+    if (round > 0) {
+      k0 += PHILOX_W0;
+      k1 += PHILOX_W1;
+    }
+    uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c0 = hi1 ^ c1 ^ k0;
+    c1 = lo1;
+    c2 = hi0 ^ c3 ^ k1;
+    c3 = lo0;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}
+
+/* Stream tags (counter word 3, low byte); instance id in the upper 24 bits. */
+enum {
+  TAG_REF_INIT = 0x01,
+  TAG_REF_STEP = 0x02,
+  TAG_SWEEP_INIT = 0x10,
+  TAG_SWEEP_THR = 0x11,
+  TAG_WARM = 0x12,
+};
+
+static void seed_key(uint64_t seed, uint32_t key[2]) {
+  key[0] = (uint32_t)seed;
+  key[1] = (uint32_t)(seed >> 32);
+}
+
+/* -ln(u) for u = (2*(w>>9)+1) * 2^-24, the open-interval 24-bit uniform of a
+ * 32-bit word.  u = m * 2^e with m in [sqrt(1/2), sqrt(2)); ln(1+f) with
+ * f = m-1 is f + f^2 * P(f), P a degree-6 least-squares fit in relative error
+ * on that interval (max rel. error 2.2e-7 in fp32 evaluation).  Every step
+ * is exact (frexp, *2, m-1) or a single correctly rounded fp32 operation, so
+ * the value is reproducible bit for bit on any IEEE-754 implementation.      */
+static const float LNP_C0 = -0x1.000080p-1f;
+static const float LNP_C1 = 0x1.5557dep-2f;
+static const float LNP_C2 = -0x1.ff5ac2p-3f;
+static const float LNP_C3 = 0x1.97f830p-3f;
+static const float LNP_C4 = -0x1.621a72p-3f;
+static const float LNP_C5 = 0x1.4953eep-3f;
+static const float LNP_C6 = -0x1.9190e0p-4f;
+static const float LN2_F = 0x1.62e430p-1f;
+static const float SQRT_HALF_F = 0x1.6a09e6p-1f;
+
+float or_neg_ln_u(uint32_t w) {
+  float u = (float)((w >> 9) * 2u + 1u) * 0x1p-24f;
+  int e;
+  float m = frexpf(u, &e);
+  if (m < SQRT_HALF_F) {
+    m = m + m;
+    e = e - 1;
+  }
+  float f = m - 1.0f;
+  float p = LNP_C6;
+  p = fmaf(p, f, LNP_C5);
+  p = fmaf(p, f, LNP_C4);
+  p = fmaf(p, f, LNP_C3);
+  p = fmaf(p, f, LNP_C2);
+  p = fmaf(p, f, LNP_C1);
+  p = fmaf(p, f, LNP_C0);
+  float f2 = f * f;
+  float l = fmaf(f2, p, f);
+  l = fmaf((float)e, LN2_F, l);
+  return -l;
+}
+
+void or_neg_ln_u_many(const uint32_t *w, int64_t count, float *out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < count; ++i) out[i] = or_neg_ln_u(w[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Model helpers                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* include/helpers/qubo_helpers.hpp:26-44.  The model is given as the
+ * contents of its two maps (unique keys).  Diagonal entries take
+ * get_variable(i); every ordered pair (i,j), i!=j, adds get_connection(i,j)
+ * to both [i + j*n] and [j + i*n], exactly as the reference loop does.       */
+static double lookup_lin(int nlin, const int *idx, const double *val, int i) {
+  for (int k = 0; k < nlin; ++k)
+    if (idx[k] == i) return val[k];
+  return 0.0;
+}
+
+void or_flatten(int n, int nlin, const int *lin_idx, const double *lin_val, int nquad,
+                const int *q_i, const int *q_j, const double *q_val, double *out) {
+  memset(out, 0, sizeof(double) * (size_t)n * (size_t)n);
+  /* get_connection lookups: build a dense table of stored (i,j) keys. */
+  double *conn = (double *)calloc((size_t)n * (size_t)n, sizeof(double));
+  for (int k = 0; k < nquad; ++k) {
+    if (q_i[k] >= 0 && q_i[k] < n && q_j[k] >= 0 && q_j[k] < n)
+      conn[(size_t)q_i[k] * n + q_j[k]] = q_val[k];
+  }
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
+      if (i == j) {
+        out[(size_t)i + (size_t)j * n] = lookup_lin(nlin, lin_idx, lin_val, i);
+      } else {
+        double c = conn[(size_t)i * n + j];
+        out[(size_t)i + (size_t)j * n] += c;
+        out[(size_t)j + (size_t)i * n] += c;
+      }
+    }
+  }
+  free(conn);
+}
+
+/* include/simulated_annealing/annealing.hpp:31-40: i ascending, j >= i
+ * ascending, result += F[i*N+j] * x_i * x_j, in double.                     */
+double or_energy(const double *flat, const uint8_t *state, int n) {
+  double result = 0.0;
+  for (int i = 0; i < n; i++) {
+    for (int j = i; j < n; j++) {
+      result += flat[(size_t)i * n + j] * (double)state[i] * (double)state[j];
+    }
+  }
+  return result;
+}
+
+void or_energy_many(const double *flat, const uint8_t *states, int n, int64_t count,
+                    double *out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < count; ++r) out[r] = or_energy(flat, states + (size_t)r * n, n);
+}
+
+/* app/one-solver-anneal.cpp:23-29.  beta_i = beta_min + beta_max*i/(n-1):
+ * the last value is beta_min+beta_max, and n == 1 gives 0/0 = NaN.          */
+void or_linear_schedule(double beta_min, double beta_max, uint32_t n, double *out) {
+  for (uint32_t i = 0; i < n; i++) {
+    out[i] = beta_min + beta_max * (double)i / (double)(uint32_t)(n - 1u);
+  }
+}
+
+/* app/one-solver-anneal.cpp:31-39.  Cumulative product in double. */
+void or_geometric_schedule(double beta_min, double beta_max, uint32_t n, double *out) {
+  if (n == 0) return;
+  out[0] = beta_min;
+  double alpha = pow(beta_max / beta_min, 1.0 / (double)(uint32_t)(n - 1u));
+  for (uint32_t i = 1; i < n; i++) out[i] = out[i - 1] * alpha;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Exhaustive search                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* include/exhaustive/exhaustive.hpp:94-138 (per-state energy over the upper
+ * triangle, strict '<', lowest index wins) and :158-166 (argmin).          */
+int or_exhaustive_literal(int n, const double *flat, uint8_t *state, double *energy) {
+  if (n < 1 || n > 24) return -1;
+  uint64_t total = 1ull << n;
+  double e_best = INFINITY;
+  uint64_t s_best = 0;
+  for (uint64_t s = 0; s < total; ++s) {
+    double e = 0.0;
+    for (int i = 0; i < n; ++i) {
+      for (int j = i; j < n; ++j) {
+        if (i == j) {
+          if ((s >> i) & 1) e += flat[(size_t)i * n + i];
+        } else {
+          if (((s >> i) & 1) && ((s >> j) & 1)) e += flat[(size_t)i * n + j];
+        }
+      }
+    }
+    if (e < e_best) {
+      e_best = e;
+      s_best = s;
+    }
+  }
+  for (int i = 0; i < n; ++i) state[i] = (uint8_t)((s_best >> i) & 1);
+  *energy = e_best;
+  return 0;
+}
+
+#define EXH_KEEP 16
+
+typedef struct {
+  double e;
+  uint64_t s;
+} cand_t;
+
+static void cand_insert(cand_t *c, int *nc, double e, uint64_t s) {
+  if (*nc == EXH_KEEP && e >= c[EXH_KEEP - 1].e) return;
+  int k = (*nc < EXH_KEEP) ? (*nc)++ : EXH_KEEP - 1;
+  while (k > 0 && c[k - 1].e > e) {
+    c[k] = c[k - 1];
+    --k;
+  }
+  c[k].e = e;
+  c[k].s = s;
+}
+
+static double energy_bits(const double *flat, uint64_t s, int n) {
+  double e = 0.0;
+  for (int i = 0; i < n; ++i) {
+    for (int j = i; j < n; ++j) {
+      if (i == j) {
+        if ((s >> i) & 1) e += flat[(size_t)i * n + i];
+      } else if (((s >> i) & 1) && ((s >> j) & 1)) {
+        e += flat[(size_t)i * n + j];
+      }
+    }
+  }
+  return e;
+}
+
+/* Gray-code enumeration with O(N) incremental energies.  The EXH_KEEP lowest
+ * incremental energies of every chunk are re-ranked with the literal energy
+ * (same summation order as the reference kernel) so that rounding of the
+ * incremental sums cannot change the argmin or its tie order.               */
+int or_exhaustive(int n, const double *flat, uint8_t *state, double *energy, int threads) {
+  if (n < 1 || n > 40) return -1;
+  uint64_t total = 1ull << n;
+  int nchunks = 1;
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+  nchunks = threads * 8;
+#else
+  (void)threads;
+#endif
+  if ((uint64_t)nchunks > total) nchunks = (int)total;
+  cand_t *all = (cand_t *)malloc(sizeof(cand_t) * EXH_KEEP * (size_t)nchunks);
+  int *counts = (int *)calloc((size_t)nchunks, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int c = 0; c < nchunks; ++c) {
+    uint64_t rem = total % (uint64_t)nchunks, uc = (uint64_t)c;
+    uint64_t g0 = total / (uint64_t)nchunks * uc + (uc < rem ? uc : rem);
+    uint64_t g1 = g0 + total / (uint64_t)nchunks + (uc < rem ? 1u : 0u);
+    cand_t *cand = all + (size_t)c * EXH_KEEP;
+    int nc = 0;
+    double *h = (double *)malloc(sizeof(double) * n);
+    uint64_t s = g0 ^ (g0 >> 1);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && ((s >> l) & 1)) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    double e = energy_bits(flat, s, n);
+    cand_insert(cand, &nc, e, s);
+    for (uint64_t g = g0 + 1; g < g1; ++g) {
+      int b = __builtin_ctzll(g);
+      int xb = (int)((s >> b) & 1);
+      e += xb ? -h[b] : h[b];
+      double d = xb ? -1.0 : 1.0;
+      s ^= 1ull << b;
+      for (int l = 0; l < n; ++l)
+        if (l != b) h[l] += flat[(size_t)l * n + b] * d;
+      cand_insert(cand, &nc, e, s);
+    }
+    counts[c] = nc;
+    free(h);
+  }
+  double e_best = INFINITY;
+  uint64_t s_best = 0;
+  double e_min_inc = INFINITY;
+  for (int c = 0; c < nchunks; ++c)
+    if (counts[c] > 0 && all[(size_t)c * EXH_KEEP].e < e_min_inc) e_min_inc = all[(size_t)c * EXH_KEEP].e;
+  double tol = 1e-9 * (fabs(e_min_inc) + 1.0);
+  for (int c = 0; c < nchunks; ++c) {
+    for (int k = 0; k < counts[c]; ++k) {
+      cand_t cd = all[(size_t)c * EXH_KEEP + k];
+      if (cd.e > e_min_inc + tol) continue;
+      double ex = energy_bits(flat, cd.s, n);
+      if (ex < e_best || (ex == e_best && cd.s < s_best)) {
+        e_best = ex;
+        s_best = cd.s;
+      }
+    }
+  }
+  for (int i = 0; i < n; ++i) state[i] = (uint8_t)((s_best >> i) & 1);
+  *energy = e_best;
+  free(all);
+  free(counts);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Reference-mode annealer                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* device_rng.hpp:46 bit(): spin j of replica i from counter
+ * (j>>7, i, 0, TAG_REF_INIT), word (j>>5)&3, bit j&31.                     */
+static void ref_init_bits(int n, uint32_t replica, const uint32_t key[2], uint8_t *x) {
+  uint32_t out[4];
+  for (int j = 0; j < n; ++j) {
+    if ((j & 127) == 0) {
+      uint32_t ctr[4] = {(uint32_t)(j >> 7), replica, 0u, TAG_REF_INIT};
+      or_philox4x32_10(ctr, key, out);
+    }
+    x[j] = (uint8_t)((out[(j >> 5) & 3] >> (j & 31)) & 1u);
+  }
+}
+
+/* device_rng.hpp:54-56 bit_index() and :62 uniform(): attempt t of replica
+ * i uses counter (t_lo, i, t_hi, TAG_REF_STEP); site = floor(w0*N/2^32),
+ * u = w1 * 2^-32 in [0,1).                                                  */
+static void ref_attempt(uint64_t t, uint32_t replica, const uint32_t key[2], int n, int *site,
+                        double *u) {
+  uint32_t ctr[4] = {(uint32_t)t, replica, (uint32_t)(t >> 32), TAG_REF_STEP};
+  uint32_t out[4];
+  or_philox4x32_10(ctr, key, out);
+  *site = (int)(((uint64_t)out[0] * (uint64_t)(uint32_t)n) >> 32);
+  *u = (double)out[1] * 0x1p-32;
+}
+
+/* annealing.hpp:85-126, one work-item per replica, literal O(N^2) energy
+ * recompute after every proposed flip (:104).                               */
+void or_anneal_literal(int n, const double *flat, const double *beta, int num_iter,
+                       int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                       uint32_t first_replica, uint8_t *best_states, double *best_energies,
+                       int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int64_t ii = 0; ii < (int64_t)num_tries; ++ii) {
+    uint32_t i = first_replica + (uint32_t)ii;
+    uint8_t *x = (uint8_t *)malloc((size_t)n);
+    uint8_t *best = best_states + (size_t)ii * n;
+    ref_init_bits(n, i, key, x);
+    memcpy(best, x, (size_t)n);
+    double best_energy = or_energy(flat, x, n);
+    double current_energy = best_energy;
+    uint64_t t = 0;
+    for (int iter = 0; iter < num_iter; iter++) {
+      double b = beta[iter];
+      for (int sweep = 0; sweep < sweeps_per_beta; sweep++) {
+        int s;
+        double u;
+        ref_attempt(t++, i, key, n, &s, &u);
+        x[s] = (uint8_t)(1 - x[s]);
+        double new_energy = or_energy(flat, x, n);
+        if ((new_energy < current_energy) || (exp((current_energy - new_energy) / b) > u)) {
+          current_energy = new_energy;
+        } else {
+          x[s] = (uint8_t)(1 - x[s]);
+        }
+        if (current_energy < best_energy) {
+          best_energy = current_energy;
+          memcpy(best, x, (size_t)n);
+        }
+      }
+    }
+    best_energies[ii] = best_energy;
+    free(x);
+  }
+}
+
+/* Same Markov chain as or_anneal_literal with local fields
+ * h_j = F_jj + sum_{l!=j} F_jl x_l, dE = (1-2x_s) h_s, E tracked by
+ * E += dE; the acceptance test is annealing.hpp:106-107 with
+ * E_cur - E_new = -dE.                                                      */
+void or_anneal_fields(int n, const double *flat, const double *beta, int num_iter,
+                      int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                      uint32_t first_replica, uint8_t *best_states, double *best_energies,
+                      int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int64_t ii = 0; ii < (int64_t)num_tries; ++ii) {
+    uint32_t i = first_replica + (uint32_t)ii;
+    uint8_t *x = (uint8_t *)malloc((size_t)n);
+    double *h = (double *)malloc(sizeof(double) * (size_t)n);
+    uint8_t *best = best_states + (size_t)ii * n;
+    ref_init_bits(n, i, key, x);
+    memcpy(best, x, (size_t)n);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && x[l]) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    double best_energy = or_energy(flat, x, n);
+    double current_energy = best_energy;
+    uint64_t t = 0;
+    for (int iter = 0; iter < num_iter; iter++) {
+      double b = beta[iter];
+      for (int sweep = 0; sweep < sweeps_per_beta; sweep++) {
+        int s;
+        double u;
+        ref_attempt(t++, i, key, n, &s, &u);
+        double de = x[s] ? -h[s] : h[s];
+        if ((de < 0.0) || (exp(-de / b) > u)) {
+          double d = x[s] ? -1.0 : 1.0;
+          x[s] = (uint8_t)(1 - x[s]);
+          for (int l = 0; l < n; ++l)
+            if (l != s) h[l] += flat[(size_t)l * n + s] * d;
+          current_energy += de;
+        }
+        if (current_energy < best_energy) {
+          best_energy = current_energy;
+          memcpy(best, x, (size_t)n);
+        }
+      }
+    }
+    best_energies[ii] = best_energy;
+    free(h);
+    free(x);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sweep-mode annealers                                                     */
+/* ------------------------------------------------------------------------ */
+
+static void sweep_init_bits(int n, uint32_t replica, uint32_t instance, const uint32_t key[2],
+                            uint8_t *x) {
+  uint32_t out[4];
+  for (int j = 0; j < n; ++j) {
+    if ((j & 127) == 0) {
+      uint32_t ctr[4] = {(uint32_t)(j >> 7), replica, 0u, (instance << 8) | TAG_SWEEP_INIT};
+      or_philox4x32_10(ctr, key, out);
+    }
+    x[j] = (uint8_t)((out[(j >> 5) & 3] >> (j & 31)) & 1u);
+  }
+}
+
+void or_sweep_init(int n, uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                   uint8_t *states) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  for (uint32_t r = 0; r < num_replicas; ++r)
+    sweep_init_bits(n, r, instance, key, states + (size_t)r * n);
+}
+
+void or_warm_init(int n, const float *p, uint32_t num_replicas, uint64_t seed,
+                  uint32_t instance, uint8_t *states) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  uint32_t out[4];
+  for (uint32_t r = 0; r < num_replicas; ++r) {
+    for (int j = 0; j < n; ++j) {
+      if ((j & 3) == 0) {
+        uint32_t ctr[4] = {(uint32_t)(j >> 2), r, 0u, (instance << 8) | TAG_WARM};
+        or_philox4x32_10(ctr, key, out);
+      }
+      float u = (float)(out[j & 3] >> 8) * 0x1p-24f;
+      states[(size_t)r * n + j] = (uint8_t)(u < p[j] ? 1 : 0);
+    }
+  }
+}
+
+/* Threshold words of sweep t for replica r: counter (j>>2, r, t, tag). */
+static void sweep_thr_words(uint32_t j4, uint32_t r, uint32_t t, uint32_t instance,
+                            const uint32_t key[2], uint32_t out[4]) {
+  uint32_t ctr[4] = {j4, r, t, (instance << 8) | TAG_SWEEP_THR};
+  or_philox4x32_10(ctr, key, out);
+}
+
+void or_sweep_dense(int n, const double *flat, const float *tscale, int num_sweeps,
+                    uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                    const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
+                    double *best_energies, int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int64_t rr = 0; rr < (int64_t)num_replicas; ++rr) {
+    uint32_t r = (uint32_t)rr;
+    uint8_t *x = (uint8_t *)malloc((size_t)n);
+    double *h = (double *)malloc(sizeof(double) * (size_t)n);
+    uint8_t *best = best_states + (size_t)r * n;
+    if (init)
+      memcpy(x, init + (size_t)r * n, (size_t)n);
+    else
+      sweep_init_bits(n, r, instance, key, x);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && x[l]) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    double e = or_energy(flat, x, n);
+    double e_best = e;
+    memcpy(best, x, (size_t)n);
+    uint32_t w[4];
+    for (int t = 0; t < num_sweeps; ++t) {
+      float ts = tscale[t];
+      for (int j = 0; j < n; ++j) {
+        if ((j & 3) == 0) sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
+        float thr = or_neg_ln_u(w[j & 3]) * ts;
+        double de = x[j] ? -h[j] : h[j];
+        if (de < (double)thr) {
+          double d = x[j] ? -1.0 : 1.0;
+          x[j] = (uint8_t)(1 - x[j]);
+          for (int l = 0; l < n; ++l)
+            if (l != j) h[l] += flat[(size_t)l * n + j] * d;
+          e += de;
+        }
+      }
+      if (e < e_best) {
+        e_best = e;
+        memcpy(best, x, (size_t)n);
+      }
+    }
+    if (final_states) memcpy(final_states + (size_t)r * n, x, (size_t)n);
+    best_energies[r] = e_best;
+    free(h);
+    free(x);
+  }
+}
+
+/* Energy of a CSR instance in the fixed order used by the GPU kernels:
+ * E = sum_j x_j * (diag_j + 0.5 * sum_{k in row j} val_k x_col_k), double. */
+static double csr_energy(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                         const float *diag, const uint8_t *x) {
+  double e = 0.0;
+  for (int j = 0; j < n; ++j) {
+    if (!x[j]) continue;
+    double off = 0.0;
+    for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
+      if (x[col[k]]) off += (double)val[k];
+    e += (double)diag[j] + 0.5 * off;
+  }
+  return e;
+}
+
+void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                  const float *diag, const float *tscale, int num_sweeps,
+                  uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                  const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
+                  double *best_energies, int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int64_t rr = 0; rr < (int64_t)num_replicas; ++rr) {
+    uint32_t r = (uint32_t)rr;
+    uint8_t *x = (uint8_t *)malloc((size_t)n);
+    uint8_t *best = best_states + (size_t)r * n;
+    if (init)
+      memcpy(x, init + (size_t)r * n, (size_t)n);
+    else
+      sweep_init_bits(n, r, instance, key, x);
+    double e = csr_energy(n, row_ptr, col, val, diag, x);
+    double e_best = e;
+    memcpy(best, x, (size_t)n);
+    uint32_t w[4];
+    for (int t = 0; t < num_sweeps; ++t) {
+      float ts = tscale[t];
+      for (int j = 0; j < n; ++j) {
+        if ((j & 3) == 0) sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
+        float thr = or_neg_ln_u(w[j & 3]) * ts;
+        float h = diag[j];
+        for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
+          if (x[col[k]]) h = h + val[k];
+        float de = x[j] ? -h : h;
+        if (de < thr) {
+          x[j] = (uint8_t)(1 - x[j]);
+          e += (double)de;
+        }
+      }
+      if (e < e_best) {
+        e_best = e;
+        memcpy(best, x, (size_t)n);
+      }
+    }
+    if (final_states) memcpy(final_states + (size_t)r * n, x, (size_t)n);
+    best_energies[r] = e_best;
+    free(x);
+  }
+}

@@ -1,0 +1,112 @@
+/*
+ * oracle.h -- CPU restatement of the oneSolver simulated-annealing QUBO path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (onesolver_amd/, the
+ * one-solver-anneal CLI, the C-ABI library) links, loads or calls this code.
+ * It is used by tests/, by __graft_entry__.smoke() as the checker, and by
+ * bench.py's cpu_baseline leg as the timed CPU baseline.
+ *
+ * Every function cites the reference file:line it restates.  Where the
+ * reference's behaviour depends on an unavailable third-party library
+ * (oneMKL device RNG, see include/simulated_annealing/device_rng.hpp:26-30),
+ * the oracle uses the counter-based Philox4x32-10 stream layout that the
+ * product uses, so that trajectories can be compared bit for bit; the law of
+ * each replica's Markov chain is the reference's.
+ *
+ * Parity pinning (how each function is checked before it is trusted):
+ *   or_philox4x32_10   Random123 / SC'11 known-answer vectors, and rocRAND's
+ *                      philox4x32_10 header compiled on the host.
+ *   or_neg_ln_u        libm log over the whole 2^23-point input grid.
+ *   or_energy          reference KATs tests/exhaustive_test.cpp:12-142 via
+ *                      or_exhaustive_*; published CSV rows of
+ *                      benchmarks/annealing/results/128/001.csv.
+ *   or_flatten         tests/qubo_helpers_test.cpp:14-27 (25 expected values).
+ *   or_*_schedule      closed forms (no reference test exists for them).
+ *   or_exhaustive_*    reference KATs (-12, -1.2, -22, -14, -32) and brute
+ *                      force cross-check literal vs Gray code.
+ *   or_anneal_literal  invariants (returned energy == or_energy of returned
+ *                      state; >= exhaustive ground energy), bit-identical to
+ *                      or_anneal_fields on integer instances, hit-rate
+ *                      against the exhaustive ground state.
+ *   or_sweep_dense /   exact greedy limit (beta -> inf reduces to
+ *   or_sweep_csr       deterministic descent, checked against a brute-force
+ *                      local-minimum test), detailed-balance check of the
+ *                      visited-state distribution on N=4 at fixed beta
+ *                      against the Boltzmann law.
+ *   or_mpnn_*          finite differences of the FP64 loss.
+ */
+#ifndef ONESOLVER_ORACLE_H_
+#define ONESOLVER_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG specification shared with the product (see oracle.c) ---------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float or_neg_ln_u(uint32_t w);
+void or_neg_ln_u_many(const uint32_t *w, int64_t count, float *out);
+
+/* ---- model helpers ------------------------------------------------------- */
+/* include/helpers/qubo_helpers.hpp:26-44 */
+void or_flatten(int n, int nlin, const int *lin_idx, const double *lin_val,
+                int nquad, const int *q_i, const int *q_j, const double *q_val,
+                double *out);
+/* include/simulated_annealing/annealing.hpp:31-40 */
+double or_energy(const double *flat, const uint8_t *state, int n);
+void or_energy_many(const double *flat, const uint8_t *states, int n,
+                    int64_t count, double *out);
+/* app/one-solver-anneal.cpp:23-29 and :31-39 */
+void or_linear_schedule(double beta_min, double beta_max, uint32_t n, double *out);
+void or_geometric_schedule(double beta_min, double beta_max, uint32_t n, double *out);
+
+/* ---- exhaustive search (include/exhaustive/exhaustive.hpp:29-167) -------- */
+/* Literal O(N^2)/state restatement, N <= 24. Returns 0 on success. */
+int or_exhaustive_literal(int n, const double *flat, uint8_t *state, double *energy);
+/* Gray-code O(N)/state with exact re-ranking of near-ties, N <= 40. */
+int or_exhaustive(int n, const double *flat, uint8_t *state, double *energy, int threads);
+
+/* ---- reference-mode annealer (include/simulated_annealing/annealing.hpp:55-140) */
+/* Per replica i in [first_replica, first_replica+num_tries): init bits, then
+ * num_iter*sweeps_per_beta random-site attempts with acceptance
+ * E_new < E_cur || exp((E_cur-E_new)/beta) > u.
+ * literal: O(N^2) energy recompute per attempt (annealing.hpp:104).
+ * fields : same chain with O(N) incremental local fields.                 */
+void or_anneal_literal(int n, const double *flat, const double *beta, int num_iter,
+                       int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                       uint32_t first_replica, uint8_t *best_states,
+                       double *best_energies, int threads);
+void or_anneal_fields(int n, const double *flat, const double *beta, int num_iter,
+                      int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
+                      uint32_t first_replica, uint8_t *best_states,
+                      double *best_energies, int threads);
+
+/* ---- sweep-mode annealers (configs with "sweeps": sequential site order) -- */
+/* tscale[t]: threshold scale of sweep t (1/beta_t for inverse-temperature
+ * schedules, beta_t when beta is used as a temperature as in
+ * annealing.hpp:107).  A spin flips iff dE < neg_ln_u(w) * tscale[t].
+ * init: NULL -> Philox random bits; else init[r*n + j] gives the start state.
+ * Best state is tracked at sweep boundaries (and at the start).            */
+void or_sweep_dense(int n, const double *flat, const float *tscale, int num_sweeps,
+                    uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                    const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
+                    double *best_energies, int threads);
+/* CSR over the off-diagonal couplings (symmetric), fp32 values, fp32 diag. */
+void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                  const float *diag, const float *tscale, int num_sweeps,
+                  uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                  const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
+                  double *best_energies, int threads);
+/* Warm-start initial state from marginals p[j]: x_j = (u < p_j). */
+void or_warm_init(int n, const float *p, uint32_t num_replicas, uint64_t seed,
+                  uint32_t instance, uint8_t *states);
+/* Random initial bits of sweep mode (shared by dense and CSR kernels). */
+void or_sweep_init(int n, uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                   uint8_t *states);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""GPU parity of the reference-semantics annealer and the FP64 energy kernel.
+
+The GPU chain draws from the same Philox counters as the oracle, so its
+trajectories are compared with or_anneal_fields state for state; the law of
+the chain is compared with the literal restatement of sa::anneal on
+independent streams (hit rate against the exhaustive ground state).
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import onesolver_amd
+import oracle
+from conftest import GOLDEN
+from qubo_io import load_qubo
+
+pytestmark = pytest.mark.gpu
+
+
+def _kats():
+    with open(os.path.join(GOLDEN, "kat", "expected.json")) as f:
+        return [(os.path.join(GOLDEN, "kat", e["file"]), e["energy"]) for e in json.load(f)]
+
+
+@pytest.mark.parametrize("n,count", [(1, 3), (5, 10), (28, 1000), (100, 300), (1000, 70)])
+def test_energy_fp64_matches_oracle(ctx, n, count):
+    rng = np.random.default_rng(n)
+    q = rng.uniform(-1, 1, (n, n))
+    q = (q + q.T) / 2
+    st = rng.integers(0, 2, (count, n)).astype(np.uint8)
+    got = ctx.energy_fp64(q, st)
+    ref = oracle.energy(q, st)
+    scale = np.abs(q).sum() + 1.0
+    assert np.all(np.abs(got - ref) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("path,ground", _kats())
+def test_reference_anneal_matches_oracle_chain_on_kats(ctx, path, ground):
+    q = load_qubo(path)
+    n = q.shape[0]
+    beta = oracle.geometric_schedule(0.1, 10.0, 500)
+    print(f"[gpu] reference anneal {os.path.basename(path)} n={n}", flush=True)
+    best, be, states, energies = ctx.anneal_reference(q, beta, 128, seed=1234)
+    s_or, e_or = oracle.anneal(q, n, beta, 128, seed=1234)
+    assert (states == s_or).all()
+    assert np.allclose(energies, oracle.energy(q, states), rtol=0, atol=1e-12)
+    assert np.allclose(energies, e_or, rtol=0, atol=1e-12)
+    assert abs(be - ground) < 1e-12  # every KAT is solved
+    i = int(np.argmin(energies))
+    assert (best == states[i]).all()
+
+
+def test_reference_anneal_config1_chain_and_hit_rate(ctx):
+    """Config 1: N=28 dense, geometric 0.1->1.0, 1000 iterations, 1024 tries."""
+    q = load_qubo(os.path.join(GOLDEN, "c1_dense28.qubo"))
+    with open(os.path.join(GOLDEN, "c1_ground.json")) as f:
+        ground = json.load(f)["energy"]
+    beta = oracle.geometric_schedule(0.1, 1.0, 1000)
+    best, be, states, energies = ctx.anneal_reference(q, beta, 1024, seed=1234)
+    # (1) FP64 energies of every returned state equal the reference energy function
+    ref_e = oracle.energy(q, states)
+    assert np.all(np.abs(energies - ref_e) <= 1e-9 * np.abs(ref_e))
+    # (2) same chains as the oracle on the same streams
+    s_or, _ = oracle.anneal(q, 28, beta, 1024, seed=1234)
+    assert (states == s_or).all(axis=1).mean() >= 0.999
+    # (3) law: per-replica ground-state hit rate vs the literal restatement on
+    #     independent streams (replicas 4096.. of the same seed), binomial CI
+    s_lit, e_lit = oracle.anneal(q, 28, beta, 1024, seed=1234, first_replica=4096, literal=True)
+    tol = 1e-9 * abs(ground)
+    k_gpu = int(np.sum(energies <= ground + tol))
+    k_ref = int(np.sum(e_lit <= ground + tol))
+    p = (k_gpu + k_ref) / 2048
+    se = np.sqrt(p * (1 - p) * (2 / 1024)) + 1e-12
+    print(f"[gpu] C1 hit rate gpu={k_gpu}/1024 ref={k_ref}/1024", flush=True)
+    assert abs(k_gpu - k_ref) / 1024 <= 1.96 * se + 1.0 / 1024
+    assert be >= ground - tol
+
+
+def test_reference_anneal_large_n_uses_global_fields(ctx):
+    rng = np.random.default_rng(5)
+    n = 4100  # above the LDS-resident field limit
+    q = np.zeros((n, n))
+    idx = rng.integers(0, n, (3 * n, 2))
+    for i, j in idx:
+        if i != j:
+            q[i, j] = q[j, i] = rng.integers(-2, 3)
+    q[np.arange(n), np.arange(n)] = rng.integers(-2, 3, n)
+    beta = oracle.geometric_schedule(0.1, 2.0, 200)
+    _, be, states, energies = ctx.anneal_reference(q, beta, 4)
+    s_or, e_or = oracle.anneal(q, n, beta, 4)
+    assert (states == s_or).all()
+    assert np.array_equal(energies, oracle.energy(q, states))
+
+
+def test_reference_anneal_rejects_bad_arguments(ctx):
+    with pytest.raises(onesolver_amd.InvalidArgument):
+        ctx.anneal_reference(np.zeros((3, 3)), np.ones(5), 0)
+
+
+def test_cli_gpu_device_end_to_end(tmp_path):
+    out = tmp_path / "r.csv"
+    r = subprocess.run([onesolver_amd.CLI_PATH, "--input",
+                        os.path.join(GOLDEN, "examples", "csp13.qubo"), "--output", str(out),
+                        "--device-type", "gpu", "--num-iter", "2000", "--num-tries", "256",
+                        "--beta-max", "5"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Using device: " in r.stdout
+    head, row = out.read_text().splitlines()
+    assert head.endswith(",energy")
+    assert float(row.split(",")[-1]) == -32.0

@@ -1,0 +1,318 @@
+"""Pins the CPU oracle before it is trusted as the checker of the GPU path.
+
+Each test checks an oracle function against something independent of it:
+published known-answer vectors, the reference's own test data and published
+outputs, closed forms, libm, brute force, or a second restatement.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from qubo_io import load_qubo
+
+# --------------------------------------------------------------------------
+# Philox4x32-10
+# --------------------------------------------------------------------------
+
+RANDOM123_KAT = [
+    # (counter, key, output) -- Random123 kat_vectors, philox4x32_10
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", RANDOM123_KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert tuple(int(v) for v in oracle.philox(ctr, key)) == out
+
+
+def test_philox_matches_rocrand_header(tmp_path):
+    src = os.path.join(os.path.dirname(GOLDEN), "rocrand_kat.cpp")
+    exe = tmp_path / "kat"
+    r = subprocess.run(["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                        src, "-o", str(exe)], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("rocRAND host header not compilable here: " + r.stderr[-300:])
+    cases = [(1234, 0), (1234, 4), (0, 8), (0xDEADBEEFCAFEF00D, 4 * 1000), (7, 4 * 123456)]
+    inp = "".join(f"{s} {o}\n" for s, o in cases)
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout
+    for (seed, off), line in zip(cases, out.strip().splitlines()):
+        got = oracle.philox((off // 4, 0, 0, 0), (seed & 0xFFFFFFFF, seed >> 32))
+        assert [int(v) for v in line.split()] == [int(v) for v in got]
+    # the survey's first words for seed 1234
+    assert [f"{int(v):08x}" for v in oracle.philox((0, 0, 0, 0), (1234, 0))] == \
+        ["2090b348", "da7cf0ab", "4401906f", "cbca470e"]
+
+
+def test_neg_ln_u_against_libm_on_whole_grid():
+    k = np.arange(1 << 23, dtype=np.uint64)
+    w = (k << np.uint64(9)).astype(np.uint32)
+    got = oracle.neg_ln_u_many(w).astype(np.float64)
+    u = (2.0 * k.astype(np.float64) + 1.0) * 2.0 ** -24
+    ref = -np.log(u)
+    rel = np.abs(got - ref) / ref
+    assert rel.max() < 4e-7
+    assert (got > 0).all()
+    # the low 9 bits of the word are ignored
+    assert oracle.neg_ln_u(0x12345600) == oracle.neg_ln_u(0x123457FF)
+
+
+# --------------------------------------------------------------------------
+# flatten, energy, schedules
+# --------------------------------------------------------------------------
+
+
+def test_flatten_reference_kat():
+    # tests/qubo_helpers_test.cpp:14-27
+    lin = {0: 0.5, 1: -2.0, 2: 1.0, 4: -1.5}
+    quad = {(0, 1): 1.0, (0, 3): 7.2, (1, 4): -1.0, (2, 3): 2.0, (2, 4): -1.5, (3, 4): -3.5}
+    expected = [0.5, 1.0, 0.0, 7.2, 0.0, 1.0, -2.0, 0.0, 0.0, -1.0, 0.0, 0.0, 1.0, 2.0, -1.5, 7.2,
+                0.0, 2.0, 0.0, -3.5, 0.0, -1.0, -1.5, -3.5, -1.5]
+    assert list(oracle.flatten(5, lin, quad)) == expected
+
+
+def test_energy_closed_form_random():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 31):
+        q = rng.uniform(-1, 1, (n, n))
+        q = (q + q.T) / 2
+        for _ in range(5):
+            x = rng.integers(0, 2, n).astype(np.uint8)
+            ref = sum(q[i, j] * x[i] * x[j] for i in range(n) for j in range(i, n))
+            assert oracle.energy(q, x) == pytest.approx(ref, rel=1e-14, abs=1e-14)
+
+
+def _kats():
+    with open(os.path.join(GOLDEN, "kat", "expected.json")) as f:
+        return [(os.path.join(GOLDEN, "kat", e["file"]), e["energy"]) for e in json.load(f)]
+
+
+@pytest.mark.parametrize("path,energy", _kats())
+def test_exhaustive_reference_kats(path, energy):
+    # tests/exhaustive_test.cpp:146-157, tolerance 1e-13 (:156)
+    q = load_qubo(path)
+    n = q.shape[0]
+    s1, e1 = oracle.exhaustive(q, n, literal=True)
+    s2, e2 = oracle.exhaustive(q, n, literal=False)
+    assert abs(e1 - energy) < 1e-13
+    assert abs(e2 - energy) < 1e-13
+    assert (s1 == s2).all()
+    assert abs(oracle.energy(q, s1) - energy) < 1e-13
+
+
+EXAMPLES = {"simple": -2.0, "test1": -12.0, "test2": -1.2, "csp5": -22.0, "csp7": -14.0,
+            "csp13": -32.0}
+
+
+@pytest.mark.parametrize("name", sorted(EXAMPLES))
+def test_example_instances_ground_energies(name):
+    q = load_qubo(os.path.join(GOLDEN, "examples", name + ".qubo"))
+    _, e = oracle.exhaustive(q, q.shape[0])
+    assert abs(e - EXAMPLES[name]) < 1e-12
+
+
+def test_exhaustive_literal_vs_gray_random():
+    rng = np.random.default_rng(11)
+    for n in (3, 9, 14):
+        for _ in range(3):
+            q = rng.uniform(-1, 1, (n, n))
+            q = (q + q.T) / 2
+            s1, e1 = oracle.exhaustive(q, n, literal=True)
+            s2, e2 = oracle.exhaustive(q, n, literal=False, threads=4)
+            assert e1 == e2 and (s1 == s2).all()
+
+
+def test_exhaustive_ties_go_to_lowest_index():
+    # all-zero model: every state has energy 0, the reference returns state 0
+    s, e = oracle.exhaustive(np.zeros((6, 6)), 6)
+    assert e == 0.0 and not s.any()
+    # x0 and x1 symmetric: minimum at {0} or {1}; index 1 (x0 = 1) is lower
+    q = np.array([[-1.0, 5.0], [5.0, -1.0]])
+    s, e = oracle.exhaustive(q, 2, literal=True)
+    assert list(s) == [1, 0] and e == -1.0
+
+
+def _published_rows(n):
+    path = os.path.join(GOLDEN, "chimera", f"{n}_results_subset.csv")
+    with open(path) as f:
+        lines = f.read().splitlines()[1:]
+    for line in lines:
+        t = line.split(",")
+        yield np.array([int(v) for v in t[:n]], dtype=np.uint8), float(t[n])
+
+
+@pytest.mark.parametrize("n", [128, 512])
+def test_energy_matches_published_annealer_outputs(n):
+    # benchmarks/annealing/results/<n>/001.csv: state + energy printed with 6
+    # significant digits (include/model/solution.hpp:58-67)
+    q = load_qubo(os.path.join(GOLDEN, "chimera", f"{n}_001.qubo"))
+    count = 0
+    for x, printed in _published_rows(n):
+        e = oracle.energy(q, x)
+        tol = 0.5 * 10 ** (np.floor(np.log10(abs(e))) - 5)
+        assert abs(e - printed) <= tol * 1.0001
+        count += 1
+    assert count >= 17
+
+
+@pytest.mark.parametrize("n,expected", [(128, -235.866654), (512, -928.746690),
+                                        (2048, -3418.266732)])
+def test_tensor_network_ground_states(n, expected):
+    q = load_qubo(os.path.join(GOLDEN, "chimera", f"{n}_001.qubo"))
+    with open(os.path.join(GOLDEN, "chimera", f"{n}_groundstate_TN.txt")) as f:
+        tok = f.read().split()
+    spins = np.array([int(v) for v in tok[3:3 + n]])
+    x = ((spins + 1) // 2).astype(np.uint8)
+    e = oracle.energy(q, x)
+    assert abs(e - expected) < 1e-5
+    if n in (128, 512):  # the published annealer never beat the ground state
+        best_pub = min(p for _, p in _published_rows(n))
+        assert e < best_pub
+
+
+def test_schedules_closed_forms():
+    lin = oracle.linear_schedule(0.1, 10.0, 1000)
+    i = np.arange(1000)
+    assert np.array_equal(lin, 0.1 + 10.0 * i / 999.0)
+    assert lin[-1] == 0.1 + 10.0  # ends at beta_min + beta_max
+    assert np.isnan(oracle.linear_schedule(0.1, 1.0, 1)[0])
+    geo = oracle.geometric_schedule(0.1, 3.0, 4096)
+    alpha = (3.0 / 0.1) ** (1.0 / 4095)
+    ref = [0.1]
+    for _ in range(4095):
+        ref.append(ref[-1] * alpha)
+    assert np.array_equal(geo, np.array(ref))
+    assert abs(geo[-1] - 3.0) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# reference-mode annealer
+# --------------------------------------------------------------------------
+
+
+def test_anneal_literal_and_fields_follow_the_same_chain_on_integer_instances():
+    for path, ground in _kats():
+        q = load_qubo(path)
+        n = q.shape[0]
+        beta = oracle.geometric_schedule(0.1, 10.0, 300)
+        s1, e1 = oracle.anneal(q, n, beta, 64, literal=True)
+        s2, e2 = oracle.anneal(q, n, beta, 64, literal=False)
+        assert (s1 == s2).all()
+        if np.array_equal(q, np.round(q)):  # exact arithmetic: identical energies
+            assert (e1 == e2).all()
+        assert np.allclose(e1, e2, rtol=0, atol=1e-12)
+        assert (e1 >= ground - 1e-12).all()
+        assert np.allclose(oracle.energy(q, s1), e1, rtol=0, atol=1e-12)
+
+
+def test_anneal_literal_vs_fields_random_real_instance():
+    q = load_qubo(os.path.join(GOLDEN, "c1_dense28.qubo"))
+    beta = oracle.geometric_schedule(0.1, 1.0, 1000)
+    s1, e1 = oracle.anneal(q, 28, beta, 64, literal=True)
+    s2, e2 = oracle.anneal(q, 28, beta, 64, literal=False)
+    assert (s1 == s2).mean() > 0.999
+    assert np.allclose(e1, e2, rtol=1e-12, atol=1e-12)
+
+
+def test_anneal_replicas_are_independent_of_num_tries():
+    q = load_qubo(os.path.join(GOLDEN, "kat", "kat_4.qubo"))
+    n = q.shape[0]
+    beta = oracle.linear_schedule(0.1, 2.0, 200)
+    s_all, e_all = oracle.anneal(q, n, beta, 40)
+    s_sub, e_sub = oracle.anneal(q, n, beta, 10, first_replica=30)
+    assert (s_all[30:] == s_sub).all() and (e_all[30:] == e_sub).all()
+
+
+def test_anneal_heats_up_under_reference_semantics():
+    # beta is a temperature (annealing.hpp:107): at beta -> 0 only downhill
+    # moves and ties... are rejected/accepted as the reference does; a hot
+    # (large beta) schedule accepts almost everything.
+    q = load_qubo(os.path.join(GOLDEN, "kat", "kat_2.qubo"))
+    n = q.shape[0]
+    cold = np.full(400, 1e-9)
+    s, e = oracle.anneal(q, n, cold, 16)
+    for x in s:  # a cold chain ends in a local minimum of single flips
+        for j in range(n):
+            y = x.copy()
+            y[j] ^= 1
+            assert oracle.energy(q, y) >= oracle.energy(q, x) - 1e-12
+
+
+# --------------------------------------------------------------------------
+# sweep-mode annealers
+# --------------------------------------------------------------------------
+
+
+def _random_csr(n, d, seed, integer_diag=False):
+    from onesolver_amd import instances
+
+    rp, col, val, diag = instances.regular_csr(n, d, seed=seed, instance=1)
+    if integer_diag:
+        diag = (np.round(diag * 2) / 2).astype(np.float32)
+    return rp, col, val, diag
+
+
+def test_sweep_greedy_limit_ends_in_local_minima():
+    q = load_qubo(os.path.join(GOLDEN, "c1_dense28.qubo"))
+    fin, best, be = oracle.sweep_dense(q, 28, np.zeros(20, np.float32), 32)
+    for x in fin:
+        e = oracle.energy(q, x)
+        for j in range(28):
+            y = x.copy()
+            y[j] ^= 1
+            assert oracle.energy(q, y) >= e - 1e-12
+    assert np.allclose(oracle.energy(q, best), be, atol=1e-9)
+
+
+def test_sweep_csr_equals_sweep_dense_on_exact_instances():
+    from onesolver_amd import instances
+
+    rp, col, val, diag = _random_csr(60, 4, 5, integer_diag=True)
+    q = instances.csr_to_dense(rp, col, val, diag)
+    ts = (1.0 / np.geomspace(0.1, 3.0, 50)).astype(np.float32)
+    f1, b1, e1 = oracle.sweep_dense(q, 60, ts, 16, seed=9)
+    f2, b2, e2 = oracle.sweep_csr(rp, col, val, diag, ts, 16, seed=9)
+    assert (f1 == f2).all() and (b1 == b2).all() and (e1 == e2).all()
+    assert np.allclose(oracle.csr_energy(rp, col, val, diag, b2), e2, atol=1e-9)
+
+
+@pytest.mark.parametrize("kind", ["dense", "csr"])
+def test_sweep_stationary_law_is_boltzmann(kind):
+    # Sequential-sweep Metropolis leaves the Boltzmann law invariant: after
+    # many sweeps at fixed beta the final states of independent replicas
+    # follow p(x) ~ exp(-beta E(x)).
+    q = np.array([[0.3, -1.0, 0.5, 0.0], [-1.0, -0.2, 0.7, 0.4], [0.5, 0.7, 0.1, -0.6],
+                  [0.0, 0.4, -0.6, 0.2]])
+    beta = 1.3
+    R = 100000
+    ts = np.full(30, 1.0 / beta, dtype=np.float32)
+    if kind == "dense":
+        fin, _, _ = oracle.sweep_dense(q, 4, ts, R, seed=21)
+    else:
+        rows, cols = np.nonzero(q - np.diag(np.diag(q)))
+        rp = np.searchsorted(rows, np.arange(5)).astype(np.int32)
+        fin, _, _ = oracle.sweep_csr(rp, cols.astype(np.int32), q[rows, cols].astype(np.float32),
+                                     np.diag(q).astype(np.float32), ts, R, seed=21)
+    idx = fin @ (1 << np.arange(4))
+    counts = np.bincount(idx, minlength=16)
+    states = ((np.arange(16)[:, None] >> np.arange(4)) & 1).astype(np.uint8)
+    e = oracle.energy(q.astype(np.float32).astype(np.float64), states)
+    p = np.exp(-beta * (e - e.min()))
+    p /= p.sum()
+    chi2 = ((counts - R * p) ** 2 / (R * p)).sum()
+    assert chi2 < 45.0  # 15 dof: p ~ 1e-4
+
+
+def test_warm_init_follows_marginals():
+    p = np.array([0.0, 1.0, 0.25, 0.9], dtype=np.float32)
+    st = oracle.warm_init(p, 20000, seed=3)
+    m = st.mean(axis=0)
+    assert m[0] == 0.0 and m[1] == 1.0
+    assert abs(m[2] - 0.25) < 0.015 and abs(m[3] - 0.9) < 0.01
