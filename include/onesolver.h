@@ -77,6 +77,50 @@ int os_anneal_reference(os_ctx *ctx, int n, const double *q_sym, const double *b
 int os_energy_fp64(os_ctx *ctx, int n, const double *q_sym, uint32_t count,
                    const uint8_t *states, double *energies);
 
+/* ---- sweep-mode annealing (configs quoted in "sweeps") -------------------
+ * A sweep visits every site once in the fixed order j = 0..N-1; site j of
+ * sweep t flips iff dE < -ln(u) * tscale[t] with u the open-interval 24-bit
+ * uniform of Philox counter (j>>2, replica, t, (instance<<8)|0x11), i.e. with
+ * probability min(1, exp(-dE / tscale[t])).  tscale[t] = 1/beta_t for an
+ * inverse-temperature schedule; tscale[t] = beta_t reproduces the reference's
+ * use of beta as a temperature (annealing.hpp:107).  The best state of each
+ * replica is tracked at sweep boundaries; returned energies are FP64
+ * recomputations of the returned states.  No reference counterpart: the
+ * reference only has random-site attempts (annealing.hpp:97-101). */
+typedef struct os_sweep_params {
+  uint32_t num_replicas;       /* replicas (rounded up to a multiple of 64 inside) */
+  int32_t num_sweeps;          /* sweeps per replica */
+  const float *tscale;         /* [num_sweeps] host array */
+  uint64_t seed;               /* Philox key */
+  uint32_t instance;           /* stream id of this instance (< 2^24) */
+  int32_t refresh_interval;    /* sweeps between FP64 field refreshes; <= 0: once at the end */
+  const uint8_t *init_states;  /* optional [num_replicas x n] start states (0/1) */
+  const float *init_marginals; /* optional [n] warm start: x_j ~ Bernoulli(p_j) */
+} os_sweep_params;
+
+typedef struct os_sweep_stats {
+  double kernel_ms;        /* summed duration of the sweep kernels (HIP events) */
+  double total_ms;         /* the whole call on the device stream */
+  int32_t kernel_launches; /* sweep-kernel launches */
+  double max_field_drift;  /* max |h_fp32 - h_fp64| over all refreshes */
+  uint64_t attempts;       /* num_replicas x num_sweeps x n spin-flip attempts */
+} os_sweep_stats;
+
+typedef struct os_dense os_dense;
+
+/* Upload a dense QUBO (q_sym: n x n FP64 symmetric, the reference's flat
+ * matrix) for sweep-mode annealing; keeps Q as FP64 and as bf16 hi+lo pairs
+ * on the GPU.  n <= 1024 in this version. */
+int os_dense_create(os_ctx *ctx, int n, const double *q_sym, os_dense **out);
+int os_dense_destroy(os_dense *dense);
+
+/* Run num_replicas sweep-mode chains.  Outputs (each may be NULL):
+ * best_energies [R], best_states [R x n], final_states [R x n],
+ * best_state [n] + best_energy (lowest replica, first index on ties). */
+int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best_energies,
+                    uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
+                    double *best_energy, os_sweep_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -58,8 +58,37 @@ def lib():
                                           ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, _u8p,
                                           _f64p, _u8p, _f64p]
         L.os_energy_fp64.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_uint32, _u8p, _f64p]
+        L.os_dense_create.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.POINTER(_vp)]
+        L.os_dense_destroy.argtypes = [_vp]
+        L.os_dense_anneal.argtypes = [_vp, ctypes.POINTER(SweepParams), _f64p, _u8p, _u8p, _u8p,
+                                      _f64p, ctypes.POINTER(SweepStats)]
         _lib = L
     return _lib
+
+
+class SweepParams(ctypes.Structure):
+    """os_sweep_params of include/onesolver.h."""
+    _fields_ = [("num_replicas", ctypes.c_uint32), ("num_sweeps", ctypes.c_int32),
+                ("tscale", _vp), ("seed", ctypes.c_uint64), ("instance", ctypes.c_uint32),
+                ("refresh_interval", ctypes.c_int32), ("init_states", _vp),
+                ("init_marginals", _vp)]
+
+
+class SweepStats(ctypes.Structure):
+    """os_sweep_stats of include/onesolver.h."""
+    _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("kernel_launches", ctypes.c_int32), ("max_field_drift", ctypes.c_double),
+                ("attempts", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def tscale_from_beta(beta, temperature=False):
+    """Per-sweep threshold scale: 1/beta (inverse temperature) or beta (the
+    reference's use of beta as a temperature)."""
+    b = np.asarray(beta, dtype=np.float64)
+    return (b if temperature else 1.0 / b).astype(np.float32)
 
 
 def _check(rc):
@@ -137,3 +166,47 @@ class Context:
         _check(lib().os_energy_fp64(self._h, n, _p(q, _f64p), st.shape[0], _p(st, _u8p),
                                     _p(out, _f64p)))
         return out
+
+
+class DenseProblem:
+    """A dense QUBO resident on the GPU for sweep-mode annealing (os_dense)."""
+
+    def __init__(self, ctx: Context, q_sym):
+        q = np.ascontiguousarray(q_sym, dtype=np.float64)
+        self.n = q.shape[0]
+        self.ctx = ctx
+        h = _vp()
+        _check(lib().os_dense_create(ctx.handle, self.n, _p(q, _f64p), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().os_dense_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def anneal(self, num_replicas, tscale, seed=1, instance=0, refresh_interval=0,
+               init_states=None, init_marginals=None, best_states=True, final_states=False):
+        ts = np.ascontiguousarray(tscale, dtype=np.float32)
+        n = self.n
+        ini = None if init_states is None else np.ascontiguousarray(init_states, dtype=np.uint8)
+        mar = None if init_marginals is None else np.ascontiguousarray(init_marginals, dtype=np.float32)
+        prm = SweepParams(num_replicas, len(ts), ts.ctypes.data, seed, instance, refresh_interval,
+                          None if ini is None else ini.ctypes.data,
+                          None if mar is None else mar.ctypes.data)
+        be = np.zeros(num_replicas, dtype=np.float64)
+        bs = np.zeros((num_replicas, n), dtype=np.uint8) if best_states else None
+        fs = np.zeros((num_replicas, n), dtype=np.uint8) if final_states else None
+        best = np.zeros(n, dtype=np.uint8)
+        e = ctypes.c_double(0.0)
+        st = SweepStats()
+        _check(lib().os_dense_anneal(self._h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p),
+                                     _p(fs, _u8p), _p(best, _u8p), ctypes.byref(e),
+                                     ctypes.byref(st)))
+        return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
+                "best_energy": e.value, "stats": st.as_dict()}

@@ -15,27 +15,11 @@
 
 namespace {
 thread_local std::string g_last_error;
-
-template <typename F>
-int guarded(F &&f) {
-  try {
-    f();
-    g_last_error.clear();
-    return OS_OK;
-  } catch (const os::Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    g_last_error = "out of host memory";
-    return OS_ERR_INTERNAL;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return OS_ERR_INTERNAL;
-  }
-}
 }  // namespace
 
 namespace os {
+
+void set_last_error(const char *msg) { g_last_error = msg; }
 
 void DevBuf::ensure(size_t bytes) {
   if (bytes <= cap) return;
@@ -83,6 +67,7 @@ void Ctx::sync() const { OS_HIP_CHECK(hipStreamSynchronize(stream)); }
 }  // namespace os
 
 using os::Ctx;
+using os::guarded;
 
 extern "C" {
 
@@ -141,8 +126,8 @@ int os_energy_fp64(os_ctx *ctx_, int n, const double *q_sym, uint32_t count,
     OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_sym, qbytes, hipMemcpyHostToDevice, c.stream));
     OS_HIP_CHECK(hipMemcpyAsync(c.b_bytes.ptr, states, (size_t)count * n, hipMemcpyHostToDevice,
                                 c.stream));
-    os::launch_pack_bytes(c.stream, c.b_bytes.as<uint8_t>(), n, count, c.b_bits.as<uint32_t>());
-    os::launch_energy_fp64(c.stream, n, c.b_q.as<double>(), count, c.b_bits.as<uint32_t>(),
+    os::launch_pack_bytes(c.stream, c.b_bytes.as<uint8_t>(), n, count, W, c.b_bits.as<uint32_t>());
+    os::launch_energy_fp64(c.stream, n, W, c.b_q.as<double>(), count, c.b_bits.as<uint32_t>(),
                            c.b_energy.as<double>(), c.b_scratch.as<double>(), nullptr, 0, nullptr);
     OS_HIP_CHECK(hipMemcpyAsync(energies, c.b_energy.ptr, sizeof(double) * count,
                                 hipMemcpyDeviceToHost, c.stream));
@@ -183,7 +168,7 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
                           sweeps_per_beta, num_tries, seed,
                           n > os::ref_lds_limit() ? c.b_fields.as<double>() : nullptr,
                           c.b_bits.as<uint32_t>(), c.b_run.as<double>());
-    os::launch_energy_fp64(c.stream, n, c.b_q.as<double>(), (int)num_tries,
+    os::launch_energy_fp64(c.stream, n, W, c.b_q.as<double>(), (int)num_tries,
                            c.b_bits.as<uint32_t>(), c.b_energy.as<double>(),
                            c.b_scratch.as<double>(), nullptr, 0, nullptr);
     std::vector<double> e(num_tries);
@@ -192,7 +177,7 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
     std::vector<uint8_t> bytes;
     if (best_states) {
       c.b_bytes.ensure((size_t)num_tries * n);
-      os::launch_unpack_bits(c.stream, c.b_bits.as<uint32_t>(), n, num_tries,
+      os::launch_unpack_bits(c.stream, c.b_bits.as<uint32_t>(), n, num_tries, W,
                              c.b_bytes.as<uint8_t>());
       OS_HIP_CHECK(hipMemcpyAsync(best_states, c.b_bytes.ptr, (size_t)num_tries * n,
                                   hipMemcpyDeviceToHost, c.stream));

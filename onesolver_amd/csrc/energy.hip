@@ -145,16 +145,16 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double *__re
   out[r] = s;
 }
 
-void launch_pack_bytes(hipStream_t st, const uint8_t *bytes, int n, int count, uint32_t *bits) {
-  const int W = (n + 31) / 32;
+void launch_pack_bytes(hipStream_t st, const uint8_t *bytes, int n, int count, int W,
+                       uint32_t *bits) {
   const size_t total = (size_t)count * W;
   if (total == 0) return;
   pack_bytes_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(bytes, n, count, W, bits);
   OS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_unpack_bits(hipStream_t st, const uint32_t *bits, int n, int count, uint8_t *bytes) {
-  const int W = (n + 31) / 32;
+void launch_unpack_bits(hipStream_t st, const uint32_t *bits, int n, int count, int W,
+                        uint8_t *bytes) {
   const size_t total = (size_t)count * n;
   if (total == 0) return;
   unpack_bits_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(bits, n, count, W, bytes);
@@ -166,11 +166,10 @@ size_t energy_scratch_bytes(int n, int count) {
   return sizeof(double) * (size_t)count * ncb;
 }
 
-void launch_energy_fp64(hipStream_t st, int n, const double *q, int count, const uint32_t *bits,
-                        double *out, double *scratch, float *fields, int ldf,
+void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count,
+                        const uint32_t *bits, double *out, double *scratch, float *fields, int ldf,
                         unsigned int *drift_max) {
   if (count <= 0) return;
-  const int W = (n + 31) / 32;
   const int ncb = (n + kTN - 1) / kTN;
   dim3 grid(ncb, (count + kTR - 1) / kTR);
   if (fields)

@@ -38,4 +38,26 @@ struct Ctx {
   void sync() const;
 };
 
+// Thread-local message behind os_last_error().
+void set_last_error(const char *msg);
+
+// Runs `f`, mapping exceptions to the C ABI's return codes.
+template <typename F>
+int guarded(F &&f) {
+  try {
+    f();
+    set_last_error("");
+    return 0;
+  } catch (const Error &e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    set_last_error("out of host memory");
+    return -4;
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return -4;
+  }
+}
+
 }  // namespace os
