@@ -1,0 +1,183 @@
+"""Benchmark of the MI355X simulated-annealing QUBO solver.
+
+Default workload (the configuration the headline metric is quoted on):
+dense SK-type Ising mapped to QUBO, N=1024, J_ij ~ N(0,1)/sqrt(N), h=0,
+16384 replicas, 4096 sweeps, geometric beta 0.1 -> 3.0 (inverse temperature),
+bf16 hi+lo MFMA delayed field updates with block k=16, FP64 field refresh
+every 512 sweeps.  One step = one full anneal of that batch with Q already
+resident in HBM.  Prints one JSON line (see the keys below).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline_dense(q, n, budget_s=12.0):
+    """The reference's host path restated: random-site attempts with an O(N^2)
+    FP64 energy recompute each (or_anneal_literal), one thread, on a bounded
+    sample; plus the O(N) local-field port on all cores for comparison."""
+    import oracle
+
+    beta = oracle.geometric_schedule(0.1, 3.0, 64)
+    # calibrate on a short run, then time about budget_s seconds of work
+    t0 = time.perf_counter()
+    oracle.anneal(q, n, beta[:8], 1, literal=True, threads=1)
+    per = (time.perf_counter() - t0) / 8
+    iters = int(max(16, min(200000, budget_s / max(per, 1e-9))))
+    sched = np.full(iters, 1.0)
+    t0 = time.perf_counter()
+    oracle.anneal(q, n, sched, 1, literal=True, threads=1)
+    lit_s = time.perf_counter() - t0
+    lit_rate = iters / lit_s
+    threads = os.cpu_count() or 1
+    try:
+        threads = min(threads, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    ts = (1.0 / np.geomspace(0.1, 3.0, 8)).astype(np.float32)
+    reps = max(threads, 1) * 4
+    t0 = time.perf_counter()
+    oracle.sweep_dense(q, n, ts, reps, threads=threads)
+    f_s = time.perf_counter() - t0
+    return {
+        "value": lit_rate, "unit": "attempts/s", "cores": 1, "kind": "port",
+        "sample": f"reference algorithm restated (O(N^2) FP64 energy per attempt, random site), "
+                  f"1 replica x {iters} attempts on the N={n} instance, {lit_s:.1f} s",
+        "local_field_port": {"value": reps * len(ts) * n / f_s, "unit": "attempts/s",
+                             "cores": threads,
+                             "sample": f"sweep-mode O(N) local fields, {reps} replicas x "
+                                       f"{len(ts)} sweeps, {f_s:.1f} s"},
+    }
+
+
+def run_dense(args):
+    import onesolver_amd
+    from onesolver_amd import instances
+
+    n, R, S = args.n, args.replicas, args.sweeps
+    log(f"generating SK instance n={n}")
+    q32, const = instances.sk_qubo(n, seed=0, instance=0)
+    q = q32.astype(np.float64)
+    beta = np.geomspace(0.1, 3.0, S)
+    ts = onesolver_amd.tscale_from_beta(beta)
+    ctx = onesolver_amd.Context(0)
+    log(f"device {ctx.device_name()}")
+    prob = onesolver_amd.DenseProblem(ctx, q)
+    kwargs = dict(seed=1234, refresh_interval=args.refresh, best_states=False)
+    for i in range(args.warmup):
+        out = prob.anneal(R, ts, **kwargs)
+        log(f"warmup {i}: best {out['best_energy']:.4f} kernel {out['stats']['kernel_ms']:.1f} ms")
+    kernel_ms, launches, drift = 0.0, 0, 0.0
+    best = np.inf
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = prob.anneal(R, ts, **kwargs)
+        st = out["stats"]
+        kernel_ms += st["kernel_ms"]
+        launches += st["kernel_launches"]
+        drift = max(drift, st["max_field_drift"])
+        best = min(best, out["best_energy"])
+        log(f"step {i}: best {out['best_energy']:.4f} (Ising {out['best_energy'] + const:.4f}) "
+            f"kernel {st['kernel_ms']:.1f} ms total {st['total_ms']:.1f} ms")
+    elapsed = time.perf_counter() - t0
+    attempts = float(R) * S * n * args.steps
+    value = attempts / elapsed
+    sweeps_per_launch = S / (launches / args.steps)
+    flops_per_launch = 4.0 * n * (R * sweeps_per_launch * n)
+    avg_launch_s = kernel_ms / launches / 1e3
+    achieved = flops_per_launch / avg_launch_s / 1e12
+    res = {
+        "metric": "spin-flip attempts/s", "value": value, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 hi+lo MFMA, fp32 fields, fp64 energies",
+        "data": "synthetic (Philox-generated SK instance, seed 0)",
+        "config": {"workload": f"dense SK QUBO N={n}, {R} replicas, {S} sweeps, geometric beta "
+                               f"0.1->3.0, block k=16, FP64 refresh every {args.refresh} sweeps",
+                   "n": n, "replicas": R, "sweeps": S, "block_k": 16,
+                   "refresh_interval": args.refresh, "parallelism": "replicas x columns on 1 GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "dense_sweep_kernel<16>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_flops_per_launch": flops_per_launch},
+        "kernel_attempts_per_s": float(R) * S * n * args.steps / (kernel_ms / 1e3),
+        "best_energy": best, "best_energy_ising": best + const,
+        "max_field_drift": drift,
+    }
+    ctx.close()
+    return res, q
+
+
+def run_reference_c1(args):
+    """Config 1: N=28 dense, reference semantics, 1000 iterations x 1024 tries."""
+    import onesolver_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from qubo_io import load_qubo
+
+    q = load_qubo(os.path.join(ROOT, "tests", "golden", "data", "c1_dense28.qubo"))
+    beta = np.geomspace(0.1, 1.0, 1000)
+    beta = np.array([0.1 * (10.0 ** (1.0 / 999)) ** i for i in range(1000)])
+    ctx = onesolver_amd.Context(0)
+    for _ in range(args.warmup):
+        ctx.anneal_reference(q, beta, 1024, want_states=False)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        best, be, _, _ = ctx.anneal_reference(q, beta, 1024, want_states=False)
+    elapsed = time.perf_counter() - t0
+    attempts = 1024.0 * 1000 * args.steps
+    ctx.close()
+    return {
+        "metric": "spin-flip attempts/s", "value": attempts / elapsed, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic (committed N=28 instance)",
+        "config": {"workload": "dense N=28, reference semantics, geometric 0.1->1.0, "
+                               "1000 iterations x 1024 tries", "n": 28},
+        "best_energy": be,
+    }, q
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--replicas", type=int, default=16384)
+    ap.add_argument("--sweeps", type=int, default=4096)
+    ap.add_argument("--refresh", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              "(replicas would shard across GPUs with one final min-reduction)", file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "c1":
+        res, q = run_reference_c1(args)
+    else:
+        res, q = run_dense(args)
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline (bounded sample)")
+        res["cpu_baseline"] = cpu_baseline_dense(q, q.shape[0])
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -40,10 +40,10 @@ struct Dense {
   Ctx *ctx = nullptr;
   int n = 0, np = 0, npw = 0;
   DevBuf q64, qpack, qdb;
-  DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg;
+  DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
-                    &scratch, &drift, &bytes, &marg})
+                    &scratch, &drift, &bytes, &marg, &status})
       b->release();
   }
 };
@@ -151,7 +151,9 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
         (void)hipEventDestroy(b);
       }
     } guard{ev, t0, t1};
+    d.status.ensure(sizeof(int));
     OS_HIP_CHECK(hipEventRecord(t0, c.stream));
+    OS_HIP_CHECK(hipMemsetAsync(d.status.ptr, 0, sizeof(int), c.stream));
     if (S > 0)
       OS_HIP_CHECK(hipMemcpyAsync(d.tscale.ptr, p->tscale, sizeof(float) * S,
                                   hipMemcpyHostToDevice, c.stream));
@@ -195,7 +197,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
                               d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
                               d.fields.as<float>(), d.xbits.as<uint32_t>(),
                               d.bestbits.as<uint32_t>(), d.energy.as<double>(),
-                              d.ebest.as<double>());
+                              d.ebest.as<double>(), d.status.as<int>());
       OS_HIP_CHECK(hipEventRecord(b, c.stream));
       ++launches;
       // FP64 refresh of fields and running energies; records the fp32 drift
@@ -208,6 +210,9 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
                            d.ebest.as<double>(), d.scratch.as<double>(), nullptr, 0, nullptr);
     std::vector<double> e(Rp);
     OS_HIP_CHECK(hipMemcpyAsync(e.data(), d.ebest.ptr, sizeof(double) * Rp, hipMemcpyDeviceToHost,
+                                c.stream));
+    int kstatus = 0;
+    OS_HIP_CHECK(hipMemcpyAsync(&kstatus, d.status.ptr, sizeof(int), hipMemcpyDeviceToHost,
                                 c.stream));
     unsigned int drift_bits = 0;
     OS_HIP_CHECK(hipMemcpyAsync(&drift_bits, d.drift.ptr, sizeof(unsigned int),
@@ -226,6 +231,8 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     }
     OS_HIP_CHECK(hipEventRecord(t1, c.stream));
     c.sync();
+    if (kstatus != 0)
+      throw os::Error(OS_ERR_INTERNAL, "dense sweep kernel aborted: a work-group wait expired");
     const size_t bi = (size_t)(std::min_element(e.begin(), e.begin() + R) - e.begin());
     if (best_energies) std::memcpy(best_energies, e.data(), sizeof(double) * R);
     if (best_energy) *best_energy = e[bi];

@@ -31,7 +31,7 @@ int dense_padded_n(int n);
 void launch_dense_sweeps(hipStream_t st, int n, int np, const uint4 *qpack, const float *qdb,
                          const float *tscale, int sweep0, int nsweeps, uint64_t seed,
                          uint32_t instance, int R_pad, float *fields, uint32_t *xbits,
-                         uint32_t *bestbits, double *energy, double *ebest);
+                         uint32_t *bestbits, double *energy, double *ebest, int *status);
 void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
                        uint32_t *xbits);
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
