@@ -114,7 +114,8 @@ def run_dense(args):
                    "refresh_interval": args.refresh, "parallelism": "replicas x columns on 1 GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
-                     "kernel": "dense_sweep_kernel<16>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "kernel": f"dense_sweep_kernel<{-(-n // 128)}>",
+                     "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_flops_per_launch": flops_per_launch},
         "kernel_attempts_per_s": float(R) * S * n * args.steps / (kernel_ms / 1e3),
         "best_energy": best, "best_energy_ising": best + const,

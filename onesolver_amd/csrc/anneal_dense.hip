@@ -5,25 +5,26 @@
 // fields h_rj = Q_jj + sum_{l!=j} Q_jl x_rl live in MFMA accumulators (fp32).
 // Spins are decided in blocks of k = 16; after block b the flip deltas D_b
 // (replicas x 16, values -1/0/+1, exact in bf16) update every field:
-// F += [D D] * [Qhi; Qlo] on v_mfma_f32_16x16x32_bf16 (Q = Qhi + Qlo; the
-// K=32 operand pairs D with the hi and with the lo half of Q's rows).
+// F += D * Qhi + D * Qlo on v_mfma_f32_32x32x16_bf16 (Q = Qhi + Qlo, both
+// bf16, rows 16b .. 16b+15 of Q).
 //
-// Work-group: 64 replicas, 4 waves (one per SIMD, so each wave has the full
-// register file and the fields sit in accumulation registers); wave w owns
-// the TPW column tiles (16 spins each) [w*TPW, (w+1)*TPW).  Block b belongs
-// to the wave owning its columns.  For block g of the global block sequence
-// (g = sweep * NB + b) the owner, lane = replica:
-//   (a) applies D_{g-1} to block b's tile (4 MFMAs), transposes it through
-//       LDS and decides its 16 spins in order; flips inside the block correct
-//       the later spins with the fp32 16x16 diagonal sub-block of Q;
-//   (b) applies D_{g-1} to its other tiles, one tile per decision step, so the
-//       matrix cores run under the decisions' VALU work;
+// Work-group: 64 replicas (two 32-replica tiles), 4 waves (one per SIMD, so
+// each wave has the full register file and the fields sit in accumulation
+// registers); wave w owns the TPW column tiles (32 spins each) [w*TPW,
+// (w+1)*TPW).  Block b (spins 16b .. 16b+15) lies in column tile b/2 and
+// belongs to the wave owning that tile.  For block g of the global block
+// sequence (g = sweep * NB + b) the owner, lane = replica:
+//   (a) applies D_{g-1} to the tile holding block b (4 MFMAs), transposes it
+//       through LDS and decides its 16 spins in order; flips inside the block
+//       correct the later spins with the fp32 16x16 diagonal sub-block of Q;
+//   (b) applies D_{g-1} to its other tiles, half a tile per decision step, so
+//       the matrix cores run under the decisions' VALU work;
 //   (c) publishes D_g as +/- bit masks in an LDS ring and bumps a counter.
-// The owner's Q operands come from LDS only: the three other waves load them
-// (a third each) early in the previous step and store them in a double-
-// buffered LDS area at its end.  Those waves also apply each published D
-// block to all their own tiles (their operands are prefetched a full block
-// ahead in registers) and compute the acceptance thresholds two blocks ahead.  Waves synchronise only through LDS counters
+// The owner's Q operands come from LDS only: the three other waves copy them
+// (a third each, global_load_lds) into a double-buffered LDS area one step
+// ahead.  Those waves also apply each published D block to all their own
+// tiles (operands staged in LDS one block ahead) and compute the acceptance
+// thresholds kLA blocks ahead.  Waves synchronise only through LDS counters
 // with bounded spin waits (an expired wait aborts the launch and reports it
 // in `status`).
 //
@@ -35,7 +36,7 @@
 namespace os {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 namespace {
 
@@ -43,17 +44,28 @@ constexpr int kWaves = 4;
 constexpr int kRT = 64;    // replicas per work-group
 constexpr int kRing = 32;  // D blocks kept in LDS
 constexpr int kTRing = 4;  // threshold blocks kept in LDS
-constexpr int kLA = 2;     // thresholds are computed this many blocks ahead
+constexpr int kLA = 3;     // thresholds are computed this many blocks ahead
 constexpr int kHelpers = kWaves - 1;
+constexpr int kFT = kRT + 4;  // row stride (floats) of the transposed field tile
 constexpr int kSpinLimit = 1 << 24;
 
+static_assert(kLA < kTRing, "a threshold slot is rewritten only after its block was decided");
+
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+
+__device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
 
 // Synchronisation between the waves only concerns LDS data written with
 // ordinary ds_write instructions.  LDS operations of one wave complete in
 // order, so publishing needs only "all my LDS writes done" (lgkmcnt(0)) and a
 // compiler barrier -- not the vmcnt(0) wait that a workgroup release fence
-// implies, which would stall on this wave's outstanding operand loads.
+// implies, which would stall on this wave's outstanding operand loads.  The
+// counter accesses are written as ds instructions in asm for the same reason:
+// as C++ atomics the compiler cannot tell them apart from the LDS areas that
+// global_load_lds is filling and waits for every outstanding load first.
+// Each asm load waits for its own result (lgkmcnt(0)) inside the statement.
 __device__ __forceinline__ void lds_writes_done() {
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt/expcnt unconstrained
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -65,14 +77,34 @@ __device__ __forceinline__ void wave_sync() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-__device__ __forceinline__ int lds_load(int *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+__device__ __forceinline__ int lds_load(const int *p) {
+  int v;
+  asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
+  return v;
+}
+
+// 16-byte LDS read that waits for its own result.
+__device__ __forceinline__ uint4 lds_read16(const uint4 *p) {
+  uint4 v;
+  asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
+  return v;
+}
+
+__device__ __forceinline__ void lds_write_f32(float *p, float v) {
+  asm volatile("ds_write_b32 %0, %1" : : "v"(lds_addr(p)), "v"(v) : "memory");
 }
 
 // Publishes v in *p after all of this wave's earlier LDS writes.
 __device__ __forceinline__ void lds_store(int *p, int v) {
-  lds_writes_done();
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  asm volatile("s_waitcnt lgkmcnt(0)\n\tds_write_b32 %0, %1" : : "v"(lds_addr(p)), "v"(v) : "memory");
+}
+
+__device__ __forceinline__ void lds_add(int *p) {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\tds_add_u32 %0, %1" : : "v"(lds_addr(p)), "v"(1) : "memory");
 }
 
 // Waits until *p >= target, then orders later LDS reads after the wait.
@@ -92,10 +124,19 @@ __device__ __forceinline__ bool wait_ge(int *p, int target, int *abort) {
   return true;
 }
 
-// thresholds of spins [16*bn + 4q, +4) of replica r for sweep (sweep0+sw)
-__device__ __forceinline__ void block_thresholds(int item, int bn, int sw, int n, uint32_t rbase,
-                                                 int sweep0, uint32_t k0, uint32_t k1,
-                                                 uint32_t tag, float ts, float (*thr)[17]) {
+// s_waitcnt vmcnt(N): at most N of this wave's vector memory loads (which
+// complete in issue order) still outstanding.
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
+}
+
+// thresholds of spins [16*bn + 4q, +4) of replica r = item/4 (q = item%4) for
+// sweep (sweep0+sw); spins beyond n never flip
+__device__ __forceinline__ void item_thresholds(int item, int bn, int sw, int n, uint32_t rbase,
+                                                int sweep0, uint32_t k0, uint32_t k1,
+                                                uint32_t tag, float ts, float (&v)[4]) {
   const int r = item >> 2, q = item & 3;
   const int j0 = bn * 16 + 4 * q;
   const uint4 o = philox4x32_10(make_uint4((uint32_t)(j0 >> 2), rbase + (uint32_t)r,
@@ -103,26 +144,55 @@ __device__ __forceinline__ void block_thresholds(int item, int bn, int sw, int n
                                 k0, k1);
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) {
-    const int j = j0 + jj;
-    thr[r][4 * q + jj] = j < n ? neg_ln_u(pick(o, jj)) * ts : -INFINITY;
+    const float x = neg_ln_u(pick(o, jj)) * ts;  // computed unconditionally: no branch
+    v[jj] = j0 + jj < n ? x : -INFINITY;
   }
 }
 
-// Q operand (16 bytes per lane) of block `blk`, this wave's column tile tt.
-__device__ __forceinline__ uint4 q_operand(const uint4 *__restrict__ qpack, int np, int blk,
-                                           int col0, int tt, int lrow, int kg) {
-  return qpack[((size_t)blk * np + col0 + 16 * tt + lrow) * 4 + kg];
+__device__ __forceinline__ void store_thresholds(int item, const float (&v)[4],
+                                                 float (*thr)[17]) {
+  const int r = item >> 2, q = item & 3;
+#pragma unroll
+  for (int jj = 0; jj < 4; ++jj) thr[r][4 * q + jj] = v[jj];
 }
 
-// A operands of a D block: replica 16m + lrow, spins 8*(kg&1) .. +7, from the
-// ring's +/- masks (low 16 bits: flips 0->1, high 16 bits: flips 1->0)
-// through the 8-bit -> 8 x bf16 tables (1.0 resp. -1.0 where the bit is set).
-__device__ __forceinline__ void load_a(const uint32_t *slot, const uint4 *lutp, const uint4 *lutn,
-                                       int lrow, int kg, bf16x8 (&A)[4]) {
-  const int sh = 8 * (kg & 1);
+__device__ __forceinline__ void store_thresholds_asm(int item, const float (&v)[4],
+                                                     float (*thr)[17]) {
+  const int r = item >> 2, q = item & 3;
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const uint32_t wd = slot[16 * m + lrow];
+  for (int jj = 0; jj < 4; ++jj) lds_write_f32(&thr[r][4 * q + jj], v[jj]);
+}
+
+__device__ __forceinline__ void block_thresholds(int item, int bn, int sw, int n, uint32_t rbase,
+                                                 int sweep0, uint32_t k0, uint32_t k1,
+                                                 uint32_t tag, float ts, float (*thr)[17]) {
+  float v[4];
+  item_thresholds(item, bn, sw, n, rbase, sweep0, k0, k1, tag, ts, v);
+  store_thresholds(item, v, thr);
+}
+
+// Round r (0-based) of Philox4x32-10; ten rounds from (ctr, k0, k1) give
+// philox4x32_10(ctr, k0, k1).
+__device__ __forceinline__ void philox_round(uint4 &c, uint32_t &k0, uint32_t &k1, int r) {
+  if (r) {
+    k0 += kPhiloxW0;
+    k1 += kPhiloxW1;
+  }
+  const uint64_t p0 = (uint64_t)kPhiloxM0 * c.x;
+  const uint64_t p1 = (uint64_t)kPhiloxM1 * c.z;
+  c = make_uint4((uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1,
+                 (uint32_t)p0);
+}
+
+// A operands of a D block: replica 32m + lc, spins 8*kh .. +7, from the ring's
+// +/- masks (low 16 bits: flips 0->1, high 16 bits: flips 1->0) through the
+// 8-bit -> 8 x bf16 tables (1.0 resp. -1.0 where the bit is set).
+__device__ __forceinline__ void load_a(const uint32_t *slot, const uint4 *lutp, const uint4 *lutn,
+                                       int lc, int kh, bf16x8 (&A)[2]) {
+  const int sh = 8 * kh;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    const uint32_t wd = slot[32 * m + lc];
     const uint4 p = lutp[(wd >> sh) & 0xFFu];
     const uint4 q = lutn[(wd >> (16 + sh)) & 0xFFu];
     A[m] = as_bf16x8(make_uint4(p.x | q.x, p.y | q.y, p.z | q.z, p.w | q.w));
@@ -139,48 +209,52 @@ __device__ __forceinline__ void stage_qd(const float *__restrict__ qdb, int blk,
 
 // Asynchronous copy (global_load_lds_dwordx4) of the Q operands of block
 // `blk` for the column tiles [t0, t1) of the wave whose columns start at col0:
-// lane l's 16 bytes land at dst[tt][l], so a lane later reads back exactly the
-// operand it would have loaded.
+// dst[tt][p][l] receives lane l's B operand of tile tt, p = 0 (hi) / 1 (lo):
+// bf16 rows 16*blk + 8*kh .. +7 of column col0 + 32*tt + lc.
 __device__ __forceinline__ void stage_tiles(const uint4 *__restrict__ qpack, int np, int blk,
-                                            int col0, int t0, int t1, int lrow, int kg,
-                                            uint4 (*dst)[64]) {
+                                            int col0, int t0, int t1, int lc, int kh,
+                                            uint4 (*dst)[2][64]) {
   for (int tt = t0; tt < t1; ++tt)
-    __builtin_amdgcn_global_load_lds(
-        (__attribute__((address_space(1))) void *)(qpack + ((size_t)blk * np + col0 + 16 * tt + lrow) * 4 + kg),
-        (__attribute__((address_space(3))) void *)&dst[tt][0], 16, 0, 0);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+      __builtin_amdgcn_global_load_lds(
+          (__attribute__((address_space(1))) void *)(qpack + ((size_t)blk * np + col0 + 32 * tt + lc) * 4 + 2 * p + kh),
+          (__attribute__((address_space(3))) void *)&dst[tt][p][0], 16, 0, 0);
 }
 
 // Non-owner step: wait for D_next and apply it to every tile with the
 // operands staged in this wave's LDS buffer (block sb_blk, staged now if
 // stale), then stage the following block's operands, which land while the
-// wave waits for the next D block.  Returns false if the wait expired.
+// wave does other work.  Returns false if the wait expired.
 template <int TPW>
 __device__ __forceinline__ bool follow_one(const uint4 *__restrict__ qpack, int np, int col0,
-                                           int lrow, int kg, int lane, int w, int G,
+                                           int lc, int kh, int lane, int w, int G,
                                            const uint32_t (*dring)[kRT], const uint4 *lutp,
                                            const uint4 *lutn, int *d_count, int *wnext,
-                                           int *abort_flag, int &next, int &sb_blk,
-                                           uint4 (*sb)[64], f32x4 (&C)[4][TPW]) {
+                                           int *abort_flag, int &next,
+                                           int &sb_blk, uint4 (*sb)[2][64],
+                                           f32x16 (&C)[2][TPW]) {
   const int nb = np / 16;
   if (sb_blk != next % nb) {
-    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lrow, kg, sb);
+    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lc, kh, sb);
     sb_blk = next % nb;
   }
   if (!wait_ge(d_count, next + 1, abort_flag)) return false;
-  bf16x8 A[4];
-  load_a(dring[next % kRing], lutp, lutn, lrow, kg, A);
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the staged operands have landed
+  bf16x8 A[2];
+  load_a(dring[next % kRing], lutp, lutn, lc, kh, A);
+  wait_vm<0>();  // the staged operands have landed
 #pragma unroll
-  for (int tt = 0; tt < TPW; ++tt) {
-    const bf16x8 Bt = as_bf16x8(sb[tt][lane]);
+  for (int p = 0; p < 2; ++p)
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-      C[m][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], Bt, C[m][tt], 0, 0, 0);
-  }
+    for (int tt = 0; tt < TPW; ++tt) {
+      const bf16x8 Bt = as_bf16x8(sb[tt][p][lane]);
+#pragma unroll
+      for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+    }
   ++next;
   if (lane == 0) lds_store(&wnext[w], next);  // also waits for the reads of sb above
   if (next < G) {
-    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lrow, kg, sb);
+    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lc, kh, sb);
     sb_blk = next % nb;
   }
   return true;
@@ -195,21 +269,22 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     uint32_t tag, float *__restrict__ fields, uint32_t *__restrict__ xbits,
     uint32_t *__restrict__ bestbits, double *__restrict__ energy, double *__restrict__ ebest,
     int *__restrict__ status) {
-  static_assert(TPW <= 16, "one interleaved tile per decision step");
-  constexpr int NP = kWaves * 16 * TPW;
+  static_assert(TPW <= 8, "half a tile per decision step");
+  constexpr int NP = kWaves * 32 * TPW;
   constexpr int NB = NP / 16;
   constexpr int NPW = NP / 32;
+  constexpr int BPW = 2 * TPW;  // blocks per wave
   __shared__ uint32_t dring[kRing][kRT];
   __shared__ uint4 lutp[256];
   __shared__ uint4 lutn[256];
-  __shared__ float fblk[kRT][17];
+  __shared__ __align__(16) float fblk[32][kFT];  // block's column tile, column-major
   __shared__ float thr[kTRing][kRT][17];
   __shared__ uint32_t xs[kRT][NPW + 1];
   __shared__ double es[kRT];
   __shared__ double ebs[kRT];
   __shared__ __align__(16) uint4 qds[2][64];  // staged diagonal 16x16 sub-block of Q (fp32)
-  __shared__ __align__(16) uint4 ob[2][TPW][64];   // owner's Q operands, staged by the helpers
-  __shared__ __align__(16) uint4 sb[kWaves][TPW][64];  // each wave's own staged operands
+  __shared__ __align__(16) uint4 ob[2][TPW][2][64];   // owner's Q operands, staged by the helpers
+  __shared__ __align__(16) uint4 sb[kWaves][TPW][2][64];  // each wave's own staged operands
   __shared__ int d_count;
   __shared__ int ob_cnt[2];
   __shared__ int thr_cnt[kTRing];
@@ -219,10 +294,10 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   const int tid = threadIdx.x;
   const int w = tid >> 6;
   const int lane = tid & 63;
-  const int lrow = lane & 15;
-  const int kg = lane >> 4;
+  const int lc = lane & 31;
+  const int kh = lane >> 5;
   const uint32_t rbase = blockIdx.x * kRT;
-  const int col0 = w * 16 * TPW;
+  const int col0 = w * 32 * TPW;
   const int G = nsweeps * NB;
 
   for (int i = tid; i < kRT * NPW; i += 256)
@@ -242,24 +317,27 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     es[tid] = energy[rbase + tid];
     ebs[tid] = ebest[rbase + tid];
   }
-  if (tid < kTRing) thr_cnt[tid] = tid < kLA ? kHelpers : 0;
+  if (tid < kTRing) thr_cnt[tid] = kHelpers;
   if (tid < kWaves) wnext[tid] = 0;
   if (tid < 2) ob_cnt[tid] = 0;
   if (tid == 0) {
     d_count = 0;
     abort_flag = 0;
   }
-  f32x4 C[4][TPW];
+  // accumulator element i of lane (lc, kh): replica 32m + 8(i/4) + 4kh + i%4,
+  // column col0 + 32tt + lc
+  f32x16 C[2][TPW];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < 2; ++m)
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        C[m][tt][i] =
-            fields[(size_t)(rbase + 16 * m + 4 * kg + i) * NP + col0 + 16 * tt + lrow];
-  // thresholds of the first kLA blocks (256 items each)
-  for (int it = tid; it < kLA * 256; it += 256) {
+      for (int i = 0; i < 16; ++i)
+        C[m][tt][i] = fields[(size_t)(rbase + 32 * m + 8 * (i >> 2) + 4 * kh + (i & 3)) * NP +
+                             col0 + 32 * tt + lc];
+  // thresholds of the first kTRing blocks (256 items each); block g >= kTRing
+  // is computed by the helpers of step g - kLA
+  for (int it = tid; it < kTRing * 256; it += 256) {
     const int gblk = it >> 8;
     if (gblk < G)
       block_thresholds(it & 255, gblk % NB, gblk / NB, n, rbase, sweep0, k0, k1, tag,
@@ -270,66 +348,75 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   int next = 0;  // D blocks applied to all of this wave's tiles
   bool ok = true;
   int sb_blk = 0;  // block whose operands this wave's sb buffer holds
-  stage_tiles(qpack, NP, 0, col0, 0, TPW, lrow, kg, sb[w]);
+  stage_tiles(qpack, NP, 0, col0, 0, TPW, lc, kh, sb[w]);
   if (w == 0) stage_qd(qdb, 0, lane, qds[0]);
-  for (int g = 0; g < G && ok; ++g) {
+  // step g == G only drains: every wave applies the remaining D blocks
+  for (int g = 0; g <= G && ok; ++g) {
     const int b = g % NB;
-    const int owner = b / TPW;
-    const int owner1 = ((g + 1) % NB) / TPW;  // owner of step g+1
+    const int owner = g < G ? b / BPW : -1;
+    const int owner1 = ((g + 1) % NB) / BPW;  // owner of step g+1
     if (owner == w) {
-      const int i = b % TPW;  // this wave's tile holding block b
-      bf16x8 A[4];
-      const uint4(*bt)[64] = ob[g & 1];  // D_{g-1}'s operands for this wave's tiles
+      const int i = (b >> 1) % TPW;  // this wave's tile holding block b
+      const int half = b & 1;
+      bf16x8 A[2];
+      const uint4(*bt)[2][64] = ob[g & 1];  // D_{g-1}'s operands for this wave's tiles
       if (g > 0) {
         ok = wait_ge(&d_count, g, &abort_flag) &&
              wait_ge(&ob_cnt[g & 1], kHelpers * ((g + 1) >> 1), &abort_flag);
         if (!ok) break;
-        load_a(dring[(g - 1) % kRing], lutp, lutn, lrow, kg, A);
+        load_a(dring[(g - 1) % kRing], lutp, lutn, lc, kh, A);
       }
-      if (g == 0) __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): qds[0] staged above
+      if (g == 0) wait_vm<0>();  // qds[0] staged above
       // (a) bring tile i up to date and transpose it (static indices, uniform branch)
 #pragma unroll
       for (int tt = 0; tt < TPW; ++tt) {
         if (tt != i) continue;
         if (g > 0) {
-          const bf16x8 Bt = as_bf16x8(bt[tt][lane]);
 #pragma unroll
-          for (int m = 0; m < 4; ++m)
-            C[m][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], Bt, C[m][tt], 0, 0, 0);
+          for (int p = 0; p < 2; ++p) {
+            const bf16x8 Bt = as_bf16x8(bt[tt][p][lane]);
+#pragma unroll
+            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+          }
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < 2; ++m)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) fblk[16 * m + 4 * kg + q][lrow] = C[m][tt][q];
+          for (int q = 0; q < 4; ++q)
+            *reinterpret_cast<float4 *>(&fblk[lc][32 * m + 8 * q + 4 * kh]) =
+                make_float4(C[m][tt][4 * q], C[m][tt][4 * q + 1], C[m][tt][4 * q + 2],
+                            C[m][tt][4 * q + 3]);
       }
       wave_sync();
       ok = wait_ge(&thr_cnt[g % kTRing], kHelpers * (g / kTRing + 1), &abort_flag);
       if (!ok) break;
       // ---- sequential decisions of block b, lane = replica ----
       const float(*tb_thr)[17] = thr[g % kTRing];
+      const float *fcol = &fblk[16 * half][lane];
       float c[16];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) c[jj] = 0.0f;
       const uint32_t word = xs[lane][b >> 1];
-      const int sh = (b & 1) * 16;
+      const int sh = half * 16;
       uint32_t xb = (word >> sh) & 0xFFFFu;
       const uint32_t x0 = xb;
       float desum = 0.0f;
       // Software-pipelined: step jj+1's field, threshold, sub-block row (row ==
       // column: the diagonal block of a symmetric Q is symmetric) and the
-      // operand of the tile updated in step jj+1 are read from LDS during step
-      // jj; sched_barrier keeps the compiler from hoisting all 16 steps' reads.
+      // operand of the half tile updated in step jj+1 are read from LDS during
+      // step jj; sched_barrier keeps the compiler from hoisting all 16 steps'
+      // reads.
       const float4 *qdv = reinterpret_cast<const float4 *>(qds[g & 1]);
-      float fn = fblk[lane][0], tn = tb_thr[lane][0];
+      float fn = fcol[0], tn = tb_thr[lane][0];
       float4 qn[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) qn[k] = qdv[k];
-      uint4 bq = bt[0][lane];
+      uint4 bq = bt[0][0][lane];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const float fc = fn, tc = tn;
         const uint4 bcur = bq;
-        if (jj + 1 < TPW) bq = bt[jj + 1][lane];
+        if (jj + 1 < BPW) bq = bt[(jj + 1) >> 1][(jj + 1) & 1][lane];
         float qc[16];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -339,7 +426,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
           qc[4 * k + 3] = qn[k].w;
         }
         if (jj < 15) {
-          fn = fblk[lane][jj + 1];
+          fn = fcol[(jj + 1) * kFT];
           tn = tb_thr[lane][jj + 1];
 #pragma unroll
           for (int k = 0; k < 4; ++k)
@@ -354,15 +441,15 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         desum += acc ? de : 0.0f;
 #pragma unroll
         for (int mm = jj + 1; mm < 16; ++mm) c[mm] = fmaf(qc[mm], d, c[mm]);
-        // (b) D_{g-1} into tile jj of this wave (tile i already has it)
-        if (jj < TPW && g > 0 && jj != i) {
+        // (b) D_{g-1} into half (jj & 1) of tile jj/2 of this wave (tile i
+        //     already has it)
+        if (jj < BPW && g > 0 && (jj >> 1) != i) {
           const bf16x8 Bt = as_bf16x8(bcur);
 #pragma unroll
           for (int tt = 0; tt < TPW; ++tt) {
-            if (tt != jj) continue;
+            if (tt != (jj >> 1)) continue;
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-              C[m][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], Bt, C[m][tt], 0, 0, 0);
+            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -392,57 +479,122 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       }
       if (owner1 != w && g + 1 < G) {
         // end of the turn: this wave applies D_g next, as a non-owner
-        stage_tiles(qpack, NP, b, col0, 0, TPW, lrow, kg, sb[w]);
+        stage_tiles(qpack, NP, b, col0, 0, TPW, lc, kh, sb[w]);
         sb_blk = b;
       }
     } else {
-      const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
+      const bool live = g < G;
+      const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1 (live steps)
       const bool staging = g + 1 < G;
       const int lo = hw * TPW / kHelpers, hi = (hw + 1) * TPW / kHelpers;
-      // (2) thresholds kLA blocks ahead: 256 items, the extra 64 rotate
-      if (g + kLA < G) {
-        const int g2 = g + kLA;
-        block_thresholds(hw * 64 + lane, g2 % NB, g2 / NB, n, rbase, sweep0, k0, k1, tag,
-                         tscale[g2 / NB], thr[g2 % kTRing]);
-        if (hw == g % kHelpers)
-          block_thresholds(192 + lane, g2 % NB, g2 / NB, n, rbase, sweep0, k0, k1, tag,
-                           tscale[g2 / NB], thr[g2 % kTRing]);
-        lds_writes_done();
-        if (lane == 0)
-          __hip_atomic_fetch_add(&thr_cnt[g2 % kTRing], 1, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      // (3) follow the D stream up to D_{g-1}
-      while (next < g) {
-        ok = follow_one<TPW>(qpack, NP, col0, lrow, kg, lane, w, G, dring, lutp, lutn, &d_count,
-                             wnext, &abort_flag, next, sb_blk, sb[w], C);
+      // step g-1 is done: D_{g-1} is published, and the owner has finished
+      // reading ob[(g+1)&1] and threshold slot (g+kLA)%kTRing
+      if (live) {
+        ok = wait_ge(&d_count, g, &abort_flag);
         if (!ok) break;
       }
-      // (4) stage this wave's share of the next owner's operands (block b) in
-      //     ob[(g+1)&1], last read in step g-1, which is complete (this wave
-      //     has seen D_{g-1}); the count is raised once the copies have landed
-      if (staging && ok) {
-        stage_tiles(qpack, NP, b, owner1 * 16 * TPW, lo, hi, lrow, kg, ob[(g + 1) & 1]);
-        if (hw == 0) stage_qd(qdb, (g + 1) % NB, lane, qds[(g + 1) & 1]);
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        if (lane == 0)
-          __hip_atomic_fetch_add(&ob_cnt[(g + 1) & 1], 1, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      // (1) catch up to D_{g-2} (at the drain step: apply everything left)
+      const int target = live ? g - 1 : G;
+      while (ok && next < target)
+        ok = follow_one<TPW>(qpack, NP, col0, lc, kh, lane, w, G, dring, lutp, lutn, &d_count,
+                             wnext, &abort_flag, next, sb_blk, sb[w], C);
+      if (!ok) break;
+      bool restaged = false;  // 2*TPW operand loads issued after the owner's
+      if (live && g > 0) {
+        // (2) D_{g-1} into every tile, with the thresholds of block g+kLA
+        //     computed under the MFMAs and the owner's operands for step g+1
+        //     (block b) copied meanwhile.  After that copy is issued, this
+        //     step touches LDS only through asm, which the compiler does not
+        //     make wait for it.
+        const int blk = next % NB;  // next == g-1
+        if (sb_blk != blk) {
+          stage_tiles(qpack, NP, blk, col0, 0, TPW, lc, kh, sb[w]);
+          sb_blk = blk;
+        }
+        bf16x8 A[2];
+        load_a(dring[next % kRing], lutp, lutn, lc, kh, A);
+        wait_vm<0>();  // this wave's operands have landed
+        if (staging) {
+          stage_tiles(qpack, NP, b, owner1 * 32 * TPW, lo, hi, lc, kh, ob[(g + 1) & 1]);
+          if (hw == kHelpers - 1) stage_qd(qdb, (g + 1) % NB, lane, qds[(g + 1) & 1]);
+        }
+        const int g2 = min(g + kLA, G - 1);
+        const float ts2 = tscale[g2 / NB];
+        // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
+        // the 10 Philox rounds and 4 logarithms are spread over the operand
+        // reads, so each piece of VALU work runs while two MFMAs execute
+        const int item = hw * 64 + lane;
+        const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
+        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rbase + (uint32_t)(item >> 2),
+                              (uint32_t)(sweep0 + g2 / NB), tag);
+        uint32_t pk0 = k0, pk1 = k1;
+        float tv[4];
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int tt = 0; tt < TPW; ++tt) {
+            const bf16x8 Bt = as_bf16x8(lds_read16(&sb[w][tt][p][lane]));
+#pragma unroll
+            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+            const int slot = p * TPW + tt;
+#pragma unroll
+            for (int u = 0; u < 14; ++u) {
+              if ((u * 2 * TPW) / 14 != slot) continue;
+              if (u < 10) {
+                philox_round(ph, pk0, pk1, u);
+              } else {
+                const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
+                tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
+              }
+            }
+          }
+        // (a slot that no later block uses may receive values past the end)
+        const int sl = (g + kLA) % kTRing;
+        store_thresholds_asm(item, tv, thr[sl]);
+        ++next;
+        if (lane == 0) lds_store(&wnext[w], next);
+        if (g + kLA < G) {
+          if (hw == g % kHelpers) {  // the fourth quarter rotates
+            item_thresholds(192 + lane, g2 % NB, g2 / NB, n, rbase, sweep0, k0, k1, tag, ts2,
+                            tv);
+            store_thresholds_asm(192 + lane, tv, thr[sl]);
+          }
+          if (lane == 0) lds_add(&thr_cnt[sl]);
+        }
+        // stage D_g's operands unless this wave owns step g+1 (it then applies
+        // D_g from the owner's area)
+        const bool own_next = next + 1 < G && ((next + 1) % NB) / BPW == w;
+        if (next < G && !own_next) {
+          stage_tiles(qpack, NP, next % NB, col0, 0, TPW, lc, kh, sb[w]);
+          sb_blk = next % NB;
+          restaged = true;
+        } else {
+          sb_blk = -1;
+        }
+      } else if (staging) {
+        stage_tiles(qpack, NP, b, owner1 * 32 * TPW, lo, hi, lc, kh, ob[(g + 1) & 1]);
+        if (hw == kHelpers - 1) stage_qd(qdb, (g + 1) % NB, lane, qds[(g + 1) & 1]);
+      }
+      // (3) the owner's operands have landed once at most the loads issued
+      //     after them are outstanding
+      if (staging) {
+        if (restaged)
+          wait_vm<2 * TPW>();
+        else
+          wait_vm<0>();
+        if (lane == 0) lds_add(&ob_cnt[(g + 1) & 1]);
       }
     }
   }
-  // drain: apply the remaining D blocks
-  while (ok && next < G)
-    ok = follow_one<TPW>(qpack, NP, col0, lrow, kg, lane, w, G, dring, lutp, lutn, &d_count, wnext,
-                         &abort_flag, next, sb_blk, sb[w], C);
   if (!ok && lane == 0) lds_store(&abort_flag, 1);
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
+  for (int m = 0; m < 2; ++m)
 #pragma unroll
     for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-        fields[(size_t)(rbase + 16 * m + 4 * kg + i) * NP + col0 + 16 * tt + lrow] = C[m][tt][i];
+      for (int i = 0; i < 16; ++i)
+        fields[(size_t)(rbase + 32 * m + 8 * (i >> 2) + 4 * kh + (i & 3)) * NP + col0 + 32 * tt +
+               lc] = C[m][tt][i];
   __syncthreads();
   for (int i = tid; i < kRT * NPW; i += 256)
     xbits[(size_t)(rbase + i / NPW) * NPW + i % NPW] = xs[i / NPW][i % NPW];
@@ -491,16 +643,16 @@ __global__ __launch_bounds__(256) void warm_init_kernel(int n, int npw, int R, u
   xbits[idx] = v;
 }
 
-int dense_max_n() { return kWaves * 16 * 16; }
+int dense_max_n() { return kWaves * 32 * 8; }
 
-// Tiles per wave are instantiated for these counts; n is padded up to the
-// next one (64 columns per tile count step).
-static const int kTpwList[] = {1, 2, 4, 6, 8, 12, 16};
+// Column tiles (32 columns) per wave are instantiated for these counts; n is
+// padded up to the next one (128 columns per step).
+static const int kTpwList[] = {1, 2, 3, 4, 6, 8};
 
 int dense_padded_n(int n) {
-  const int need = (n + kWaves * 16 - 1) / (kWaves * 16);
+  const int need = (n + kWaves * 32 - 1) / (kWaves * 32);
   for (int t : kTpwList)
-    if (t >= need) return t * kWaves * 16;
+    if (t >= need) return t * kWaves * 32;
   return -1;
 }
 
@@ -511,7 +663,7 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, const uint4 *qpack, cons
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const uint32_t tag = (instance << 8) | TAG_SWEEP_THR;
   const dim3 grid(R_pad / kRT), block(kWaves * 64);
-  switch (np / (kWaves * 16)) {
+  switch (np / (kWaves * 32)) {
 #define OS_DENSE_CASE(T)                                                                    \
   case T:                                                                                   \
     dense_sweep_kernel<T><<<grid, block, 0, st>>>(n, qpack, qdb, tscale, sweep0, nsweeps, k0, \
@@ -520,11 +672,10 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, const uint4 *qpack, cons
     break;
     OS_DENSE_CASE(1)
     OS_DENSE_CASE(2)
+    OS_DENSE_CASE(3)
     OS_DENSE_CASE(4)
     OS_DENSE_CASE(6)
     OS_DENSE_CASE(8)
-    OS_DENSE_CASE(12)
-    OS_DENSE_CASE(16)
 #undef OS_DENSE_CASE
     default:
       throw Error(-3, "dense sweep kernel: unsupported padded size");
