@@ -121,6 +121,45 @@ int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats);
 
+/* ---- batched sparse instances (CSR) ------------------------------------
+ * Replaces, for many sparse instances at once, the loop a caller would write
+ * around sa::anneal (include/simulated_annealing/annealing.hpp:55-140) and the
+ * flatten of helpers::flatten_qubo (include/helpers/qubo_helpers.hpp:26-44):
+ * the couplings stay sparse.  Instances are concatenated: instance i has n[i]
+ * spins, n[i]+1 row_ptr entries (starting at 0, local offsets), row_ptr[n[i]]
+ * col/val entries (local column indices, both (j,l) and (l,j) stored, i.e.
+ * the symmetric off-diagonal part of the reference's flat matrix) and n[i]
+ * diag entries.  Sweeps compute in fp32 (weights rounded from FP64); returned
+ * energies are FP64 recomputations with the FP64 input. */
+typedef struct os_csr_batch os_csr_batch;
+
+typedef struct os_csr_params {
+  uint32_t num_replicas;       /* replicas per instance */
+  int32_t num_sweeps;          /* sweeps per replica */
+  const float *tscale;         /* [num_sweeps] acceptance scale per sweep (1/beta) */
+  uint64_t seed;               /* Philox key */
+  uint32_t instance_base;      /* stream id of instance 0; instance i uses base + i (< 2^24) */
+  const uint8_t *init_states;  /* optional start states, per instance [num_replicas x n[i]] */
+  int32_t use_graph;           /* replay the run from a captured hipGraph */
+  int32_t tune;                /* pick the launch shape by timing candidates on first use */
+} os_csr_params;
+
+int os_csr_batch_create(os_ctx *ctx, int32_t num_instances, const int32_t *n,
+                        const int32_t *row_ptr, const int32_t *col, const double *val,
+                        const double *diag, os_csr_batch **out);
+int os_csr_batch_destroy(os_csr_batch *batch);
+/* All outputs optional.  best_energies / run_best_energies: [num_instances x
+ * num_replicas] FP64 energy of each best state / the sweep kernel's running
+ * best; best_states / final_states: per instance [num_replicas x n[i]];
+ * inst_best_energy / inst_best_replica: [num_instances] (first replica on
+ * ties). */
+int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
+                        double *best_energies, double *run_best_energies, uint8_t *best_states,
+                        uint8_t *final_states, double *inst_best_energy,
+                        int32_t *inst_best_replica, os_sweep_stats *stats);
+/* The launch shape in use (0 before the first run). */
+int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups);
+
 #ifdef __cplusplus
 }
 #endif

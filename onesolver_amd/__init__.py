@@ -210,3 +210,98 @@ class DenseProblem:
                                      ctypes.byref(st)))
         return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
                 "best_energy": e.value, "stats": st.as_dict()}
+
+
+class CsrParams(ctypes.Structure):
+    """os_csr_params of include/onesolver.h."""
+    _fields_ = [("num_replicas", ctypes.c_uint32), ("num_sweeps", ctypes.c_int32),
+                ("tscale", _vp), ("seed", ctypes.c_uint64), ("instance_base", ctypes.c_uint32),
+                ("init_states", _vp), ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32)]
+
+
+def _csr_lib():
+    L = lib()
+    if not getattr(L, "_csr_ready", False):
+        L.os_csr_batch_create.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                          ctypes.POINTER(_vp)]
+        L.os_csr_batch_destroy.argtypes = [_vp]
+        L.os_csr_batch_anneal.argtypes = [_vp, ctypes.POINTER(CsrParams), _f64p, _f64p, _u8p,
+                                          _u8p, _f64p, _i32p, ctypes.POINTER(SweepStats)]
+        L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p]
+        L._csr_ready = True
+    return L
+
+
+class CsrBatch:
+    """A batch of sparse QUBOs resident on the GPU (os_csr_batch).
+
+    ``instances``: sequence of (row_ptr, col, val, diag) with the symmetric
+    off-diagonal couplings in CSR (both (j,l) and (l,j)) and the diagonal."""
+
+    def __init__(self, ctx: Context, instances):
+        self.ctx = ctx
+        self.ns = np.array([len(d) for (_, _, _, d) in instances], dtype=np.int32)
+        rp = np.concatenate([np.asarray(r, dtype=np.int32) for (r, _, _, _) in instances])
+        cl = np.concatenate([np.asarray(c, dtype=np.int32) for (_, c, _, _) in instances])
+        vl = np.concatenate([np.asarray(v, dtype=np.float64) for (_, _, v, _) in instances])
+        dg = np.concatenate([np.asarray(d, dtype=np.float64) for (_, _, _, d) in instances])
+        rp, cl, vl, dg = (np.ascontiguousarray(a) for a in (rp, cl, vl, dg))
+        h = _vp()
+        _check(_csr_lib().os_csr_batch_create(ctx.handle, len(self.ns), _p(self.ns, _i32p),
+                                              _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f64p),
+                                              _p(dg, _f64p), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _csr_lib().os_csr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def launch_shape(self):
+        w, g = ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(_csr_lib().os_csr_batch_launch_shape(self._h, ctypes.byref(w), ctypes.byref(g)))
+        return w.value, g.value
+
+    def anneal(self, num_replicas, tscale, seed=1, instance_base=0, init_states=None,
+               best_states=True, final_states=False, use_graph=True, tune=True):
+        """Returns per-instance lists of best/final states ([R, n_i] uint8) and
+        [I, R] arrays of FP64 best energies and running-best energies."""
+        ts = np.ascontiguousarray(tscale, dtype=np.float32)
+        I, R = len(self.ns), int(num_replicas)
+        ini = None
+        if init_states is not None:
+            ini = np.ascontiguousarray(np.concatenate(
+                [np.asarray(s, dtype=np.uint8).reshape(R, n).ravel()
+                 for s, n in zip(init_states, self.ns)]))
+        prm = CsrParams(R, len(ts), ts.ctypes.data, seed, instance_base,
+                        None if ini is None else ini.ctypes.data, int(use_graph), int(tune))
+        tot = int(self.ns.sum()) * R
+        be = np.zeros((I, R), dtype=np.float64)
+        rb = np.zeros((I, R), dtype=np.float64)
+        bs = np.zeros(tot, dtype=np.uint8) if best_states else None
+        fs = np.zeros(tot, dtype=np.uint8) if final_states else None
+        ib = np.zeros(I, dtype=np.float64)
+        ir = np.zeros(I, dtype=np.int32)
+        st = SweepStats()
+        _check(_csr_lib().os_csr_batch_anneal(self._h, ctypes.byref(prm), _p(be, _f64p),
+                                              _p(rb, _f64p), _p(bs, _u8p), _p(fs, _u8p),
+                                              _p(ib, _f64p), _p(ir, _i32p), ctypes.byref(st)))
+
+        def split(flat):
+            if flat is None:
+                return None
+            out, at = [], 0
+            for n in self.ns:
+                out.append(flat[at:at + R * n].reshape(R, n))
+                at += R * n
+            return out
+
+        return {"best_energies": be, "run_best_energies": rb, "best_states": split(bs),
+                "final_states": split(fs), "inst_best_energy": ib, "inst_best_replica": ir,
+                "stats": st.as_dict()}

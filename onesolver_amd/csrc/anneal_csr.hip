@@ -1,0 +1,241 @@
+// anneal_csr.hip -- batched sweep-mode annealing of sparse QUBOs (gfx950).
+//
+// One persistent kernel anneals a whole batch of instances of uneven sizes.
+// Work items are (instance, group of 64 replicas); waves take items from a
+// global counter in the order the host gives (largest first), so one launch
+// drains the batch whatever its size mix.  A wave runs its 64 replicas in
+// lockstep, lane = replica: the site loop is uniform, so the coupling row of
+// site j (ELL layout: `width` columns padded with the spin index n, whose bit
+// is always 0, and their weights) is read once per wave with scalar loads and
+// shared by all 64 replicas, and each lane gathers its own neighbour bits
+// from LDS, where the states are stored bit-packed and transposed
+// ([word][replica], conflict-free for a wave).
+//
+// Per site (reference semantics of a Metropolis sweep, in the oracle's
+// order): h = diag_j, then h = h + w_jk for each set neighbour in row order
+// (fp32, no contraction); dE = x_j ? -h : h; flip iff dE < neg_ln_u(w) *
+// tscale[t], w = Philox word of counter (j>>2, replica, sweep, tag); the
+// FP64 running energy adds dE.  At the end of every sweep a replica whose
+// running energy is below its best copies its state to the best-state planes.
+#include "os_kernels.h"
+
+namespace os {
+
+namespace {
+
+// Bit j of replica lane in a wave's transposed planes.
+__device__ __forceinline__ uint32_t plane_bit(const uint32_t *xs, int j, int lane) {
+  return (xs[(j >> 5) * 64 + lane] >> (j & 31)) & 1u;
+}
+
+__device__ __forceinline__ float row_field(const uint32_t *xs, int lane, const int4 *ecol,
+                                           const float4 *eval, int chunks, float h) {
+  for (int c = 0; c < chunks; ++c) {
+    const int4 cc = ecol[c];
+    const float4 vv = eval[c];
+    h = plane_bit(xs, cc.x, lane) ? h + vv.x : h;
+    h = plane_bit(xs, cc.y, lane) ? h + vv.y : h;
+    h = plane_bit(xs, cc.z, lane) ? h + vv.z : h;
+    h = plane_bit(xs, cc.w, lane) ? h + vv.w : h;
+  }
+  return h;
+}
+
+}  // namespace
+
+__global__ void csr_queue_reset_kernel(int *queue) {
+  if (threadIdx.x == 0) queue[0] = 0;
+}
+
+template <int kW>
+__global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
+    const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
+    const float4 *__restrict__ ell_val, const float *__restrict__ diag,
+    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
+    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
+    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
+    int max_words) {
+  extern __shared__ uint32_t lds[];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  uint32_t *xs = lds + (size_t)w * max_words * 64;
+  const int num_items = run->num_items, S = run->num_sweeps, Rp = run->Rp;
+  const uint32_t k0 = run->k0, k1 = run->k1;
+  const uint32_t base = run->instance_base;
+  const int init_mode = run->init_mode;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= num_items) break;
+    const int2 wi = items[it];
+    const CsrInst d = inst[wi.x];
+    const int n = d.n, words = d.words, chunks = d.width >> 2;
+    const uint32_t r = (uint32_t)(wi.y * 64 + lane);
+    const uint32_t id = (base + (uint32_t)wi.x) << 8;
+    const int4 *ecol = ell_col + (d.ell_off >> 2);
+    const float4 *eval = ell_val + (d.ell_off >> 2);
+    const float *dg = diag + d.spin_off;
+    uint32_t *bb = best_bits + d.state_off + r;
+    // start state: Philox bits (spin j: counter (j>>7, r, 0, init tag), word
+    // (j>>5)&3, bit j&31) or the caller's planes; bits >= n are zero
+    for (int wd = 0; wd < words; ++wd) {
+      uint32_t v;
+      if (init_mode) {
+        v = init_bits[d.state_off + (size_t)wd * Rp + r];
+      } else {
+        const uint4 o = philox4x32_10(make_uint4((uint32_t)(wd >> 2), r, 0u, id | TAG_SWEEP_INIT),
+                                      k0, k1);
+        v = pick(o, (uint32_t)wd & 3u);
+      }
+      const int valid = n - 32 * wd;
+      if (valid < 32) v = valid <= 0 ? 0u : (v & ((1u << valid) - 1u));
+      xs[wd * 64 + lane] = v;
+    }
+    // FP64 energy in the oracle's order: sum over set j of diag_j + off_j / 2
+    double e = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double off = 0.0;
+      for (int c = 0; c < chunks; ++c) {
+        const int4 cc = ecol[(size_t)j * chunks + c];
+        const float4 vv = eval[(size_t)j * chunks + c];
+        off = plane_bit(xs, cc.x, lane) ? off + (double)vv.x : off;
+        off = plane_bit(xs, cc.y, lane) ? off + (double)vv.y : off;
+        off = plane_bit(xs, cc.z, lane) ? off + (double)vv.z : off;
+        off = plane_bit(xs, cc.w, lane) ? off + (double)vv.w : off;
+      }
+      if (plane_bit(xs, j, lane)) e += (double)dg[j] + 0.5 * off;
+    }
+    double eb = e;
+    for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+    for (int t = 0; t < S; ++t) {
+      const float ts = tscale[t];
+      for (int j4 = 0; j4 < n; j4 += 4) {
+        const uint4 o = philox4x32_10(
+            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j4 + q;
+          if (j >= n) break;
+          const float thr = neg_ln_u(pick(o, (uint32_t)q)) * ts;
+          const float h = row_field(xs, lane, ecol + (size_t)j * chunks,
+                                    eval + (size_t)j * chunks, chunks, dg[j]);
+          uint32_t *xw = &xs[(j >> 5) * 64 + lane];
+          const uint32_t word = *xw;
+          const bool xj = (word >> (j & 31)) & 1u;
+          const float de = xj ? -h : h;
+          if (de < thr) {
+            *xw = word ^ (1u << (j & 31));
+            e += (double)de;
+          }
+        }
+      }
+      if (e < eb) {
+        eb = e;
+        for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+      }
+    }
+    uint32_t *fb = final_bits + d.state_off + r;
+    for (int wd = 0; wd < words; ++wd) fb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+    e_best[d.rep_off + r] = eb;
+    e_final[d.rep_off + r] = e;
+  }
+}
+
+// FP64 energy of states given as planes: E = sum_j x_j (Q_jj + sum_{l>j} Q_jl x_l),
+// the reference's energy function (upper triangle incl. diagonal), with the
+// double-precision couplings of the CSR input.  Thread = (item, lane).
+__global__ __launch_bounds__(64) void csr_energy_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items, int num_items, int Rp,
+    const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
+    const double *__restrict__ val, const double *__restrict__ diag64,
+    const uint32_t *__restrict__ bits, double *__restrict__ out) {
+  const int it = blockIdx.x;
+  if (it >= num_items) return;
+  const int lane = threadIdx.x;
+  const int2 wi = items[it];
+  const CsrInst d = inst[wi.x];
+  const uint32_t r = (uint32_t)(wi.y * 64 + lane);
+  const uint32_t *x = bits + d.state_off + r;
+  const int32_t *rp = row_ptr + d.row_off;
+  const int32_t *cl = col + d.csr_off;
+  const double *vl = val + d.csr_off;
+  const double *dg = diag64 + d.spin_off;
+  double e = 0.0;
+  for (int j = 0; j < d.n; ++j) {
+    if (!((x[(size_t)(j >> 5) * Rp] >> (j & 31)) & 1u)) continue;
+    double s = dg[j];
+    for (int k = rp[j]; k < rp[j + 1]; ++k) {
+      const int l = cl[k];
+      if (l > j && ((x[(size_t)(l >> 5) * Rp] >> (l & 31)) & 1u)) s += vl[k];
+    }
+    e += s;
+  }
+  out[d.rep_off + r] = e;
+}
+
+int csr_lds_bytes(int waves, int max_words) { return waves * max_words * 64 * 4; }
+
+void launch_csr_queue_reset(hipStream_t st, int *queue) {
+  csr_queue_reset_kernel<<<1, 64, 0, st>>>(queue);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_csr_sweeps(hipStream_t st, int waves, int groups, int max_words, const CsrInst *inst,
+                       const int2 *items, const CsrRun *run, const int4 *ell_col,
+                       const float4 *ell_val, const float *diag, const float *tscale,
+                       const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
+                       double *e_best, double *e_final, int *queue) {
+  const size_t lds = (size_t)csr_lds_bytes(waves, max_words);
+  switch (waves) {
+#define OS_CSR_CASE(W)                                                                       \
+  case W:                                                                                    \
+    if (lds > 65536)                                                                         \
+      OS_HIP_CHECK(hipFuncSetAttribute((const void *)csr_sweep_kernel<W>,                    \
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+    csr_sweep_kernel<W><<<groups, 64 * W, lds, st>>>(inst, items, run, ell_col, ell_val, diag,  \
+                                                     tscale, init_bits, best_bits, final_bits,  \
+                                                     e_best, e_final, queue, max_words);        \
+    break;
+    OS_CSR_CASE(1)
+    OS_CSR_CASE(2)
+    OS_CSR_CASE(4)
+#undef OS_CSR_CASE
+    default:
+      throw Error(-3, "csr sweep kernel: unsupported waves per group");
+  }
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+int csr_max_groups_per_cu(int waves, int max_words) {
+  const size_t lds = (size_t)csr_lds_bytes(waves, max_words);
+  int blocks = 0;
+  hipError_t e = hipSuccess;
+  switch (waves) {
+    case 1:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_kernel<1>, 64, lds);
+      break;
+    case 2:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_kernel<2>, 128, lds);
+      break;
+    case 4:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_kernel<4>, 256, lds);
+      break;
+    default:
+      throw Error(-3, "csr sweep kernel: unsupported waves per group");
+  }
+  OS_HIP_CHECK(e);
+  return blocks;
+}
+
+void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                       int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
+                       const double *diag64, const uint32_t *bits, double *out) {
+  if (num_items == 0) return;
+  csr_energy_kernel<<<num_items, 64, 0, st>>>(inst, items, num_items, Rp, row_ptr, col, val,
+                                             diag64, bits, out);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace os

@@ -37,4 +37,41 @@ void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uin
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
                       const float *p, uint32_t *xbits);
 
+// anneal_csr.hip
+// Per-instance descriptor of a sparse batch (offsets into the batch arrays).
+struct CsrInst {
+  int32_t n;          // spins
+  int32_t words;      // ceil((n + 1) / 32) state words (bit n: the ELL padding target)
+  int32_t width;      // ELL row width (multiple of 4)
+  int32_t pad_;
+  int64_t ell_off;    // first ELL entry (multiple of 4)
+  int64_t spin_off;   // first spin in diag / diag64
+  int64_t row_off;    // first entry in row_ptr (n + 1 entries per instance)
+  int64_t csr_off;    // first entry in col / val
+  int64_t state_off;  // first word of the instance's [words][Rp] state planes
+  int64_t rep_off;    // first replica in per-replica arrays (Rp per instance)
+};
+// Parameters of one batched run, read by the kernel from device memory so a
+// captured graph replays with new values.
+struct CsrRun {
+  int32_t num_items;  // work items (instance, 64-replica group)
+  int32_t num_sweeps;
+  int32_t Rp;         // replicas per instance, padded to a multiple of 64
+  int32_t init_mode;  // 0: Philox start states, 1: caller's planes
+  uint32_t k0, k1;    // Philox key
+  uint32_t instance_base;
+  int32_t pad_;
+};
+int csr_lds_bytes(int waves, int max_words);
+int csr_max_groups_per_cu(int waves, int max_words);
+void launch_csr_queue_reset(hipStream_t st, int *queue);
+void launch_csr_sweeps(hipStream_t st, int waves, int groups, int max_words, const CsrInst *inst,
+                       const int2 *items, const CsrRun *run, const int4 *ell_col,
+                       const float4 *ell_val, const float *diag, const float *tscale,
+                       const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
+                       double *e_best, double *e_final, int *queue);
+void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                       int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
+                       const double *diag64, const uint32_t *bits, double *out);
+
 }  // namespace os

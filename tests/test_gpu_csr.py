@@ -1,0 +1,154 @@
+"""GPU parity of the batched sparse (CSR) annealer.
+
+The sweep kernel computes each local field as diag_j followed by the set
+neighbours' weights in row order in fp32, exactly as the oracle's
+or_sweep_csr does, and draws the same Philox thresholds, so final states,
+best states and the running best energies must equal the oracle's bit for
+bit on every instance of a batch, whatever its size mix.  The FP64 energies
+of the returned states must match the energy function within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import onesolver_amd
+import oracle
+from onesolver_amd import instances
+
+pytestmark = pytest.mark.gpu
+
+
+def _tscale(S, bmin=0.1, bmax=3.0):
+    return onesolver_amd.tscale_from_beta(np.geomspace(bmin, bmax, S))
+
+
+def _irregular(n, seed):
+    """Symmetric CSR with isolated spins, a hub of high degree and real weights."""
+    rng = np.random.default_rng(seed)
+    pairs = set()
+    hub = n // 2
+    for l in rng.choice(n, size=min(n - 1, 13), replace=False):
+        if l != hub:
+            pairs.add((min(hub, l), max(hub, l)))
+    for _ in range(n):
+        a, b = rng.integers(0, n, 2)
+        if a != b and a % 7 and b % 7:  # spins 0, 7, 14, ... stay isolated except the hub
+            pairs.add((min(a, b), max(a, b)))
+    rows, cols, vals = [], [], []
+    for (a, b) in sorted(pairs):
+        w = float(np.float32(rng.normal()))
+        rows += [a, b]
+        cols += [b, a]
+        vals += [w, w]
+    rows, cols, vals = np.array(rows), np.array(cols), np.array(vals)
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    rp = np.zeros(n + 1, dtype=np.int32)
+    np.add.at(rp, rows + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    diag = (rng.random(n, dtype=np.float32) - np.float32(0.5)).astype(np.float64)
+    return rp, cols.astype(np.int32), vals.astype(np.float64), diag
+
+
+def _batch():
+    insts = [instances.regular_csr(n, d, seed=5, instance=i)
+             for i, (n, d) in enumerate([(6, 3), (32, 5), (33, 4), (64, 8), (100, 3), (258, 7)])]
+    insts.append(_irregular(70, 1))
+    insts.append(_irregular(31, 2))
+    return insts
+
+
+def _check_against_oracle(out, insts, ts, R, seed, base, init=None):
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        ini = None if init is None else init[i]
+        fin, bs, be = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=seed, instance=base + i,
+                                       init=ini)
+        assert (out["final_states"][i] == fin).all(), f"final states differ, instance {i}"
+        assert (out["best_states"][i] == bs).all(), f"best states differ, instance {i}"
+        assert (out["run_best_energies"][i] == be).all(), f"running best differs, instance {i}"
+        e64 = oracle.csr_energy(rp, cl, vl, dg, bs)
+        np.testing.assert_allclose(out["best_energies"][i], e64, rtol=1e-9, atol=1e-12)
+        r = int(np.argmin(e64))
+        assert out["inst_best_replica"][i] == r
+        assert out["inst_best_energy"][i] == out["best_energies"][i][r]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_csr_batch_matches_oracle_bit_for_bit(ctx, use_graph):
+    insts = _batch()
+    R, S = 100, 25
+    ts = _tscale(S)
+    print(f"[gpu] csr batch of {len(insts)} R={R} S={S} graph={use_graph}", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=11, instance_base=4, final_states=True, use_graph=use_graph)
+    _check_against_oracle(out, insts, ts, R, 11, 4)
+    w, g = b.launch_shape()
+    assert w in (1, 2, 4) and g >= 1
+    b.close()
+
+
+def test_csr_graph_replays_with_new_parameters(ctx):
+    insts = _batch()
+    R, S = 64, 12
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    ts = _tscale(S)
+    first = b.anneal(R, ts, seed=3, final_states=True)
+    ts2 = _tscale(S, 0.5, 5.0)
+    print("[gpu] csr graph replay, new seed and schedule", flush=True)
+    second = b.anneal(R, ts2, seed=4, final_states=True)
+    _check_against_oracle(second, insts, ts2, R, 4, 0)
+    third = b.anneal(R, ts, seed=3, final_states=True)
+    for k in ("best_energies", "run_best_energies"):
+        assert (first[k] == third[k]).all()
+    for a, c in zip(first["final_states"], third["final_states"]):
+        assert (a == c).all()
+    b.close()
+
+
+def test_csr_start_states(ctx):
+    insts = _batch()
+    R, S = 70, 9
+    ts = _tscale(S)
+    rng = np.random.default_rng(0)
+    init = [rng.integers(0, 2, (R, len(d)), dtype=np.uint8) for (_, _, _, d) in insts]
+    print("[gpu] csr start states", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=2, init_states=init, final_states=True)
+    _check_against_oracle(out, insts, ts, R, 2, 0, init=init)
+    # zero sweeps: the best state is the start state
+    z = b.anneal(R, ts[:0], seed=2, init_states=init, final_states=True)
+    for i in range(len(insts)):
+        assert (z["best_states"][i] == init[i]).all()
+    b.close()
+
+
+def test_csr_c3_batch_energies(ctx):
+    """The 512-instance batch (n in [64, 2048], d in 3..8) at a few sweeps:
+    every returned energy is the FP64 energy of its state, and one instance is
+    checked against the oracle bit for bit."""
+    insts = instances.batch_c3(512)
+    R, S = 256, 4
+    ts = _tscale(S)
+    print(f"[gpu] csr C3 batch R={R} S={S}", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=1)
+    for i in range(0, 512, 37):
+        rp, cl, vl, dg = insts[i]
+        e64 = oracle.csr_energy(rp, cl, vl, dg, out["best_states"][i])
+        np.testing.assert_allclose(out["best_energies"][i], e64, rtol=1e-9, atol=1e-12)
+    i = int(np.argmax([len(d) for (_, _, _, d) in insts]))
+    rp, cl, vl, dg = insts[i]
+    fin, bs, be = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=1, instance=i)
+    assert (out["best_states"][i] == bs).all()
+    assert (out["run_best_energies"][i] == be).all()
+    b.close()
+
+
+def test_csr_rejects_bad_input(ctx):
+    rp = np.array([0, 1, 2], dtype=np.int32)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        onesolver_amd.CsrBatch(ctx, [(rp, np.array([1, 5], dtype=np.int32),
+                                      np.array([1.0, 1.0]), np.zeros(2))])
+    with pytest.raises(onesolver_amd.OneSolverError):
+        onesolver_amd.CsrBatch(ctx, [(np.array([1, 1, 2], dtype=np.int32),
+                                      np.array([1, 0], dtype=np.int32), np.array([1.0, 1.0]),
+                                      np.zeros(2))])
