@@ -154,12 +154,91 @@ def run_reference_c1(args):
     }, q
 
 
+def cpu_baseline_csr(inst, ts_full, budget_s=10.0):
+    """The sparse sweep restated on the CPU (oracle.sweep_csr, the same
+    fp32 gather per site), all host cores over replicas, on a bounded sample:
+    one instance of the batch, a few sweeps."""
+    import oracle
+
+    rp, cl, vl, dg = inst
+    threads = os.cpu_count() or 1
+    try:
+        threads = min(threads, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    reps = max(threads, 1) * 8
+    t0 = time.perf_counter()
+    oracle.sweep_csr(rp, cl, vl, dg, ts_full[:2], reps, threads=threads)
+    per = (time.perf_counter() - t0) / 2
+    S = int(max(2, min(len(ts_full), budget_s / max(per, 1e-9))))
+    t0 = time.perf_counter()
+    oracle.sweep_csr(rp, cl, vl, dg, ts_full[:S], reps, threads=threads)
+    el = time.perf_counter() - t0
+    n = len(dg)
+    return {"value": reps * S * n / el, "unit": "attempts/s", "cores": threads, "kind": "port",
+            "sample": f"sparse sweep restated (fp32 gather per site), instance n={n} "
+                      f"d={int(np.diff(rp).max())}, {reps} replicas x {S} sweeps, {el:.1f} s"}
+
+
+def run_csr_c3(args):
+    """Config 3: 512 sparse d-regular instances, n log-uniform in [64, 2048],
+    256 replicas each, 2048 sweeps, one persistent kernel replayed from a
+    hipGraph with the tuned launch shape."""
+    import onesolver_amd
+    from onesolver_amd import instances
+
+    insts = instances.batch_c3(512)
+    R, S = 256, args.sweeps_c3
+    beta = np.geomspace(0.1, 3.0, S)
+    ts = onesolver_amd.tscale_from_beta(beta)
+    ctx = onesolver_amd.Context(0)
+    log(f"device {ctx.device_name()}")
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    for i in range(args.warmup):
+        out = b.anneal(R, ts, seed=1234, best_states=False)
+        log(f"warmup {i}: graph {out['stats']['kernel_ms']:.1f} ms shape {b.launch_shape()}")
+    kernel_ms = 0.0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = b.anneal(R, ts, seed=1234 + i, best_states=False)
+        kernel_ms += out["stats"]["kernel_ms"]
+        log(f"step {i}: graph {out['stats']['kernel_ms']:.1f} ms")
+    elapsed = time.perf_counter() - t0
+    ns = np.array([len(d) for (_, _, _, d) in insts], dtype=np.float64)
+    deg = np.array([np.diff(rp).mean() for (rp, _, _, _) in insts])
+    attempts = float(ns.sum()) * R * S * args.steps
+    bytes_alg = float((ns * (8.0 * deg + 8.0)).sum()) * R * S  # per launch
+    avg_launch_s = kernel_ms / args.steps / 1e3
+    achieved = bytes_alg / avg_launch_s / 1e9
+    res = {
+        "metric": "spin-flip attempts/s", "value": attempts / elapsed, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 fields, fp64 energies",
+        "data": "synthetic (seeded random d-regular graphs)",
+        "config": {"workload": f"batched sparse: 512 instances n in [64,2048] log-uniform, "
+                               f"d-regular d in 3..8, weights +-1, {R} replicas, {S} sweeps, "
+                               f"persistent kernel + hipGraph", "instances": 512,
+                   "replicas": R, "sweeps": S, "sum_n": int(ns.sum()),
+                   "launch_shape": list(b.launch_shape())},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "csr_sweep_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_alg},
+    }
+    imax = int(np.argmax(ns))
+    b.close()
+    ctx.close()
+    return res, (insts[imax], ts)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3"])
+    ap.add_argument("--sweeps-c3", type=int, default=2048)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--replicas", type=int, default=16384)
     ap.add_argument("--sweeps", type=int, default=4096)
@@ -170,6 +249,13 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(replicas would shard across GPUs with one final min-reduction)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        res, (inst, ts) = run_csr_c3(args)
+        if not args.no_cpu_baseline:
+            log("timing the CPU baseline (bounded sample)")
+            res["cpu_baseline"] = cpu_baseline_csr(inst, ts)
+        print(json.dumps(res))
+        return
     if args.workload == "c1":
         res, q = run_reference_c1(args)
     else:

@@ -45,6 +45,9 @@ Ctx::Ctx(int dev) : device(dev) {
   name = prop.name;
   arch = prop.gcnArchName;
   num_cus = prop.multiProcessorCount;
+  // the marketing name can be empty (e.g. without access to the board's
+  // product data): name the device by its architecture and CU count then
+  if (name.empty()) name = arch + " (" + std::to_string(num_cus) + " CUs)";
   OS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   OS_HIP_CHECK(hipEventCreate(&ev_start));
   OS_HIP_CHECK(hipEventCreate(&ev_stop));
