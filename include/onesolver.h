@@ -112,11 +112,28 @@ typedef struct os_dense os_dense;
  * matrix) for sweep-mode annealing; keeps Q as FP64 and as bf16 hi+lo pairs
  * on the GPU.  n <= 1024 in this version. */
 int os_dense_create(os_ctx *ctx, int n, const double *q_sym, os_dense **out);
+/* A batch of num_instances same-size dense QUBOs (q_sym: num_instances
+ * consecutive n x n matrices), annealed together in one launch per segment;
+ * instance i draws its random numbers as stream instance + i.  No reference
+ * counterpart: the reference anneals one model per sa::anneal call
+ * (include/simulated_annealing/annealing.hpp:55-58). */
+int os_dense_batch_create(os_ctx *ctx, int num_instances, int n, const double *q_sym,
+                          os_dense **out);
+/* A batch of num_instances SK instances of the dense benchmark distribution
+ * (J_ij ~ N(0,1)/sqrt(n) fp32, h = 0, as a QUBO: Q_ij = 4 J_ij, Q_ii =
+ * -2 sum_j J_ij), generated on the GPU from Philox key (seed, instance_base +
+ * i): no host generation or transfer.  No reference counterpart. */
+int os_dense_batch_create_sk(os_ctx *ctx, int num_instances, int n, uint64_t seed,
+                             uint32_t instance_base, os_dense **out);
+/* Copy the FP64 matrices of a dense batch to the host ([I x n x n]). */
+int os_dense_copy_q(os_dense *dense, double *q_out);
 int os_dense_destroy(os_dense *dense);
 
-/* Run num_replicas sweep-mode chains.  Outputs (each may be NULL):
- * best_energies [R], best_states [R x n], final_states [R x n],
- * best_state [n] + best_energy (lowest replica, first index on ties). */
+/* Run num_replicas sweep-mode chains per instance.  Outputs (each may be
+ * NULL), instance-major for a batch of I instances (I = 1 for os_dense_create):
+ * best_energies [I x R], best_states [I x R x n], final_states [I x R x n],
+ * best_state [I x n] + best_energy [I] (lowest replica, first index on ties).
+ * init_states is [I x R x n], init_marginals [I x n]. */
 int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best_energies,
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats);

@@ -212,6 +212,80 @@ class DenseProblem:
                 "best_energy": e.value, "stats": st.as_dict()}
 
 
+class DenseBatch:
+    """Same-size dense QUBOs annealed together (os_dense_batch_create)."""
+
+    def __init__(self, ctx: Context, q_batch=None, *, sk=None):
+        """q_batch: [I, n, n] FP64 matrices; or sk=(I, n, seed, instance_base)
+        to generate SK instances on the GPU."""
+        L = lib()
+        L.os_dense_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p,
+                                            ctypes.POINTER(_vp)]
+        L.os_dense_batch_create_sk.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                               ctypes.c_uint32, ctypes.POINTER(_vp)]
+        L.os_dense_copy_q.argtypes = [_vp, _f64p]
+        self.ctx = ctx
+        h = _vp()
+        if sk is not None:
+            I, n, seed, base = sk
+            self.I, self.n = int(I), int(n)
+            _check(L.os_dense_batch_create_sk(ctx.handle, self.I, self.n, seed, base,
+                                              ctypes.byref(h)))
+        else:
+            q = np.ascontiguousarray(q_batch, dtype=np.float64)
+            if q.ndim != 3 or q.shape[1] != q.shape[2]:
+                raise ValueError("q_batch must be [instances, n, n]")
+            self.I, self.n = q.shape[0], q.shape[1]
+            _check(L.os_dense_batch_create(ctx.handle, self.I, self.n, _p(q, _f64p),
+                                           ctypes.byref(h)))
+        self._h = h
+
+    def q(self):
+        """The batch's FP64 matrices [I, n, n] (copied from the GPU)."""
+        out = np.zeros((self.I, self.n, self.n), dtype=np.float64)
+        _check(lib().os_dense_copy_q(self._h, _p(out, _f64p)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().os_dense_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def anneal(self, num_replicas, tscale, seed=1, instance=0, refresh_interval=0,
+               init_states=None, init_marginals=None, best_states=True, final_states=False):
+        """Arrays are instance-major: best_energies [I, R], best_states [I, R, n],
+        best_state [I, n], best_energy [I]."""
+        ts = np.ascontiguousarray(tscale, dtype=np.float32)
+        I, n, R = self.I, self.n, int(num_replicas)
+        ini = None if init_states is None else np.ascontiguousarray(init_states, dtype=np.uint8)
+        mar = None if init_marginals is None else np.ascontiguousarray(init_marginals,
+                                                                       dtype=np.float32)
+        if ini is not None and ini.size != I * R * n:
+            raise ValueError("init_states must be [I, R, n]")
+        if mar is not None and mar.size != I * n:
+            raise ValueError("init_marginals must be [I, n]")
+        prm = SweepParams(R, len(ts), ts.ctypes.data, seed, instance, refresh_interval,
+                          None if ini is None else ini.ctypes.data,
+                          None if mar is None else mar.ctypes.data)
+        be = np.zeros((I, R), dtype=np.float64)
+        bs = np.zeros((I, R, n), dtype=np.uint8) if best_states else None
+        fs = np.zeros((I, R, n), dtype=np.uint8) if final_states else None
+        best = np.zeros((I, n), dtype=np.uint8)
+        e = np.zeros(I, dtype=np.float64)
+        st = SweepStats()
+        _check(lib().os_dense_anneal(self._h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p),
+                                     _p(fs, _u8p), _p(best, _u8p), _p(e, _f64p),
+                                     ctypes.byref(st)))
+        return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
+                "best_energy": e, "stats": st.as_dict()}
+
+
 class CsrParams(ctypes.Structure):
     """os_csr_params of include/onesolver.h."""
     _fields_ = [("num_replicas", ctypes.c_uint32), ("num_sweeps", ctypes.c_int32),

@@ -266,7 +266,7 @@ template <int TPW>
 __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     int n, const uint4 *__restrict__ qpack, const float *__restrict__ qdb,
     const float *__restrict__ tscale, int sweep0, int nsweeps, uint32_t k0, uint32_t k1,
-    uint32_t tag, float *__restrict__ fields, uint32_t *__restrict__ xbits,
+    uint32_t inst_base, int gpi, float *__restrict__ fields, uint32_t *__restrict__ xbits,
     uint32_t *__restrict__ bestbits, double *__restrict__ energy, double *__restrict__ ebest,
     int *__restrict__ status) {
   static_assert(TPW <= 8, "half a tile per decision step");
@@ -296,7 +296,14 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   const int lane = tid & 63;
   const int lc = lane & 31;
   const int kh = lane >> 5;
+  // gpi work-groups per instance: this one anneals replicas [rloc, rloc+64)
+  // of instance inst, stored at rows [rbase, rbase+64) of the state arrays
+  const int inst = blockIdx.x / gpi;
   const uint32_t rbase = blockIdx.x * kRT;
+  const uint32_t rloc = (uint32_t)(blockIdx.x % gpi) * kRT;
+  const uint32_t tag = ((inst_base + (uint32_t)inst) << 8) | TAG_SWEEP_THR;
+  qpack += (size_t)inst * NB * NP * 4;
+  qdb += (size_t)inst * NB * 256;
   const int col0 = w * 32 * TPW;
   const int G = nsweeps * NB;
 
@@ -340,7 +347,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   for (int it = tid; it < kTRing * 256; it += 256) {
     const int gblk = it >> 8;
     if (gblk < G)
-      block_thresholds(it & 255, gblk % NB, gblk / NB, n, rbase, sweep0, k0, k1, tag,
+      block_thresholds(it & 255, gblk % NB, gblk / NB, n, rloc, sweep0, k0, k1, tag,
                        tscale[gblk / NB], thr[gblk]);
   }
   __syncthreads();
@@ -525,7 +532,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         // reads, so each piece of VALU work runs while two MFMAs execute
         const int item = hw * 64 + lane;
         const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
-        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rbase + (uint32_t)(item >> 2),
+        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
                               (uint32_t)(sweep0 + g2 / NB), tag);
         uint32_t pk0 = k0, pk1 = k1;
         float tv[4];
@@ -555,7 +562,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         if (lane == 0) lds_store(&wnext[w], next);
         if (g + kLA < G) {
           if (hw == g % kHelpers) {  // the fourth quarter rotates
-            item_thresholds(192 + lane, g2 % NB, g2 / NB, n, rbase, sweep0, k0, k1, tag, ts2,
+            item_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
                             tv);
             store_thresholds_asm(192 + lane, tv, thr[sl]);
           }
@@ -656,19 +663,20 @@ int dense_padded_n(int n) {
   return -1;
 }
 
-void launch_dense_sweeps(hipStream_t st, int n, int np, const uint4 *qpack, const float *qdb,
-                         const float *tscale, int sweep0, int nsweeps, uint64_t seed,
-                         uint32_t instance, int R_pad, float *fields, uint32_t *xbits,
-                         uint32_t *bestbits, double *energy, double *ebest, int *status) {
+void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const uint4 *qpack,
+                         const float *qdb, const float *tscale, int sweep0, int nsweeps,
+                         uint64_t seed, uint32_t instance, int R_pad, float *fields,
+                         uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
+                         int *status) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  const uint32_t tag = (instance << 8) | TAG_SWEEP_THR;
-  const dim3 grid(R_pad / kRT), block(kWaves * 64);
+  const int gpi = R_pad / kRT;
+  const dim3 grid(num_instances * gpi), block(kWaves * 64);
   switch (np / (kWaves * 32)) {
 #define OS_DENSE_CASE(T)                                                                    \
   case T:                                                                                   \
     dense_sweep_kernel<T><<<grid, block, 0, st>>>(n, qpack, qdb, tscale, sweep0, nsweeps, k0, \
-                                                  k1, tag, fields, xbits, bestbits, energy, \
-                                                  ebest, status);                           \
+                                                  k1, instance, gpi, fields, xbits, bestbits, \
+                                                  energy, ebest, status);                   \
     break;
     OS_DENSE_CASE(1)
     OS_DENSE_CASE(2)

@@ -5,7 +5,9 @@
 //   qpack  bf16 [N/16 blocks][Np columns][32]: for block b and column j the
 //          16 values hi(Q[16b+k][j]) then the 16 values lo(Q[16b+k][j]),
 //          Q = Qhi + Qlo + O(2^-17 |Q|), diagonal zeroed (the MFMA B operand),
-//   qdb    fp32 [N/16][16][16] diagonal sub-blocks (in-block corrections).
+//   qdb    fp32 [N/16][16][16] diagonal sub-blocks (in-block corrections),
+// packed on the GPU from q64 (gen.hip).  A batch stacks I same-size
+// instances in every array.
 // A run keeps per-replica fields (fp32 R x Np), states and best states
 // (bit-packed), running and best energies (FP64) on the GPU; sweeps run in
 // segments of `refresh_interval`, after each of which the fields and energies
@@ -13,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/onesolver.h"
@@ -21,23 +24,9 @@
 
 namespace os {
 
-uint16_t f32_to_bf16(float f) {
-  uint32_t u;
-  std::memcpy(&u, &f, 4);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
-float bf16_to_f32(uint16_t h) {
-  const uint32_t u = (uint32_t)h << 16;
-  float f;
-  std::memcpy(&f, &u, 4);
-  return f;
-}
-
 struct Dense {
   Ctx *ctx = nullptr;
+  int I = 1;  // instances (same n), stacked in every array
   int n = 0, np = 0, npw = 0;
   DevBuf q64, qpack, qdb;
   DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
@@ -57,56 +46,77 @@ using os::Dense;
 
 extern "C" {
 
-int os_dense_create(os_ctx *ctx_, int n, const double *q_sym, os_dense **out) {
+namespace {
+
+// Allocates a batch of I instances of size n and packs the FP64 matrices
+// already in d->q64 (device) into the MFMA operand layout.
+os::Dense *dense_alloc(Ctx &c, int I, int n) {
+  if (n > os::dense_max_n())
+    throw os::Error(OS_ERR_UNSUPPORTED, "dense sweep path supports n <= " +
+                                            std::to_string(os::dense_max_n()));
+  auto d = std::make_unique<Dense>();
+  d->ctx = &c;
+  d->I = I;
+  d->n = n;
+  d->np = os::dense_padded_n(n);
+  d->npw = d->np / 32;
+  const int np = d->np, nb = np / 16;
+  d->q64.ensure(sizeof(double) * (size_t)n * n * I);
+  d->qpack.ensure(sizeof(uint16_t) * (size_t)nb * np * 32 * I);
+  d->qdb.ensure(sizeof(float) * (size_t)nb * 256 * I);
+  return d.release();
+}
+
+}  // namespace
+
+int os_dense_batch_create(os_ctx *ctx_, int num_instances, int n, const double *q_sym,
+                          os_dense **out) {
   return dguard([&] {
     OS_REQUIRE(ctx_ && q_sym && out, "null argument");
     OS_REQUIRE(n >= 1, "n must be positive");
-    if (n > os::dense_max_n())
-      throw os::Error(OS_ERR_UNSUPPORTED, "dense sweep path supports n <= " +
-                                              std::to_string(os::dense_max_n()));
+    OS_REQUIRE(num_instances >= 1, "num_instances must be positive");
     Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
     c.activate();
-    auto d = new Dense();
-    d->ctx = &c;
-    d->n = n;
-    d->np = os::dense_padded_n(n);
-    d->npw = d->np / 32;
-    const int np = d->np, nb = np / 16;
-    std::vector<uint16_t> pack((size_t)nb * np * 32, 0);
-    std::vector<float> qdb((size_t)nb * 256, 0.0f);
-    for (int b = 0; b < nb; ++b) {
-      for (int k = 0; k < 16; ++k) {
-        const int i = b * 16 + k;
-        if (i >= n) continue;
-        for (int j = 0; j < n; ++j) {
-          if (j == i) continue;
-          const float qf = (float)q_sym[(size_t)i * n + j];
-          const uint16_t hi = os::f32_to_bf16(qf);
-          const uint16_t lo = os::f32_to_bf16(qf - os::bf16_to_f32(hi));
-          uint16_t *dst = &pack[((size_t)b * np + j) * 32];
-          dst[k] = hi;
-          dst[16 + k] = lo;
-          if (j / 16 == b) qdb[(size_t)b * 256 + k * 16 + (j % 16)] = qf;
-        }
-      }
-    }
-    try {
-      d->q64.ensure(sizeof(double) * (size_t)n * n);
-      d->qpack.ensure(pack.size() * sizeof(uint16_t));
-      d->qdb.ensure(qdb.size() * sizeof(float));
-      OS_HIP_CHECK(hipMemcpyAsync(d->q64.ptr, q_sym, sizeof(double) * (size_t)n * n,
-                                  hipMemcpyHostToDevice, c.stream));
-      OS_HIP_CHECK(hipMemcpyAsync(d->qpack.ptr, pack.data(), pack.size() * sizeof(uint16_t),
-                                  hipMemcpyHostToDevice, c.stream));
-      OS_HIP_CHECK(hipMemcpyAsync(d->qdb.ptr, qdb.data(), qdb.size() * sizeof(float),
-                                  hipMemcpyHostToDevice, c.stream));
-      c.sync();
-    } catch (...) {
-      delete d;
-      throw;
-    }
-    *out = reinterpret_cast<os_dense *>(d);
+    std::unique_ptr<Dense> d(dense_alloc(c, num_instances, n));
+    OS_HIP_CHECK(hipMemcpyAsync(d->q64.ptr, q_sym, sizeof(double) * (size_t)n * n * num_instances,
+                                hipMemcpyHostToDevice, c.stream));
+    os::launch_pack_q(c.stream, num_instances, n, d->np, d->q64.as<double>(),
+                      d->qpack.as<uint16_t>(), d->qdb.as<float>());
+    c.sync();
+    *out = reinterpret_cast<os_dense *>(d.release());
   });
+}
+
+int os_dense_batch_create_sk(os_ctx *ctx_, int num_instances, int n, uint64_t seed,
+                             uint32_t instance_base, os_dense **out) {
+  return dguard([&] {
+    OS_REQUIRE(ctx_ && out, "null argument");
+    OS_REQUIRE(n >= 2, "n must be at least 2");
+    OS_REQUIRE(num_instances >= 1, "num_instances must be positive");
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    std::unique_ptr<Dense> d(dense_alloc(c, num_instances, n));
+    os::launch_gen_sk(c.stream, num_instances, n, seed, instance_base, d->q64.as<double>());
+    os::launch_pack_q(c.stream, num_instances, n, d->np, d->q64.as<double>(),
+                      d->qpack.as<uint16_t>(), d->qdb.as<float>());
+    c.sync();
+    *out = reinterpret_cast<os_dense *>(d.release());
+  });
+}
+
+int os_dense_copy_q(os_dense *d_, double *q_out) {
+  return dguard([&] {
+    OS_REQUIRE(d_ && q_out, "null argument");
+    Dense &d = *reinterpret_cast<Dense *>(d_);
+    d.ctx->activate();
+    OS_HIP_CHECK(hipMemcpyAsync(q_out, d.q64.ptr, sizeof(double) * (size_t)d.n * d.n * d.I,
+                                hipMemcpyDeviceToHost, d.ctx->stream));
+    d.ctx->sync();
+  });
+}
+
+int os_dense_create(os_ctx *ctx_, int n, const double *q_sym, os_dense **out) {
+  return os_dense_batch_create(ctx_, 1, n, q_sym, out);
 }
 
 int os_dense_destroy(os_dense *d) {
@@ -123,18 +133,22 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     OS_REQUIRE(p->num_sweeps == 0 || p->tscale, "tscale is null");
     OS_REQUIRE(!(p->init_states && p->init_marginals), "give init_states or init_marginals, not both");
     Dense &d = *reinterpret_cast<Dense *>(d_);
+    OS_REQUIRE((uint64_t)p->instance + (uint64_t)d.I <= (1u << 24),
+               "instance ids must stay below 2^24");
     Ctx &c = *d.ctx;
     c.activate();
-    const int n = d.n, np = d.np, npw = d.npw;
+    const int I = d.I, n = d.n, np = d.np, npw = d.npw;
     const int R = (int)p->num_replicas;
     const int Rp = (R + 63) / 64 * 64;
+    const size_t T = (size_t)I * Rp;  // state rows
+    const size_t nn = (size_t)n * n;
     const int S = p->num_sweeps;
     const int seg = p->refresh_interval > 0 ? p->refresh_interval : std::max(S, 1);
-    d.fields.ensure(sizeof(float) * (size_t)Rp * np);
-    d.xbits.ensure(sizeof(uint32_t) * (size_t)Rp * npw);
-    d.bestbits.ensure(sizeof(uint32_t) * (size_t)Rp * npw);
-    d.energy.ensure(sizeof(double) * Rp);
-    d.ebest.ensure(sizeof(double) * Rp);
+    d.fields.ensure(sizeof(float) * T * np);
+    d.xbits.ensure(sizeof(uint32_t) * T * npw);
+    d.bestbits.ensure(sizeof(uint32_t) * T * npw);
+    d.energy.ensure(sizeof(double) * T);
+    d.ebest.ensure(sizeof(double) * T);
     d.tscale.ensure(sizeof(float) * std::max(S, 1));
     d.scratch.ensure(os::energy_scratch_bytes(n, Rp));
     d.drift.ensure(sizeof(unsigned int));
@@ -152,38 +166,51 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       }
     } guard{ev, t0, t1};
     d.status.ensure(sizeof(int));
-    OS_HIP_CHECK(hipEventRecord(t0, c.stream));
-    OS_HIP_CHECK(hipMemsetAsync(d.status.ptr, 0, sizeof(int), c.stream));
+    hipStream_t st = c.stream;
+    auto qi = [&](int s) { return d.q64.as<double>() + s * nn; };
+    auto rows = [&](int s) { return (size_t)s * Rp; };
+    // FP64 energies (and optionally fields) of the states in `bits`, per instance
+    auto energies = [&](uint32_t *bits, double *out, float *fields) {
+      for (int s = 0; s < I; ++s)
+        os::launch_energy_fp64(st, n, npw, qi(s), Rp, bits + rows(s) * npw, out + rows(s),
+                               d.scratch.as<double>(), fields ? fields + rows(s) * np : nullptr,
+                               np, fields ? d.drift.as<unsigned int>() : nullptr);
+    };
+    OS_HIP_CHECK(hipEventRecord(t0, st));
+    OS_HIP_CHECK(hipMemsetAsync(d.status.ptr, 0, sizeof(int), st));
     if (S > 0)
       OS_HIP_CHECK(hipMemcpyAsync(d.tscale.ptr, p->tscale, sizeof(float) * S,
-                                  hipMemcpyHostToDevice, c.stream));
+                                  hipMemcpyHostToDevice, st));
     // initial states
     if (p->init_states) {
-      d.bytes.ensure((size_t)Rp * n);
-      OS_HIP_CHECK(hipMemsetAsync(d.bytes.ptr, 0, (size_t)Rp * n, c.stream));
-      OS_HIP_CHECK(hipMemcpyAsync(d.bytes.ptr, p->init_states, (size_t)R * n,
-                                  hipMemcpyHostToDevice, c.stream));
-      os::launch_pack_bytes(c.stream, d.bytes.as<uint8_t>(), n, Rp, npw, d.xbits.as<uint32_t>());
+      d.bytes.ensure(T * n);
+      OS_HIP_CHECK(hipMemsetAsync(d.bytes.ptr, 0, T * n, st));
+      for (int s = 0; s < I; ++s)
+        OS_HIP_CHECK(hipMemcpyAsync(d.bytes.as<uint8_t>() + rows(s) * n,
+                                    p->init_states + (size_t)s * R * n, (size_t)R * n,
+                                    hipMemcpyHostToDevice, st));
+      os::launch_pack_bytes(st, d.bytes.as<uint8_t>(), n, (int)T, npw, d.xbits.as<uint32_t>());
     } else if (p->init_marginals) {
-      d.marg.ensure(sizeof(float) * n);
-      OS_HIP_CHECK(hipMemcpyAsync(d.marg.ptr, p->init_marginals, sizeof(float) * n,
-                                  hipMemcpyHostToDevice, c.stream));
-      os::launch_warm_init(c.stream, n, npw, Rp, p->seed, p->instance, d.marg.as<float>(),
-                           d.xbits.as<uint32_t>());
+      d.marg.ensure(sizeof(float) * n * I);
+      OS_HIP_CHECK(hipMemcpyAsync(d.marg.ptr, p->init_marginals, sizeof(float) * n * I,
+                                  hipMemcpyHostToDevice, st));
+      for (int s = 0; s < I; ++s)
+        os::launch_warm_init(st, n, npw, Rp, p->seed, p->instance + s, d.marg.as<float>() + s * n,
+                             d.xbits.as<uint32_t>() + rows(s) * npw);
     } else {
-      os::launch_sweep_init(c.stream, n, npw, Rp, p->seed, p->instance, d.xbits.as<uint32_t>());
+      for (int s = 0; s < I; ++s)
+        os::launch_sweep_init(st, n, npw, Rp, p->seed, p->instance + s,
+                              d.xbits.as<uint32_t>() + rows(s) * npw);
     }
     // fields and energies of the start states (FP64), best = start
-    OS_HIP_CHECK(hipMemsetAsync(d.fields.ptr, 0, sizeof(float) * (size_t)Rp * np, c.stream));
-    OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), c.stream));
-    os::launch_energy_fp64(c.stream, n, npw, d.q64.as<double>(), Rp, d.xbits.as<uint32_t>(),
-                           d.energy.as<double>(), d.scratch.as<double>(), d.fields.as<float>(), np,
-                           d.drift.as<unsigned int>());
-    OS_HIP_CHECK(hipMemcpyAsync(d.ebest.ptr, d.energy.ptr, sizeof(double) * Rp,
-                                hipMemcpyDeviceToDevice, c.stream));
-    OS_HIP_CHECK(hipMemcpyAsync(d.bestbits.ptr, d.xbits.ptr, sizeof(uint32_t) * (size_t)Rp * npw,
-                                hipMemcpyDeviceToDevice, c.stream));
-    OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), c.stream));
+    OS_HIP_CHECK(hipMemsetAsync(d.fields.ptr, 0, sizeof(float) * T * np, st));
+    OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), st));
+    energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), d.fields.as<float>());
+    OS_HIP_CHECK(hipMemcpyAsync(d.ebest.ptr, d.energy.ptr, sizeof(double) * T,
+                                hipMemcpyDeviceToDevice, st));
+    OS_HIP_CHECK(hipMemcpyAsync(d.bestbits.ptr, d.xbits.ptr, sizeof(uint32_t) * T * npw,
+                                hipMemcpyDeviceToDevice, st));
+    OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), st));
     int launches = 0;
     for (int s0 = 0; s0 < S; s0 += seg) {
       const int ns = std::min(seg, S - s0);
@@ -192,55 +219,53 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       ev.push_back(a);
       OS_HIP_CHECK(hipEventCreate(&b));
       ev.push_back(b);
-      OS_HIP_CHECK(hipEventRecord(a, c.stream));
-      os::launch_dense_sweeps(c.stream, n, np, d.qpack.as<uint4>(), d.qdb.as<float>(),
+      OS_HIP_CHECK(hipEventRecord(a, st));
+      os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
                               d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
                               d.fields.as<float>(), d.xbits.as<uint32_t>(),
                               d.bestbits.as<uint32_t>(), d.energy.as<double>(),
                               d.ebest.as<double>(), d.status.as<int>());
-      OS_HIP_CHECK(hipEventRecord(b, c.stream));
+      OS_HIP_CHECK(hipEventRecord(b, st));
       ++launches;
       // FP64 refresh of fields and running energies; records the fp32 drift
-      os::launch_energy_fp64(c.stream, n, npw, d.q64.as<double>(), Rp, d.xbits.as<uint32_t>(),
-                             d.energy.as<double>(), d.scratch.as<double>(), d.fields.as<float>(),
-                             np, d.drift.as<unsigned int>());
+      energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), d.fields.as<float>());
     }
     // FP64 energies of the best states (the returned energies)
-    os::launch_energy_fp64(c.stream, n, npw, d.q64.as<double>(), Rp, d.bestbits.as<uint32_t>(),
-                           d.ebest.as<double>(), d.scratch.as<double>(), nullptr, 0, nullptr);
-    std::vector<double> e(Rp);
-    OS_HIP_CHECK(hipMemcpyAsync(e.data(), d.ebest.ptr, sizeof(double) * Rp, hipMemcpyDeviceToHost,
-                                c.stream));
+    energies(d.bestbits.as<uint32_t>(), d.ebest.as<double>(), nullptr);
+    std::vector<double> e(T);
+    OS_HIP_CHECK(hipMemcpyAsync(e.data(), d.ebest.ptr, sizeof(double) * T, hipMemcpyDeviceToHost,
+                                st));
     int kstatus = 0;
-    OS_HIP_CHECK(hipMemcpyAsync(&kstatus, d.status.ptr, sizeof(int), hipMemcpyDeviceToHost,
-                                c.stream));
+    OS_HIP_CHECK(hipMemcpyAsync(&kstatus, d.status.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
     unsigned int drift_bits = 0;
     OS_HIP_CHECK(hipMemcpyAsync(&drift_bits, d.drift.ptr, sizeof(unsigned int),
-                                hipMemcpyDeviceToHost, c.stream));
-    if (best_states || final_states) d.bytes.ensure((size_t)Rp * n);
-    if (best_states) {
-      os::launch_unpack_bits(c.stream, d.bestbits.as<uint32_t>(), n, R, npw, d.bytes.as<uint8_t>());
-      OS_HIP_CHECK(hipMemcpyAsync(best_states, d.bytes.ptr, (size_t)R * n, hipMemcpyDeviceToHost,
-                                  c.stream));
-    }
-    if (final_states) {
+                                hipMemcpyDeviceToHost, st));
+    if (best_states || final_states) d.bytes.ensure(T * n);
+    auto fetch = [&](uint32_t *bits, uint8_t *dst) {
+      os::launch_unpack_bits(st, bits, n, (int)T, npw, d.bytes.as<uint8_t>());
+      for (int s = 0; s < I; ++s)
+        OS_HIP_CHECK(hipMemcpyAsync(dst + (size_t)s * R * n, d.bytes.as<uint8_t>() + rows(s) * n,
+                                    (size_t)R * n, hipMemcpyDeviceToHost, st));
       c.sync();
-      os::launch_unpack_bits(c.stream, d.xbits.as<uint32_t>(), n, R, npw, d.bytes.as<uint8_t>());
-      OS_HIP_CHECK(hipMemcpyAsync(final_states, d.bytes.ptr, (size_t)R * n, hipMemcpyDeviceToHost,
-                                  c.stream));
-    }
-    OS_HIP_CHECK(hipEventRecord(t1, c.stream));
+    };
+    if (best_states) fetch(d.bestbits.as<uint32_t>(), best_states);
+    if (final_states) fetch(d.xbits.as<uint32_t>(), final_states);
+    OS_HIP_CHECK(hipEventRecord(t1, st));
     c.sync();
     if (kstatus != 0)
       throw os::Error(OS_ERR_INTERNAL, "dense sweep kernel aborted: a work-group wait expired");
-    const size_t bi = (size_t)(std::min_element(e.begin(), e.begin() + R) - e.begin());
-    if (best_energies) std::memcpy(best_energies, e.data(), sizeof(double) * R);
-    if (best_energy) *best_energy = e[bi];
-    if (best_state) {
-      std::vector<uint32_t> wbits(npw);
-      OS_HIP_CHECK(hipMemcpy(wbits.data(), d.bestbits.as<uint32_t>() + bi * npw,
-                             sizeof(uint32_t) * npw, hipMemcpyDeviceToHost));
-      for (int j = 0; j < n; ++j) best_state[j] = (uint8_t)((wbits[j >> 5] >> (j & 31)) & 1u);
+    std::vector<uint32_t> wbits(npw);
+    for (int s = 0; s < I; ++s) {
+      const double *es = e.data() + rows(s);
+      const size_t bi = (size_t)(std::min_element(es, es + R) - es);
+      if (best_energies) std::memcpy(best_energies + (size_t)s * R, es, sizeof(double) * R);
+      if (best_energy) best_energy[s] = es[bi];
+      if (best_state) {
+        OS_HIP_CHECK(hipMemcpy(wbits.data(), d.bestbits.as<uint32_t>() + (rows(s) + bi) * npw,
+                               sizeof(uint32_t) * npw, hipMemcpyDeviceToHost));
+        for (int j = 0; j < n; ++j)
+          best_state[(size_t)s * n + j] = (uint8_t)((wbits[j >> 5] >> (j & 31)) & 1u);
+      }
     }
     if (stats) {
       double kms = 0.0;
@@ -257,7 +282,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       stats->total_ms = tms;
       stats->kernel_launches = launches;
       stats->max_field_drift = drift;
-      stats->attempts = (uint64_t)R * (uint64_t)S * (uint64_t)n;
+      stats->attempts = (uint64_t)I * (uint64_t)R * (uint64_t)S * (uint64_t)n;
     }
   });
 }

@@ -28,14 +28,25 @@ void launch_ref_anneal(hipStream_t st, int n, const double *q, const double *bet
 // anneal_dense.hip
 int dense_max_n();
 int dense_padded_n(int n);
-void launch_dense_sweeps(hipStream_t st, int n, int np, const uint4 *qpack, const float *qdb,
-                         const float *tscale, int sweep0, int nsweeps, uint64_t seed,
-                         uint32_t instance, int R_pad, float *fields, uint32_t *xbits,
-                         uint32_t *bestbits, double *energy, double *ebest, int *status);
+// num_instances instances of n spins (qpack / qdb stacked per instance), R_pad
+// replicas each (instance i at rows [i*R_pad, (i+1)*R_pad), stream id
+// instance + i)
+void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const uint4 *qpack,
+                         const float *qdb, const float *tscale, int sweep0, int nsweeps,
+                         uint64_t seed, uint32_t instance, int R_pad, float *fields,
+                         uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
+                         int *status);
 void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
                        uint32_t *xbits);
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
                       const float *p, uint32_t *xbits);
+
+// gen.hip
+// q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed
+void launch_gen_sk(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_base, double *q);
+// bf16 hi+lo operand packing and fp32 diagonal sub-blocks of I instances
+void launch_pack_q(hipStream_t st, int I, int n, int np, const double *q, uint16_t *qpack,
+                   float *qdb);
 
 // anneal_csr.hip
 // Per-instance descriptor of a sparse batch (offsets into the batch arrays).

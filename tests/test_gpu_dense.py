@@ -94,3 +94,54 @@ def test_dense_start_states_and_marginals(ctx):
 def test_dense_rejects_oversized(ctx):
     with pytest.raises(onesolver_amd.OneSolverError):
         onesolver_amd.DenseProblem(ctx, np.zeros((2000, 2000)))
+
+
+def test_dense_batch_equals_single_instance_runs(ctx):
+    """A batch of instances in one launch gives every instance exactly the
+    chains of a run on its own (stream id instance + i) and of the oracle."""
+    qs = np.stack([_int_instance(100, s) for s in (1, 2, 3)])
+    R, S = 96, 15
+    ts = _tscale(S)
+    print("[gpu] dense batch of 3, n=100", flush=True)
+    batch = onesolver_amd.DenseBatch(ctx, qs)
+    out = batch.anneal(R, ts, seed=5, instance=10, refresh_interval=4, final_states=True)
+    for i in range(3):
+        fin, bs, be = oracle.sweep_dense(qs[i], 100, ts, R, seed=5, instance=10 + i)
+        assert (out["final_states"][i] == fin).all()
+        assert (out["best_states"][i] == bs).all()
+        np.testing.assert_allclose(out["best_energies"][i], oracle.energy(qs[i], bs), rtol=1e-12)
+        single = onesolver_amd.DenseProblem(ctx, qs[i]).anneal(R, ts, seed=5, instance=10 + i,
+                                                                refresh_interval=4)
+        assert (single["best_states"] == out["best_states"][i]).all()
+        assert single["best_energy"] == out["best_energy"][i]
+    # warm start per instance
+    p = np.stack([np.full(100, 0.2, np.float32), np.full(100, 0.5, np.float32),
+                  np.full(100, 0.9, np.float32)])
+    w = batch.anneal(R, ts[:0], seed=5, instance=10, init_marginals=p)
+    for i in range(3):
+        ref = oracle.warm_init(p[i], R, seed=5, instance=10 + i)
+        assert (w["best_states"][i] == ref).all()
+    batch.close()
+
+
+def test_sk_instances_generated_on_the_gpu_match_the_host_generator(ctx):
+    """The GPU generator restates instances.sk_qubo: same Philox counters and
+    Box-Muller transform; off-diagonal couplings agree to fp32 rounding (the
+    device and host FP64 log/cos may differ in the last bit), diagonals to the
+    summation order."""
+    I, n = 3, 96
+    print("[gpu] SK generator n=96", flush=True)
+    b = onesolver_amd.DenseBatch(ctx, sk=(I, n, 7, 20))
+    qg = b.q()
+    for s in range(I):
+        qh, _ = instances.sk_qubo(n, seed=7, instance=20 + s)
+        off = ~np.eye(n, dtype=bool)
+        np.testing.assert_allclose(qg[s][off], qh[off], rtol=2 ** -22, atol=0)
+        np.testing.assert_allclose(np.diag(qg[s]), np.diag(qh), rtol=1e-6, atol=1e-7)
+        assert (qg[s] == qg[s].T).all()
+    # the packed operands of a generated batch anneal like an uploaded one
+    ts = _tscale(8)
+    out = b.anneal(64, ts, seed=3, instance=20)
+    up = onesolver_amd.DenseBatch(ctx, qg).anneal(64, ts, seed=3, instance=20)
+    assert (out["best_states"] == up["best_states"]).all()
+    b.close()
