@@ -138,6 +138,33 @@ int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats);
 
+/* ---- learned warm start --------------------------------------------------
+ * A 3-layer message-passing network (hidden 64, bf16 weights, fp32
+ * accumulation) over the coupling graph of dense instances predicts per-spin
+ * marginals p_j of good annealed states; os_dense_anneal's init_marginals
+ * seeds replicas from them.  Trained on the GPU from best states of the plain
+ * annealer.  Labels are 0/1 states [count x n]; a state and its complement
+ * have the same SK energy, so labels are taken with spin 0 flipped to 1 (and
+ * the network pins spin 0).  Parameters are exchanged as fp32 in the packed
+ * order: per layer Ws [fin x 64], Wm [fin x 64], b [64] (fin = 4, 64, 64),
+ * then wo [64], bo [1].  No reference counterpart: the reference only starts
+ * from uniform random bits (include/simulated_annealing/annealing.hpp:88-91). */
+typedef struct os_mpnn os_mpnn;
+
+int os_mpnn_num_params(void); /* returns the count (17153), not a status */
+int os_mpnn_create(os_ctx *ctx, uint64_t seed, os_mpnn **out);
+int os_mpnn_destroy(os_mpnn *net);
+int os_mpnn_get_params(os_mpnn *net, float *params);
+int os_mpnn_set_params(os_mpnn *net, const float *params);
+/* marginals [count x n] of instances [first, first+count) of a dense batch */
+int os_mpnn_predict(os_mpnn *net, os_dense *data, int first, int count, float *marginals);
+/* mean binary cross-entropy over all spins and its gradient (packed order) */
+int os_mpnn_loss_grad(os_mpnn *net, os_dense *data, int first, int count,
+                      const uint8_t *labels, double *loss, float *grad);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss */
+int os_mpnn_train_step(os_mpnn *net, os_dense *data, int first, int count,
+                       const uint8_t *labels, float lr, double *loss);
+
 /* ---- batched sparse instances (CSR) ------------------------------------
  * Replaces, for many sparse instances at once, the loop a caller would write
  * around sa::anneal (include/simulated_annealing/annealing.hpp:55-140) and the

@@ -379,3 +379,75 @@ class CsrBatch:
         return {"best_energies": be, "run_best_energies": rb, "best_states": split(bs),
                 "final_states": split(fs), "inst_best_energy": ib, "inst_best_replica": ir,
                 "stats": st.as_dict()}
+
+
+def _mpnn_lib():
+    L = lib()
+    if not getattr(L, "_mpnn_ready", False):
+        L.os_mpnn_num_params.restype = ctypes.c_int
+        L.os_mpnn_create.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]
+        L.os_mpnn_destroy.argtypes = [_vp]
+        L.os_mpnn_get_params.argtypes = [_vp, _f32p]
+        L.os_mpnn_set_params.argtypes = [_vp, _f32p]
+        L.os_mpnn_predict.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _f32p]
+        L.os_mpnn_loss_grad.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _u8p,
+                                        ctypes.POINTER(ctypes.c_double), _f32p]
+        L.os_mpnn_train_step.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _u8p,
+                                         ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+        L._mpnn_ready = True
+    return L
+
+
+class WarmStartNet:
+    """The warm-start message-passing network on the GPU (os_mpnn)."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        L = _mpnn_lib()
+        self.ctx = ctx
+        self.num_params = int(L.os_mpnn_num_params())
+        h = _vp()
+        _check(L.os_mpnn_create(ctx.handle, seed, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _mpnn_lib().os_mpnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(self.num_params, dtype=np.float32)
+        _check(_mpnn_lib().os_mpnn_get_params(self._h, _p(out, _f32p)))
+        return out
+
+    def set_params(self, flat):
+        w = np.ascontiguousarray(flat, dtype=np.float32)
+        if w.size != self.num_params:
+            raise ValueError("wrong parameter count")
+        _check(_mpnn_lib().os_mpnn_set_params(self._h, _p(w, _f32p)))
+
+    def predict(self, data: "DenseBatch", first=0, count=None):
+        count = data.I - first if count is None else count
+        out = np.zeros((count, data.n), dtype=np.float32)
+        _check(_mpnn_lib().os_mpnn_predict(self._h, data._h, first, count, _p(out, _f32p)))
+        return out
+
+    def loss_grad(self, data: "DenseBatch", labels, first=0):
+        y = np.ascontiguousarray(labels, dtype=np.uint8).reshape(-1, data.n)
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        _check(_mpnn_lib().os_mpnn_loss_grad(self._h, data._h, first, y.shape[0], _p(y, _u8p),
+                                             ctypes.byref(loss), _p(g, _f32p)))
+        return loss.value, g
+
+    def train_step(self, data: "DenseBatch", labels, lr, first=0):
+        y = np.ascontiguousarray(labels, dtype=np.uint8).reshape(-1, data.n)
+        loss = ctypes.c_double(0.0)
+        _check(_mpnn_lib().os_mpnn_train_step(self._h, data._h, first, y.shape[0], _p(y, _u8p),
+                                              lr, ctypes.byref(loss)))
+        return loss.value

@@ -20,24 +20,8 @@
 
 #include "../../include/onesolver.h"
 #include "os_ctx.h"
+#include "os_dense.h"
 #include "os_kernels.h"
-
-namespace os {
-
-struct Dense {
-  Ctx *ctx = nullptr;
-  int I = 1;  // instances (same n), stacked in every array
-  int n = 0, np = 0, npw = 0;
-  DevBuf q64, qpack, qdb;
-  DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
-  ~Dense() {
-    for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
-                    &scratch, &drift, &bytes, &marg, &status})
-      b->release();
-  }
-};
-
-}  // namespace os
 
 using os::Ctx;
 using os::Dense;
