@@ -232,12 +232,56 @@ def run_csr_c3(args):
     return res, (insts[imax], ts)
 
 
+def run_warmstart_c4(args):
+    """Config 4: learned warm start on the dense SK distribution (N=1024):
+    train the network on the GPU on instances labelled by the plain annealer,
+    then time-to-99% of the best known FP64 energy on unseen instances, warm
+    start (inference included, 256 replicas) vs plain GPU annealing."""
+    import onesolver_amd
+    from onesolver_amd import warmstart as ws
+
+    ctx = onesolver_amd.Context(0)
+    log(f"device {ctx.device_name()}")
+    net = onesolver_amd.WarmStartNet(ctx, seed=1)
+    t0 = time.perf_counter()
+    losses, t_labels = ws.train(ctx, net, n=1024, instances=args.c4_train, epochs=args.c4_epochs,
+                                lr=2e-3, seed=1, log=log)
+    t_train = time.perf_counter() - t0
+    heldout = ws.heldout_loss(ctx, net, instances=64)
+    log(f"trained in {t_train:.1f} s (labels {t_labels:.1f} s), held-out BCE {heldout:.4f}")
+    res = ws.evaluate(ctx, net, n=1024, instances=args.c4_eval, max_sweeps=1024, log=log)
+    plain, learned = res["plain"][1], res["learned"][1]
+    ctx.close()
+    out = {
+        "metric": "time-to-99%-best energy (learned warm start)",
+        "value": learned["wall_s"] if learned else None, "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 weights fp32 acc (network); bf16 hi+lo MFMA (SA)",
+        "data": "synthetic (SK instances generated on the GPU)",
+        "config": {"workload": f"dense SK N=1024, network trained on {args.c4_train} instances "
+                               f"x {args.c4_epochs} epochs, {args.c4_eval} unseen instances, "
+                               f"256 replicas, sweep ladder 4..1024", "n": 1024},
+        "plain_time_to_99_s": plain["wall_s"] if plain else None,
+        "speedup_vs_plain": (plain["wall_s"] / learned["wall_s"]) if plain and learned else None,
+        "random_init_warm_schedule_s": (res["random_warm_schedule"][1] or {}).get("wall_s"),
+        "untrained_net_s": (res["untrained_net"][1] or {}).get("wall_s"),
+        "heldout_bce": heldout, "uninformed_bce": float(np.log(2.0)),
+        "train_s": t_train, "label_s": t_labels, "infer_ms": res["t_infer_s"] * 1e3,
+        "ladder": {k: res[k][0] for k in ("plain", "random_warm_schedule", "untrained_net",
+                                          "learned")},
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4"])
+    ap.add_argument("--c4-train", type=int, default=4096)
+    ap.add_argument("--c4-epochs", type=int, default=2)
+    ap.add_argument("--c4-eval", type=int, default=32)
     ap.add_argument("--sweeps-c3", type=int, default=2048)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--replicas", type=int, default=16384)
@@ -249,6 +293,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(replicas would shard across GPUs with one final min-reduction)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        print(json.dumps(run_warmstart_c4(args)))
+        return
     if args.workload == "c3":
         res, (inst, ts) = run_csr_c3(args)
         if not args.no_cpu_baseline:

@@ -1,0 +1,58 @@
+"""Ablation of the learned warm start (GPU): what the trained network gains
+over the method without it.
+
+The network is trained on the GPU on SK instances (n=1024, the dense
+benchmark distribution) labelled by the plain annealer, then compared on
+unseen instances with
+  * the uninformed predictor p = 1/2 and the untrained network (held-out
+    cross-entropy: information about good states), and
+  * plain random-init annealing (time-to-99% of the best known energy).
+Every energy compared is the FP64 energy of a returned state.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import onesolver_amd
+from onesolver_amd import warmstart as ws
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    net = onesolver_amd.WarmStartNet(ctx, seed=1)
+    print("[gpu] warm start: training on 2048 SK instances n=1024", flush=True)
+    losses, _ = ws.train(ctx, net, n=1024, instances=2048, epochs=2, lr=2e-3, seed=1)
+    yield net, losses
+    net.close()
+
+
+def test_training_lowers_the_loss(trained):
+    _, losses = trained
+    assert np.mean(losses[-32:]) < np.mean(losses[:8]) - 0.02
+
+
+def test_trained_network_predicts_unseen_instances_better_than_without_it(ctx, trained):
+    net, _ = trained
+    fresh = onesolver_amd.WarmStartNet(ctx, seed=1)
+    print("[gpu] warm start: held-out cross-entropy", flush=True)
+    l_trained = ws.heldout_loss(ctx, net, instances=64)
+    l_fresh = ws.heldout_loss(ctx, fresh, instances=64)
+    fresh.close()
+    print(f"[gpu] held-out BCE trained {l_trained:.4f} untrained {l_fresh:.4f} "
+          f"uninformed {math.log(2):.4f}", flush=True)
+    assert l_trained < l_fresh
+    assert l_trained < math.log(2) - 0.003
+
+
+def test_warm_start_reaches_99_percent_no_later_than_plain_annealing(ctx, trained):
+    net, _ = trained
+    print("[gpu] warm start: time-to-99% ladder", flush=True)
+    res = ws.evaluate(ctx, net, n=1024, instances=32, max_sweeps=1024, log=print)
+    plain = res["plain"][1]
+    learned = res["learned"][1]
+    assert plain is not None and learned is not None
+    assert learned["sweeps"] <= plain["sweeps"]
+    assert learned["wall_s"] <= plain["wall_s"] * 1.05
