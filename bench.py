@@ -273,12 +273,33 @@ def run_warmstart_c4(args):
     return out
 
 
+def run_copy_calibration(args):
+    """A plain device-to-device copy of known size (torch's copy kernel), whose
+    FETCH_SIZE / WRITE_SIZE counters calibrate the traffic figures of the
+    other kernels (bytes moved = 2 x size: one read, one write)."""
+    import torch
+
+    nbytes = 1 << 30
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"metric": "copy bandwidth", "value": 2 * nbytes * args.steps / el / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": True,
+            "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "copy"])
     ap.add_argument("--c4-train", type=int, default=4096)
     ap.add_argument("--c4-epochs", type=int, default=2)
     ap.add_argument("--c4-eval", type=int, default=32)
@@ -293,6 +314,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               "(replicas would shard across GPUs with one final min-reduction)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copy":
+        print(json.dumps(run_copy_calibration(args)))
+        return
     if args.workload == "c4":
         print(json.dumps(run_warmstart_c4(args)))
         return

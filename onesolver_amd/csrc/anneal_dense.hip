@@ -612,15 +612,19 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   if (tid == 0 && abort_flag) atomicOr(status, 1);
 }
 
-// Random initial bits: spin j of replica r from counter (j>>7, r, 0, tag_init).
-__global__ __launch_bounds__(256) void sweep_init_kernel(int n, int npw, int R, uint32_t k0,
-                                                         uint32_t k1, uint32_t tag,
+// Random initial bits of instance i (rows [i*R, (i+1)*R)): spin j of replica
+// r from counter (j>>7, r, 0, ((instance + i) << 8) | TAG_SWEEP_INIT).
+__global__ __launch_bounds__(256) void sweep_init_kernel(int n, int npw, int R, int I, uint32_t k0,
+                                                         uint32_t k1, uint32_t instance,
                                                          uint32_t *__restrict__ xbits) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)R * npw) return;
-  const uint32_t r = (uint32_t)(idx / npw);
+  if (idx >= (size_t)I * R * npw) return;
+  const size_t row = idx / npw;
+  const uint32_t inst = (uint32_t)(row / R);
+  const uint32_t r = (uint32_t)(row % R);
   const int wd = (int)(idx % npw);
   const int j0 = wd * 32;
+  const uint32_t tag = ((instance + inst) << 8) | TAG_SWEEP_INIT;
   const uint4 o = philox4x32_10(make_uint4((uint32_t)(j0 >> 7), r, 0u, tag), k0, k1);
   uint32_t v = pick(o, (j0 >> 5) & 3);
   const int valid = n - j0;
@@ -628,15 +632,20 @@ __global__ __launch_bounds__(256) void sweep_init_kernel(int n, int npw, int R, 
   xbits[idx] = v;
 }
 
-// Warm start: x_j = (u < p_j), u = (w >> 8) * 2^-24 from counter (j>>2, r, 0, tag_warm).
-__global__ __launch_bounds__(256) void warm_init_kernel(int n, int npw, int R, uint32_t k0,
-                                                        uint32_t k1, uint32_t tag,
+// Warm start of instance i: x_j = (u < p[i][j]), u = (w >> 8) * 2^-24 from
+// counter (j>>2, r, 0, ((instance + i) << 8) | TAG_WARM).
+__global__ __launch_bounds__(256) void warm_init_kernel(int n, int npw, int R, int I, uint32_t k0,
+                                                        uint32_t k1, uint32_t instance,
                                                         const float *__restrict__ p,
                                                         uint32_t *__restrict__ xbits) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)R * npw) return;
-  const uint32_t r = (uint32_t)(idx / npw);
+  if (idx >= (size_t)I * R * npw) return;
+  const size_t row = idx / npw;
+  const uint32_t inst = (uint32_t)(row / R);
+  const uint32_t r = (uint32_t)(row % R);
   const int wd = (int)(idx % npw);
+  const float *pi = p + (size_t)inst * n;
+  const uint32_t tag = ((instance + inst) << 8) | TAG_WARM;
   uint32_t v = 0;
   for (int q = 0; q < 8; ++q) {
     const int j0 = wd * 32 + 4 * q;
@@ -644,7 +653,7 @@ __global__ __launch_bounds__(256) void warm_init_kernel(int n, int npw, int R, u
     const uint4 o = philox4x32_10(make_uint4((uint32_t)(j0 >> 2), r, 0u, tag), k0, k1);
     for (int jj = 0; jj < 4; ++jj) {
       const int j = j0 + jj;
-      if (j < n && (float)(pick(o, jj) >> 8) * 0x1p-24f < p[j]) v |= 1u << (4 * q + jj);
+      if (j < n && (float)(pick(o, jj) >> 8) * 0x1p-24f < pi[j]) v |= 1u << (4 * q + jj);
     }
   }
   xbits[idx] = v;
@@ -692,18 +701,20 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const
 }
 
 void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
-                       uint32_t *xbits) {
-  const size_t total = (size_t)R * npw;
+                       uint32_t *xbits, int num_instances) {
+  const size_t total = (size_t)num_instances * R * npw;
+  if (total == 0) return;
   sweep_init_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(
-      n, npw, R, (uint32_t)seed, (uint32_t)(seed >> 32), (instance << 8) | TAG_SWEEP_INIT, xbits);
+      n, npw, R, num_instances, (uint32_t)seed, (uint32_t)(seed >> 32), instance, xbits);
   OS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
-                      const float *p, uint32_t *xbits) {
-  const size_t total = (size_t)R * npw;
+                      const float *p, uint32_t *xbits, int num_instances) {
+  const size_t total = (size_t)num_instances * R * npw;
+  if (total == 0) return;
   warm_init_kernel<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(
-      n, npw, R, (uint32_t)seed, (uint32_t)(seed >> 32), (instance << 8) | TAG_WARM, p, xbits);
+      n, npw, R, num_instances, (uint32_t)seed, (uint32_t)(seed >> 32), instance, p, xbits);
   OS_HIP_CHECK(hipGetLastError());
 }
 

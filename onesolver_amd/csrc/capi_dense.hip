@@ -134,7 +134,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     d.energy.ensure(sizeof(double) * T);
     d.ebest.ensure(sizeof(double) * T);
     d.tscale.ensure(sizeof(float) * std::max(S, 1));
-    d.scratch.ensure(os::energy_scratch_bytes(n, Rp));
+    d.scratch.ensure(os::energy_scratch_bytes(n, (int)T));
     d.drift.ensure(sizeof(unsigned int));
     hipEvent_t t0, t1;
     OS_HIP_CHECK(hipEventCreate(&t0));
@@ -153,12 +153,11 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     hipStream_t st = c.stream;
     auto qi = [&](int s) { return d.q64.as<double>() + s * nn; };
     auto rows = [&](int s) { return (size_t)s * Rp; };
-    // FP64 energies (and optionally fields) of the states in `bits`, per instance
+    // FP64 energies (and optionally fields) of the states in `bits`, all
+    // instances in one launch
     auto energies = [&](uint32_t *bits, double *out, float *fields) {
-      for (int s = 0; s < I; ++s)
-        os::launch_energy_fp64(st, n, npw, qi(s), Rp, bits + rows(s) * npw, out + rows(s),
-                               d.scratch.as<double>(), fields ? fields + rows(s) * np : nullptr,
-                               np, fields ? d.drift.as<unsigned int>() : nullptr);
+      os::launch_energy_fp64(st, n, npw, qi(0), Rp, bits, out, d.scratch.as<double>(), fields, np,
+                             fields ? d.drift.as<unsigned int>() : nullptr, I);
     };
     OS_HIP_CHECK(hipEventRecord(t0, st));
     OS_HIP_CHECK(hipMemsetAsync(d.status.ptr, 0, sizeof(int), st));
@@ -178,13 +177,10 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       d.marg.ensure(sizeof(float) * n * I);
       OS_HIP_CHECK(hipMemcpyAsync(d.marg.ptr, p->init_marginals, sizeof(float) * n * I,
                                   hipMemcpyHostToDevice, st));
-      for (int s = 0; s < I; ++s)
-        os::launch_warm_init(st, n, npw, Rp, p->seed, p->instance + s, d.marg.as<float>() + s * n,
-                             d.xbits.as<uint32_t>() + rows(s) * npw);
+      os::launch_warm_init(st, n, npw, Rp, p->seed, p->instance, d.marg.as<float>(),
+                           d.xbits.as<uint32_t>(), I);
     } else {
-      for (int s = 0; s < I; ++s)
-        os::launch_sweep_init(st, n, npw, Rp, p->seed, p->instance + s,
-                              d.xbits.as<uint32_t>() + rows(s) * npw);
+      os::launch_sweep_init(st, n, npw, Rp, p->seed, p->instance, d.xbits.as<uint32_t>(), I);
     }
     // fields and energies of the start states (FP64), best = start
     OS_HIP_CHECK(hipMemsetAsync(d.fields.ptr, 0, sizeof(float) * T * np, st));
@@ -238,17 +234,21 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     c.sync();
     if (kstatus != 0)
       throw os::Error(OS_ERR_INTERNAL, "dense sweep kernel aborted: a work-group wait expired");
-    std::vector<uint32_t> wbits(npw);
+    std::vector<uint32_t> wbits;
+    if (best_state) {
+      wbits.resize(T * npw);
+      OS_HIP_CHECK(hipMemcpy(wbits.data(), d.bestbits.ptr, sizeof(uint32_t) * T * npw,
+                             hipMemcpyDeviceToHost));
+    }
     for (int s = 0; s < I; ++s) {
       const double *es = e.data() + rows(s);
       const size_t bi = (size_t)(std::min_element(es, es + R) - es);
       if (best_energies) std::memcpy(best_energies + (size_t)s * R, es, sizeof(double) * R);
       if (best_energy) best_energy[s] = es[bi];
       if (best_state) {
-        OS_HIP_CHECK(hipMemcpy(wbits.data(), d.bestbits.as<uint32_t>() + (rows(s) + bi) * npw,
-                               sizeof(uint32_t) * npw, hipMemcpyDeviceToHost));
+        const uint32_t *wb = wbits.data() + (rows(s) + bi) * npw;
         for (int j = 0; j < n; ++j)
-          best_state[(size_t)s * n + j] = (uint8_t)((wbits[j >> 5] >> (j & 31)) & 1u);
+          best_state[(size_t)s * n + j] = (uint8_t)((wb[j >> 5] >> (j & 31)) & 1u);
       }
     }
     if (stats) {

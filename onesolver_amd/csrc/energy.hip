@@ -61,6 +61,11 @@ __global__ __launch_bounds__(256) void xq_tile_kernel(int n, int W, const double
   const int ty = tid >> 4;
   const int r0 = blockIdx.y * kTR;
   const int c0 = blockIdx.x * kTN;
+  // instance blockIdx.z: its matrix and its R rows of states / outputs
+  q += (size_t)blockIdx.z * n * n;
+  bits += (size_t)blockIdx.z * R * W;
+  partial += (size_t)blockIdx.z * R * ncb;
+  if (kFields) fields += (size_t)blockIdx.z * R * ldf;
   if (tid < kTN) {
     const int c = c0 + tid;
     qdiag[tid] = c < n ? q[(size_t)c * n + c] : 0.0;
@@ -168,10 +173,10 @@ size_t energy_scratch_bytes(int n, int count) {
 
 void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count,
                         const uint32_t *bits, double *out, double *scratch, float *fields, int ldf,
-                        unsigned int *drift_max) {
-  if (count <= 0) return;
+                        unsigned int *drift_max, int num_instances) {
+  if (count <= 0 || num_instances <= 0) return;
   const int ncb = (n + kTN - 1) / kTN;
-  dim3 grid(ncb, (count + kTR - 1) / kTR);
+  dim3 grid(ncb, (count + kTR - 1) / kTR, num_instances);
   if (fields)
     xq_tile_kernel<true><<<grid, 256, 0, st>>>(n, W, q, count, bits, scratch, ncb, fields, ldf,
                                                 drift_max);
@@ -179,7 +184,8 @@ void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count
     xq_tile_kernel<false><<<grid, 256, 0, st>>>(n, W, q, count, bits, scratch, ncb, nullptr, 0,
                                                  nullptr);
   OS_HIP_CHECK(hipGetLastError());
-  reduce_partials_kernel<<<(count + 255) / 256, 256, 0, st>>>(scratch, count, ncb, out);
+  const int rows = count * num_instances;
+  reduce_partials_kernel<<<(rows + 255) / 256, 256, 0, st>>>(scratch, rows, ncb, out);
   OS_HIP_CHECK(hipGetLastError());
 }
 

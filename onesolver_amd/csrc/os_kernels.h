@@ -14,9 +14,10 @@ void launch_unpack_bits(hipStream_t st, const uint32_t *bits, int n, int count, 
 size_t energy_scratch_bytes(int n, int count);
 // out[r] = FP64 energy of bit-packed state r.  With `fields` non-null, also
 // refreshes fields[r*ldf + j] (fp32) and raises *drift_max (fp32 bit pattern).
+// num_instances matrices (stride n*n) with `count` consecutive states each.
 void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count,
                         const uint32_t *bits, double *out, double *scratch, float *fields, int ldf,
-                        unsigned int *drift_max);
+                        unsigned int *drift_max, int num_instances = 1);
 
 // anneal_ref.hip
 int ref_lds_limit();
@@ -36,10 +37,12 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const
                          uint64_t seed, uint32_t instance, int R_pad, float *fields,
                          uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
                          int *status);
+// start states of num_instances instances (R rows each, stream ids instance + i;
+// warm start: marginals p[i][n])
 void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
-                       uint32_t *xbits);
+                       uint32_t *xbits, int num_instances = 1);
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
-                      const float *p, uint32_t *xbits);
+                      const float *p, uint32_t *xbits, int num_instances = 1);
 
 // gen.hip
 // q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed
