@@ -10,23 +10,23 @@
 //
 // Work-group: 64 replicas (two 32-replica tiles), 4 waves (one per SIMD, so
 // each wave has the full register file and the fields sit in accumulation
-// registers); wave w owns the TPW column tiles (32 spins each) [w*TPW,
-// (w+1)*TPW).  Block b (spins 16b .. 16b+15) lies in column tile b/2 and
-// belongs to the wave owning that tile.  For block g of the global block
-// sequence (g = sweep * NB + b) the owner, lane = replica:
-//   (a) applies D_{g-1} to the tile holding block b (4 MFMAs), transposes it
-//       through LDS and decides its 16 spins in order; flips inside the block
-//       correct the later spins with the fp32 16x16 diagonal sub-block of Q;
-//   (b) applies D_{g-1} to its other tiles, half a tile per decision step, so
-//       the matrix cores run under the decisions' VALU work;
-//   (c) publishes D_g as +/- bit masks in an LDS ring and bumps a counter.
-// The owner's Q operands come from LDS only: the three other waves copy them
-// (a third each, global_load_lds) into a double-buffered LDS area one step
-// ahead.  Those waves also apply each published D block to all their own
-// tiles (operands staged in LDS one block ahead) and compute the acceptance
-// thresholds kLA blocks ahead.  Waves synchronise only through LDS counters
-// with bounded spin waits (an expired wait aborts the launch and reports it
-// in `status`).
+// registers).  Blocks are dealt to the waves round-robin: block b belongs to
+// wave b % 4, and wave w holds the columns of its blocks as TPW tiles of 32
+// columns (tile t = blocks w + 8t and w + 8t + 4).  Step g of the global
+// block sequence (g = sweep * NB + b):
+//   * the owner of block b applies D_{g-1} to the one tile holding b (4
+//     MFMAs), transposes it through LDS, decides the 16 spins in order (lane
+//     = replica; flips inside the block correct the later spins with the fp32
+//     16x16 diagonal sub-block of Q) and publishes D_g as +/- bit masks in an
+//     LDS ring; only then does it apply D_{g-1} to its other tiles;
+//   * the three other waves apply D_{g-1} to all their tiles, with the
+//     Philox rounds of the acceptance thresholds kLA blocks ahead spread
+//     between the MFMAs.
+// So the chain of decisions passes from SIMD to SIMD, and each SIMD's MFMA
+// work runs while the next owner decides.  Q operands stream from L2 into
+// registers one step ahead (the owner's tile and sub-block two steps ahead);
+// waves synchronise only through LDS counters with bounded spin waits (an
+// expired wait aborts the launch and reports it in `status`).
 //
 // Acceptance: flip iff dE < neg_ln_u(w) * tscale[t] with w the Philox word of
 // counter (j>>2, replica, sweep, tag), i.e. with probability
@@ -62,10 +62,9 @@ __device__ __forceinline__ f32x16 mfma32(bf16x8 a, bf16x8 b, f32x16 c) {
 // order, so publishing needs only "all my LDS writes done" (lgkmcnt(0)) and a
 // compiler barrier -- not the vmcnt(0) wait that a workgroup release fence
 // implies, which would stall on this wave's outstanding operand loads.  The
-// counter accesses are written as ds instructions in asm for the same reason:
-// as C++ atomics the compiler cannot tell them apart from the LDS areas that
-// global_load_lds is filling and waits for every outstanding load first.
-// Each asm load waits for its own result (lgkmcnt(0)) inside the statement.
+// counters are accessed by ds instructions in asm, so no memory-model fence
+// or atomic lowering adds such a wait either; each asm load waits for its own
+// result (lgkmcnt(0)) inside the statement.
 __device__ __forceinline__ void lds_writes_done() {
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0), vmcnt/expcnt unconstrained
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -85,17 +84,6 @@ __device__ __forceinline__ int lds_load(const int *p) {
   int v;
   asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
   return v;
-}
-
-// 16-byte LDS read that waits for its own result.
-__device__ __forceinline__ uint4 lds_read16(const uint4 *p) {
-  uint4 v;
-  asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_addr(p)) : "memory");
-  return v;
-}
-
-__device__ __forceinline__ void lds_write_f32(float *p, float v) {
-  asm volatile("ds_write_b32 %0, %1" : : "v"(lds_addr(p)), "v"(v) : "memory");
 }
 
 // Publishes v in *p after all of this wave's earlier LDS writes.
@@ -124,14 +112,6 @@ __device__ __forceinline__ bool wait_ge(int *p, int target, int *abort) {
   return true;
 }
 
-// s_waitcnt vmcnt(N): at most N of this wave's vector memory loads (which
-// complete in issue order) still outstanding.
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
-}
-
 // thresholds of spins [16*bn + 4q, +4) of replica r = item/4 (q = item%4) for
 // sweep (sweep0+sw); spins beyond n never flip
 __device__ __forceinline__ void item_thresholds(int item, int bn, int sw, int n, uint32_t rbase,
@@ -154,13 +134,6 @@ __device__ __forceinline__ void store_thresholds(int item, const float (&v)[4],
   const int r = item >> 2, q = item & 3;
 #pragma unroll
   for (int jj = 0; jj < 4; ++jj) thr[r][4 * q + jj] = v[jj];
-}
-
-__device__ __forceinline__ void store_thresholds_asm(int item, const float (&v)[4],
-                                                     float (*thr)[17]) {
-  const int r = item >> 2, q = item & 3;
-#pragma unroll
-  for (int jj = 0; jj < 4; ++jj) lds_write_f32(&thr[r][4 * q + jj], v[jj]);
 }
 
 __device__ __forceinline__ void block_thresholds(int item, int bn, int sw, int n, uint32_t rbase,
@@ -199,65 +172,11 @@ __device__ __forceinline__ void load_a(const uint32_t *slot, const uint4 *lutp, 
   }
 }
 
-// Asynchronous copy of block `blk`'s diagonal 16x16 sub-block (1 KB) into LDS.
-__device__ __forceinline__ void stage_qd(const float *__restrict__ qdb, int blk, int lane,
-                                         uint4 *dst) {
-  __builtin_amdgcn_global_load_lds(
-      (__attribute__((address_space(1))) void *)(reinterpret_cast<const uint4 *>(qdb) + (size_t)blk * 64 + lane),
-      (__attribute__((address_space(3))) void *)dst, 16, 0, 0);
-}
 
-// Asynchronous copy (global_load_lds_dwordx4) of the Q operands of block
-// `blk` for the column tiles [t0, t1) of the wave whose columns start at col0:
-// dst[tt][p][l] receives lane l's B operand of tile tt, p = 0 (hi) / 1 (lo):
-// bf16 rows 16*blk + 8*kh .. +7 of column col0 + 32*tt + lc.
-__device__ __forceinline__ void stage_tiles(const uint4 *__restrict__ qpack, int np, int blk,
-                                            int col0, int t0, int t1, int lc, int kh,
-                                            uint4 (*dst)[2][64]) {
-  for (int tt = t0; tt < t1; ++tt)
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-      __builtin_amdgcn_global_load_lds(
-          (__attribute__((address_space(1))) void *)(qpack + ((size_t)blk * np + col0 + 32 * tt + lc) * 4 + 2 * p + kh),
-          (__attribute__((address_space(3))) void *)&dst[tt][p][0], 16, 0, 0);
-}
-
-// Non-owner step: wait for D_next and apply it to every tile with the
-// operands staged in this wave's LDS buffer (block sb_blk, staged now if
-// stale), then stage the following block's operands, which land while the
-// wave does other work.  Returns false if the wait expired.
-template <int TPW>
-__device__ __forceinline__ bool follow_one(const uint4 *__restrict__ qpack, int np, int col0,
-                                           int lc, int kh, int lane, int w, int G,
-                                           const uint32_t (*dring)[kRT], const uint4 *lutp,
-                                           const uint4 *lutn, int *d_count, int *wnext,
-                                           int *abort_flag, int &next,
-                                           int &sb_blk, uint4 (*sb)[2][64],
-                                           f32x16 (&C)[2][TPW]) {
-  const int nb = np / 16;
-  if (sb_blk != next % nb) {
-    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lc, kh, sb);
-    sb_blk = next % nb;
-  }
-  if (!wait_ge(d_count, next + 1, abort_flag)) return false;
-  bf16x8 A[2];
-  load_a(dring[next % kRing], lutp, lutn, lc, kh, A);
-  wait_vm<0>();  // the staged operands have landed
-#pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int tt = 0; tt < TPW; ++tt) {
-      const bf16x8 Bt = as_bf16x8(sb[tt][p][lane]);
-#pragma unroll
-      for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
-    }
-  ++next;
-  if (lane == 0) lds_store(&wnext[w], next);  // also waits for the reads of sb above
-  if (next < G) {
-    stage_tiles(qpack, np, next % nb, col0, 0, TPW, lc, kh, sb);
-    sb_blk = next % nb;
-  }
-  return true;
+// Column of lane lc in tile t of wave w: the tile holds blocks w + 8t (lanes
+// 0..15) and w + 8t + 4 (lanes 16..31).
+__device__ __forceinline__ int tile_col(int w, int t, int lc) {
+  return 16 * (w + 8 * t + 4 * (lc >> 4)) + (lc & 15);
 }
 
 }  // namespace
@@ -269,24 +188,20 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     uint32_t inst_base, int gpi, float *__restrict__ fields, uint32_t *__restrict__ xbits,
     uint32_t *__restrict__ bestbits, double *__restrict__ energy, double *__restrict__ ebest,
     int *__restrict__ status) {
-  static_assert(TPW <= 8, "half a tile per decision step");
+  static_assert(TPW >= 1 && TPW <= 8, "tiles per wave");
   constexpr int NP = kWaves * 32 * TPW;
   constexpr int NB = NP / 16;
   constexpr int NPW = NP / 32;
-  constexpr int BPW = 2 * TPW;  // blocks per wave
   __shared__ uint32_t dring[kRing][kRT];
   __shared__ uint4 lutp[256];
   __shared__ uint4 lutn[256];
-  __shared__ __align__(16) float fblk[32][kFT];  // block's column tile, column-major
+  __shared__ __align__(16) float fblk[32][kFT];  // the owner's tile, column-major
   __shared__ float thr[kTRing][kRT][17];
   __shared__ uint32_t xs[kRT][NPW + 1];
   __shared__ double es[kRT];
   __shared__ double ebs[kRT];
-  __shared__ __align__(16) uint4 qds[2][64];  // staged diagonal 16x16 sub-block of Q (fp32)
-  __shared__ __align__(16) uint4 ob[2][TPW][2][64];   // owner's Q operands, staged by the helpers
-  __shared__ __align__(16) uint4 sb[kWaves][TPW][2][64];  // each wave's own staged operands
+  __shared__ __align__(16) uint4 qds[64];  // the owner's fp32 16x16 diagonal sub-block
   __shared__ int d_count;
-  __shared__ int ob_cnt[2];
   __shared__ int thr_cnt[kTRing];
   __shared__ int wnext[kWaves];
   __shared__ int abort_flag;
@@ -304,7 +219,6 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   const uint32_t tag = ((inst_base + (uint32_t)inst) << 8) | TAG_SWEEP_THR;
   qpack += (size_t)inst * NB * NP * 4;
   qdb += (size_t)inst * NB * 256;
-  const int col0 = w * 32 * TPW;
   const int G = nsweeps * NB;
 
   for (int i = tid; i < kRT * NPW; i += 256)
@@ -326,13 +240,12 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   }
   if (tid < kTRing) thr_cnt[tid] = kHelpers;
   if (tid < kWaves) wnext[tid] = 0;
-  if (tid < 2) ob_cnt[tid] = 0;
   if (tid == 0) {
     d_count = 0;
     abort_flag = 0;
   }
   // accumulator element i of lane (lc, kh): replica 32m + 8(i/4) + 4kh + i%4,
-  // column col0 + 32tt + lc
+  // column tile_col(w, tt, lc)
   f32x16 C[2][TPW];
 #pragma unroll
   for (int m = 0; m < 2; ++m)
@@ -341,9 +254,9 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
 #pragma unroll
       for (int i = 0; i < 16; ++i)
         C[m][tt][i] = fields[(size_t)(rbase + 32 * m + 8 * (i >> 2) + 4 * kh + (i & 3)) * NP +
-                             col0 + 32 * tt + lc];
+                             tile_col(w, tt, lc)];
   // thresholds of the first kTRing blocks (256 items each); block g >= kTRing
-  // is computed by the helpers of step g - kLA
+  // is computed by the three non-owners of step g - kLA
   for (int it = tid; it < kTRing * 256; it += 256) {
     const int gblk = it >> 8;
     if (gblk < G)
@@ -352,36 +265,54 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   }
   __syncthreads();
 
-  int next = 0;  // D blocks applied to all of this wave's tiles
+  // B operands (rows of block x, all tiles): the next D block to apply
+  uint4 Bn[TPW][2];
+  auto load_ops = [&](int x) {
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt)
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        Bn[tt][p] = qpack[((size_t)x * NP + tile_col(w, tt, lc)) * 4 + 2 * p + kh];
+  };
+  // fetched a step early for this wave's next owner step: its tile's operands
+  // and its diagonal sub-block
+  uint4 Bo[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
+  uint4 qdn = make_uint4(0u, 0u, 0u, 0u);
+  if (w == 0) qdn = reinterpret_cast<const uint4 *>(qdb)[lane];  // block 0 (step 0)
+  load_ops(0);
+
   bool ok = true;
-  int sb_blk = 0;  // block whose operands this wave's sb buffer holds
-  stage_tiles(qpack, NP, 0, col0, 0, TPW, lc, kh, sb[w]);
-  if (w == 0) stage_qd(qdb, 0, lane, qds[0]);
-  // step g == G only drains: every wave applies the remaining D blocks
-  for (int g = 0; g <= G && ok; ++g) {
+  for (int g = 0; g < G && ok; ++g) {
     const int b = g % NB;
-    const int owner = g < G ? b / BPW : -1;
-    const int owner1 = ((g + 1) % NB) / BPW;  // owner of step g+1
+    const int owner = b & 3;
+    const int b1 = (g + 1) % NB;
+    bf16x8 A[2];
+    if (g > 0) {
+      ok = wait_ge(&d_count, g, &abort_flag);  // D_{g-1} published
+      if (!ok) break;
+      load_a(dring[(g - 1) % kRing], lutp, lutn, lc, kh, A);
+    }
+    if ((b1 & 3) == w && g + 1 < G) {
+      // next step is this wave's: its tile of block b1 needs D_g (rows of b)
+      const int t1 = (b1 >> 2) >> 1;
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+        Bo[p] = qpack[((size_t)b * NP + tile_col(w, t1, lc)) * 4 + 2 * p + kh];
+      qdn = reinterpret_cast<const uint4 *>(qdb)[(size_t)b1 * 64 + lane];
+    }
     if (owner == w) {
-      const int i = (b >> 1) % TPW;  // this wave's tile holding block b
-      const int half = b & 1;
-      bf16x8 A[2];
-      const uint4(*bt)[2][64] = ob[g & 1];  // D_{g-1}'s operands for this wave's tiles
-      if (g > 0) {
-        ok = wait_ge(&d_count, g, &abort_flag) &&
-             wait_ge(&ob_cnt[g & 1], kHelpers * ((g + 1) >> 1), &abort_flag);
-        if (!ok) break;
-        load_a(dring[(g - 1) % kRing], lutp, lutn, lc, kh, A);
-      }
-      if (g == 0) wait_vm<0>();  // qds[0] staged above
-      // (a) bring tile i up to date and transpose it (static indices, uniform branch)
+      const int k = b >> 2;
+      const int t = k >> 1;     // this wave's tile holding block b
+      const int half = k & 1;   // its lanes 16*half .. +15
+      qds[lane] = qdn;
+      // (a) bring tile t up to date and transpose it (static indices, uniform branch)
 #pragma unroll
       for (int tt = 0; tt < TPW; ++tt) {
-        if (tt != i) continue;
+        if (tt != t) continue;
         if (g > 0) {
 #pragma unroll
           for (int p = 0; p < 2; ++p) {
-            const bf16x8 Bt = as_bf16x8(bt[tt][p][lane]);
+            const bf16x8 Bt = as_bf16x8(Bo[p]);
 #pragma unroll
             for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
           }
@@ -404,40 +335,36 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) c[jj] = 0.0f;
       const uint32_t word = xs[lane][b >> 1];
-      const int sh = half * 16;
+      const int sh = (b & 1) * 16;
       uint32_t xb = (word >> sh) & 0xFFFFu;
       const uint32_t x0 = xb;
       float desum = 0.0f;
-      // Software-pipelined: step jj+1's field, threshold, sub-block row (row ==
-      // column: the diagonal block of a symmetric Q is symmetric) and the
-      // operand of the half tile updated in step jj+1 are read from LDS during
-      // step jj; sched_barrier keeps the compiler from hoisting all 16 steps'
-      // reads.
-      const float4 *qdv = reinterpret_cast<const float4 *>(qds[g & 1]);
+      // Software-pipelined: step jj+1's field, threshold and sub-block row
+      // (row == column: the diagonal block of a symmetric Q is symmetric) are
+      // read from LDS during step jj; sched_barrier keeps the compiler from
+      // hoisting all 16 steps' reads.
+      const float4 *qdv = reinterpret_cast<const float4 *>(qds);
       float fn = fcol[0], tn = tb_thr[lane][0];
       float4 qn[4];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) qn[k] = qdv[k];
-      uint4 bq = bt[0][0][lane];
+      for (int q = 0; q < 4; ++q) qn[q] = qdv[q];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
         const float fc = fn, tc = tn;
-        const uint4 bcur = bq;
-        if (jj + 1 < BPW) bq = bt[(jj + 1) >> 1][(jj + 1) & 1][lane];
         float qc[16];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          qc[4 * k] = qn[k].x;
-          qc[4 * k + 1] = qn[k].y;
-          qc[4 * k + 2] = qn[k].z;
-          qc[4 * k + 3] = qn[k].w;
+        for (int q = 0; q < 4; ++q) {
+          qc[4 * q] = qn[q].x;
+          qc[4 * q + 1] = qn[q].y;
+          qc[4 * q + 2] = qn[q].z;
+          qc[4 * q + 3] = qn[q].w;
         }
         if (jj < 15) {
           fn = fcol[(jj + 1) * kFT];
           tn = tb_thr[lane][jj + 1];
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (4 * k + 3 > jj + 1) qn[k] = qdv[(jj + 1) * 4 + k];
+          for (int q = 0; q < 4; ++q)
+            if (4 * q + 3 > jj + 1) qn[q] = qdv[(jj + 1) * 4 + q];
         }
         const float h = fc + c[jj];
         const bool xj = (xb >> jj) & 1u;
@@ -448,17 +375,6 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         desum += acc ? de : 0.0f;
 #pragma unroll
         for (int mm = jj + 1; mm < 16; ++mm) c[mm] = fmaf(qc[mm], d, c[mm]);
-        // (b) D_{g-1} into half (jj & 1) of tile jj/2 of this wave (tile i
-        //     already has it)
-        if (jj < BPW && g > 0 && (jj >> 1) != i) {
-          const bf16x8 Bt = as_bf16x8(bcur);
-#pragma unroll
-          for (int tt = 0; tt < TPW; ++tt) {
-            if (tt != (jj >> 1)) continue;
-#pragma unroll
-            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
-          }
-        }
         __builtin_amdgcn_sched_barrier(0);
       }
       xs[lane][b >> 1] = (word & ~(0xFFFFu << sh)) | (xb << sh);
@@ -479,118 +395,79 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         uint32_t *dst = bestbits + (size_t)(rbase + lane) * NPW;
         for (int q = 0; q < NPW; ++q) dst[q] = xs[lane][q];
       }
-      next = g;  // every tile now holds D_0 .. D_{g-1}
-      if (lane == 0) {
-        lds_store(&d_count, g + 1);
-        lds_store(&wnext[w], next);
-      }
-      if (owner1 != w && g + 1 < G) {
-        // end of the turn: this wave applies D_g next, as a non-owner
-        stage_tiles(qpack, NP, b, col0, 0, TPW, lc, kh, sb[w]);
-        sb_blk = b;
-      }
-    } else {
-      const bool live = g < G;
-      const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1 (live steps)
-      const bool staging = g + 1 < G;
-      const int lo = hw * TPW / kHelpers, hi = (hw + 1) * TPW / kHelpers;
-      // step g-1 is done: D_{g-1} is published, and the owner has finished
-      // reading ob[(g+1)&1] and threshold slot (g+kLA)%kTRing
-      if (live) {
-        ok = wait_ge(&d_count, g, &abort_flag);
-        if (!ok) break;
-      }
-      // (1) catch up to D_{g-2} (at the drain step: apply everything left)
-      const int target = live ? g - 1 : G;
-      while (ok && next < target)
-        ok = follow_one<TPW>(qpack, NP, col0, lc, kh, lane, w, G, dring, lutp, lutn, &d_count,
-                             wnext, &abort_flag, next, sb_blk, sb[w], C);
-      if (!ok) break;
-      bool restaged = false;  // 2*TPW operand loads issued after the owner's
-      if (live && g > 0) {
-        // (2) D_{g-1} into every tile, with the thresholds of block g+kLA
-        //     computed under the MFMAs and the owner's operands for step g+1
-        //     (block b) copied meanwhile.  After that copy is issued, this
-        //     step touches LDS only through asm, which the compiler does not
-        //     make wait for it.
-        const int blk = next % NB;  // next == g-1
-        if (sb_blk != blk) {
-          stage_tiles(qpack, NP, blk, col0, 0, TPW, lc, kh, sb[w]);
-          sb_blk = blk;
-        }
-        bf16x8 A[2];
-        load_a(dring[next % kRing], lutp, lutn, lc, kh, A);
-        wait_vm<0>();  // this wave's operands have landed
-        if (staging) {
-          stage_tiles(qpack, NP, b, owner1 * 32 * TPW, lo, hi, lc, kh, ob[(g + 1) & 1]);
-          if (hw == kHelpers - 1) stage_qd(qdb, (g + 1) % NB, lane, qds[(g + 1) & 1]);
-        }
-        const int g2 = min(g + kLA, G - 1);
-        const float ts2 = tscale[g2 / NB];
-        // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
-        // the 10 Philox rounds and 4 logarithms are spread over the operand
-        // reads, so each piece of VALU work runs while two MFMAs execute
-        const int item = hw * 64 + lane;
-        const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
-        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
-                              (uint32_t)(sweep0 + g2 / NB), tag);
-        uint32_t pk0 = k0, pk1 = k1;
-        float tv[4];
+      if (lane == 0) lds_store(&d_count, g + 1);
+      // (b) after publishing: D_{g-1} into the other tiles
+      if (g > 0) {
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
           for (int tt = 0; tt < TPW; ++tt) {
-            const bf16x8 Bt = as_bf16x8(lds_read16(&sb[w][tt][p][lane]));
+            if (tt == t) continue;
+            const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
 #pragma unroll
             for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
-            const int slot = p * TPW + tt;
+          }
+      }
+    } else if (g > 0) {
+      // D_{g-1} into every tile, with the thresholds of block g+kLA computed
+      // under the MFMAs (their slot was last read in step g+kLA-kTRing < g)
+      const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
+      const int g2 = min(g + kLA, G - 1);
+      const float ts2 = tscale[g2 / NB];
+      // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
+      // the 10 Philox rounds and 4 logarithms are spread over the MFMA pairs
+      const int item = hw * 64 + lane;
+      const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
+      uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
+                            (uint32_t)(sweep0 + g2 / NB), tag);
+      uint32_t pk0 = k0, pk1 = k1;
+      float tv[4];
 #pragma unroll
-            for (int u = 0; u < 14; ++u) {
-              if ((u * 2 * TPW) / 14 != slot) continue;
-              if (u < 10) {
-                philox_round(ph, pk0, pk1, u);
-              } else {
-                const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
-                tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
-              }
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
+#pragma unroll
+          for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+          const int slot = p * TPW + tt;
+#pragma unroll
+          for (int u = 0; u < 14; ++u) {
+            if ((u * 2 * TPW) / 14 != slot) continue;
+            if (u < 10) {
+              philox_round(ph, pk0, pk1, u);
+            } else {
+              const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
+              tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
             }
           }
-        // (a slot that no later block uses may receive values past the end)
+        }
+      if (g + kLA < G) {
         const int sl = (g + kLA) % kTRing;
-        store_thresholds_asm(item, tv, thr[sl]);
-        ++next;
-        if (lane == 0) lds_store(&wnext[w], next);
-        if (g + kLA < G) {
-          if (hw == g % kHelpers) {  // the fourth quarter rotates
-            item_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
-                            tv);
-            store_thresholds_asm(192 + lane, tv, thr[sl]);
-          }
-          if (lane == 0) lds_add(&thr_cnt[sl]);
-        }
-        // stage D_g's operands unless this wave owns step g+1 (it then applies
-        // D_g from the owner's area)
-        const bool own_next = next + 1 < G && ((next + 1) % NB) / BPW == w;
-        if (next < G && !own_next) {
-          stage_tiles(qpack, NP, next % NB, col0, 0, TPW, lc, kh, sb[w]);
-          sb_blk = next % NB;
-          restaged = true;
-        } else {
-          sb_blk = -1;
-        }
-      } else if (staging) {
-        stage_tiles(qpack, NP, b, owner1 * 32 * TPW, lo, hi, lc, kh, ob[(g + 1) & 1]);
-        if (hw == kHelpers - 1) stage_qd(qdb, (g + 1) % NB, lane, qds[(g + 1) & 1]);
+        store_thresholds(item, tv, thr[sl]);
+        if (hw == g % kHelpers)  // the fourth quarter rotates
+          block_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
+                           thr[sl]);
+        if (lane == 0) lds_add(&thr_cnt[sl]);
       }
-      // (3) the owner's operands have landed once at most the loads issued
-      //     after them are outstanding
-      if (staging) {
-        if (restaged)
-          wait_vm<2 * TPW>();
-        else
-          wait_vm<0>();
-        if (lane == 0) lds_add(&ob_cnt[(g + 1) & 1]);
-      }
+    }
+    // D_0 .. D_{g-1} are in every tile of this wave; fetch D_g's operands
+    if (lane == 0) lds_store(&wnext[w], g);
+    load_ops(b);
+  }
+  // drain: D_{G-1} into every tile
+  if (ok && G > 0) {
+    ok = wait_ge(&d_count, G, &abort_flag);
+    if (ok) {
+      bf16x8 A[2];
+      load_a(dring[(G - 1) % kRing], lutp, lutn, lc, kh, A);
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
+#pragma unroll
+          for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+        }
     }
   }
   if (!ok && lane == 0) lds_store(&abort_flag, 1);
@@ -600,8 +477,8 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     for (int tt = 0; tt < TPW; ++tt)
 #pragma unroll
       for (int i = 0; i < 16; ++i)
-        fields[(size_t)(rbase + 32 * m + 8 * (i >> 2) + 4 * kh + (i & 3)) * NP + col0 + 32 * tt +
-               lc] = C[m][tt][i];
+        fields[(size_t)(rbase + 32 * m + 8 * (i >> 2) + 4 * kh + (i & 3)) * NP +
+               tile_col(w, tt, lc)] = C[m][tt][i];
   __syncthreads();
   for (int i = tid; i < kRT * NPW; i += 256)
     xbits[(size_t)(rbase + i / NPW) * NPW + i % NPW] = xs[i / NPW][i % NPW];
