@@ -32,23 +32,36 @@ extern "C" {
 
 namespace {
 
-// Allocates a batch of I instances of size n and packs the FP64 matrices
-// already in d->q64 (device) into the MFMA operand layout.
+constexpr int kMaxStreamN = 16384;
+
+// Allocates a batch of I instances of size n; pack() then fills the MFMA
+// operand layout from the FP64 matrices in d->q64.
 os::Dense *dense_alloc(Ctx &c, int I, int n) {
-  if (n > os::dense_max_n())
-    throw os::Error(OS_ERR_UNSUPPORTED, "dense sweep path supports n <= " +
-                                            std::to_string(os::dense_max_n()));
+  if (n > kMaxStreamN)
+    throw os::Error(OS_ERR_UNSUPPORTED,
+                    "dense sweep path supports n <= " + std::to_string(kMaxStreamN));
   auto d = std::make_unique<Dense>();
   d->ctx = &c;
   d->I = I;
   d->n = n;
-  d->np = os::dense_padded_n(n);
+  d->stream = n > os::dense_max_n();
+  const int kb = d->stream ? os::stream_block() : 16;
+  d->np = d->stream ? (n + kb - 1) / kb * kb : os::dense_padded_n(n);
   d->npw = d->np / 32;
-  const int np = d->np, nb = np / 16;
+  const int np = d->np, nb = np / kb;
   d->q64.ensure(sizeof(double) * (size_t)n * n * I);
-  d->qpack.ensure(sizeof(uint16_t) * (size_t)nb * np * 32 * I);
-  d->qdb.ensure(sizeof(float) * (size_t)nb * 256 * I);
+  d->qpack.ensure(sizeof(uint16_t) * (size_t)nb * np * 2 * kb * I);
+  d->qdb.ensure(sizeof(float) * (size_t)nb * kb * kb * I);
   return d.release();
+}
+
+void pack(Dense &d) {
+  if (d.stream)
+    os::launch_pack_q64(d.ctx->stream, d.I, d.n, d.np, d.q64.as<double>(), d.qpack.as<uint16_t>(),
+                        d.qdb.as<float>());
+  else
+    os::launch_pack_q(d.ctx->stream, d.I, d.n, d.np, d.q64.as<double>(), d.qpack.as<uint16_t>(),
+                      d.qdb.as<float>());
 }
 
 }  // namespace
@@ -64,8 +77,7 @@ int os_dense_batch_create(os_ctx *ctx_, int num_instances, int n, const double *
     std::unique_ptr<Dense> d(dense_alloc(c, num_instances, n));
     OS_HIP_CHECK(hipMemcpyAsync(d->q64.ptr, q_sym, sizeof(double) * (size_t)n * n * num_instances,
                                 hipMemcpyHostToDevice, c.stream));
-    os::launch_pack_q(c.stream, num_instances, n, d->np, d->q64.as<double>(),
-                      d->qpack.as<uint16_t>(), d->qdb.as<float>());
+    pack(*d);
     c.sync();
     *out = reinterpret_cast<os_dense *>(d.release());
   });
@@ -81,8 +93,7 @@ int os_dense_batch_create_sk(os_ctx *ctx_, int num_instances, int n, uint64_t se
     c.activate();
     std::unique_ptr<Dense> d(dense_alloc(c, num_instances, n));
     os::launch_gen_sk(c.stream, num_instances, n, seed, instance_base, d->q64.as<double>());
-    os::launch_pack_q(c.stream, num_instances, n, d->np, d->q64.as<double>(),
-                      d->qpack.as<uint16_t>(), d->qdb.as<float>());
+    pack(*d);
     c.sync();
     *out = reinterpret_cast<os_dense *>(d.release());
   });
@@ -192,6 +203,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
                                 hipMemcpyDeviceToDevice, st));
     OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), st));
     int launches = 0;
+    if (d.stream) d.dblk.ensure(sizeof(uint4) * 2 * T);
     for (int s0 = 0; s0 < S; s0 += seg) {
       const int ns = std::min(seg, S - s0);
       hipEvent_t a, b;
@@ -200,13 +212,33 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       OS_HIP_CHECK(hipEventCreate(&b));
       ev.push_back(b);
       OS_HIP_CHECK(hipEventRecord(a, st));
-      os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
-                              d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
-                              d.fields.as<float>(), d.xbits.as<uint32_t>(),
-                              d.bestbits.as<uint32_t>(), d.energy.as<double>(),
-                              d.ebest.as<double>(), d.status.as<int>());
+      if (d.stream) {
+        // one pass over the fields per block of 64 spins, then one more that
+        // applies the last block's flips
+        const int nb = np / os::stream_block();
+        uint4 *db = d.dblk.as<uint4>();
+        const int steps = ns * nb;
+        for (int g = 0; g <= steps; ++g) {
+          const int blk = g < steps ? g % nb : -1;
+          const int bprev = g > 0 ? (g - 1) % nb : -1;
+          const int sw = s0 + (g < steps ? g / nb : 0);
+          os::launch_stream_block(st, I, n, np, Rp, blk, bprev, sw, g < steps ? p->tscale[sw] : 0.f,
+                                  p->seed, p->instance, blk == nb - 1, d.qpack.as<uint4>(),
+                                  d.qdb.as<float>(), d.fields.as<float>(),
+                                  d.xbits.as<uint32_t>(), db + (size_t)((g + 1) & 1) * T,
+                                  db + (size_t)(g & 1) * T, d.energy.as<double>(),
+                                  d.ebest.as<double>(), d.bestbits.as<uint32_t>());
+          ++launches;
+        }
+      } else {
+        os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
+                                d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
+                                d.fields.as<float>(), d.xbits.as<uint32_t>(),
+                                d.bestbits.as<uint32_t>(), d.energy.as<double>(),
+                                d.ebest.as<double>(), d.status.as<int>());
+        ++launches;
+      }
       OS_HIP_CHECK(hipEventRecord(b, st));
-      ++launches;
       // FP64 refresh of fields and running energies; records the fp32 drift
       energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), d.fields.as<float>());
     }

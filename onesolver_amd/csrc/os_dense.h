@@ -12,11 +12,15 @@ struct Dense {
   Ctx *ctx = nullptr;
   int I = 1;  // instances (same n), stacked in every array
   int n = 0, np = 0, npw = 0;
+  // n > dense_max_n(): fields in HBM, blocks of 64 spins (anneal_stream.hip);
+  // qpack is then [I][n/64][np][128] and qdb [I][n/64][64][64]
+  bool stream = false;
   DevBuf q64, qpack, qdb;
   DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
+  DevBuf dblk;  // stream path: flip masks of the last two blocks
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
-                    &scratch, &drift, &bytes, &marg, &status})
+                    &scratch, &drift, &bytes, &marg, &status, &dblk})
       b->release();
   }
 };

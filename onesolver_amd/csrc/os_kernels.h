@@ -44,6 +44,18 @@ void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uin
 void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
                       const float *p, uint32_t *xbits, int num_instances = 1);
 
+// anneal_stream.hip (dense n > dense_max_n(): fields in HBM, blocks of 64)
+int stream_block();
+void launch_pack_q64(hipStream_t st, int I, int n, int np, const double *q, uint16_t *qpack,
+                     float *qdb);
+// one pass: apply D (dprev) of block bprev (if >= 0) to all fields, then decide
+// block b (if >= 0) of sweep `sweep` into dcur
+void launch_stream_block(hipStream_t st, int I, int n, int np, int Rp, int b, int bprev,
+                         int sweep, float ts, uint64_t seed, uint32_t inst_base,
+                         bool last_of_sweep, const uint4 *qpack, const float *qdb, float *F,
+                         uint32_t *xbits, const uint4 *dprev, uint4 *dcur, double *energy,
+                         double *ebest, uint32_t *bestbits);
+
 // gen.hip
 // q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed
 void launch_gen_sk(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_base, double *q);

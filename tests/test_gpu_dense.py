@@ -93,7 +93,37 @@ def test_dense_start_states_and_marginals(ctx):
 
 def test_dense_rejects_oversized(ctx):
     with pytest.raises(onesolver_amd.OneSolverError):
-        onesolver_amd.DenseProblem(ctx, np.zeros((2000, 2000)))
+        onesolver_amd.DenseBatch(ctx, sk=(1, 20000, 0, 0))
+
+
+@pytest.mark.parametrize("n,R,S,refresh", [(1100, 64, 3, 0), (2048, 100, 2, 1)])
+def test_dense_streaming_path_matches_oracle_bit_for_bit(ctx, n, R, S, refresh):
+    """n > 1024: fields in HBM, blocks of 64 spins, one pass per block."""
+    q = _int_instance(n, n + 1)
+    ts = _tscale(S)
+    print(f"[gpu] dense streaming n={n} R={R} S={S}", flush=True)
+    prob = onesolver_amd.DenseProblem(ctx, q)
+    out = prob.anneal(R, ts, seed=9, instance=2, refresh_interval=refresh, final_states=True)
+    fin, bs, be = oracle.sweep_dense(q, n, ts, R, seed=9, instance=2)
+    assert (out["final_states"] == fin).all()
+    assert (out["best_states"] == bs).all()
+    np.testing.assert_allclose(out["best_energies"], oracle.energy(q, bs), rtol=1e-12)
+    assert out["stats"]["max_field_drift"] == 0.0
+
+
+def test_dense_streaming_sk_energies_and_drift(ctx):
+    n, R, S = 2048, 64, 6
+    b = onesolver_amd.DenseBatch(ctx, sk=(1, n, 4, 0))
+    q = b.q()[0]
+    ts = _tscale(S)
+    print("[gpu] dense streaming SK n=2048", flush=True)
+    out = b.anneal(R, ts, seed=2, refresh_interval=3)
+    e64 = oracle.energy(q, out["best_states"][0])
+    np.testing.assert_allclose(out["best_energies"][0], e64, rtol=1e-9)
+    assert 0.0 < out["stats"]["max_field_drift"] < 1e-3
+    # a few sweeps from random states already go far below the random-state energy
+    assert out["best_energy"][0] < -0.3 * n
+    b.close()
 
 
 def test_dense_batch_equals_single_instance_runs(ctx):
