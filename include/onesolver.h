@@ -125,6 +125,11 @@ int os_dense_batch_create(os_ctx *ctx, int num_instances, int n, const double *q
  * i): no host generation or transfer.  No reference counterpart. */
 int os_dense_batch_create_sk(os_ctx *ctx, int num_instances, int n, uint64_t seed,
                              uint32_t instance_base, os_dense **out);
+/* Dense uniform instances (Q_ij, i <= j, i.i.d. U[-1,1) fp32, mirrored),
+ * generated on the GPU from Philox key (seed, instance_base + i).  No
+ * reference counterpart. */
+int os_dense_batch_create_uniform(os_ctx *ctx, int num_instances, int n, uint64_t seed,
+                                  uint32_t instance_base, os_dense **out);
 /* Copy the FP64 matrices of a dense batch to the host ([I x n x n]). */
 int os_dense_copy_q(os_dense *dense, double *q_out);
 int os_dense_destroy(os_dense *dense);

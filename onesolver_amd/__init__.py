@@ -215,18 +215,25 @@ class DenseProblem:
 class DenseBatch:
     """Same-size dense QUBOs annealed together (os_dense_batch_create)."""
 
-    def __init__(self, ctx: Context, q_batch=None, *, sk=None):
+    def __init__(self, ctx: Context, q_batch=None, *, sk=None, uniform=None):
         """q_batch: [I, n, n] FP64 matrices; or sk=(I, n, seed, instance_base)
-        to generate SK instances on the GPU."""
+        / uniform=(I, n, seed, instance_base) to generate SK / dense uniform
+        instances on the GPU."""
         L = lib()
         L.os_dense_batch_create.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p,
                                             ctypes.POINTER(_vp)]
         L.os_dense_batch_create_sk.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                ctypes.c_uint32, ctypes.POINTER(_vp)]
         L.os_dense_copy_q.argtypes = [_vp, _f64p]
+        L.os_dense_batch_create_uniform.argtypes = L.os_dense_batch_create_sk.argtypes
         self.ctx = ctx
         h = _vp()
-        if sk is not None:
+        if uniform is not None:
+            I, n, seed, base = uniform
+            self.I, self.n = int(I), int(n)
+            _check(L.os_dense_batch_create_uniform(ctx.handle, self.I, self.n, seed, base,
+                                                   ctypes.byref(h)))
+        elif sk is not None:
             I, n, seed, base = sk
             self.I, self.n = int(I), int(n)
             _check(L.os_dense_batch_create_sk(ctx.handle, self.I, self.n, seed, base,

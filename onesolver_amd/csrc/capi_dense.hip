@@ -99,6 +99,22 @@ int os_dense_batch_create_sk(os_ctx *ctx_, int num_instances, int n, uint64_t se
   });
 }
 
+int os_dense_batch_create_uniform(os_ctx *ctx_, int num_instances, int n, uint64_t seed,
+                                  uint32_t instance_base, os_dense **out) {
+  return dguard([&] {
+    OS_REQUIRE(ctx_ && out, "null argument");
+    OS_REQUIRE(n >= 1, "n must be positive");
+    OS_REQUIRE(num_instances >= 1, "num_instances must be positive");
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    std::unique_ptr<Dense> d(dense_alloc(c, num_instances, n));
+    os::launch_gen_uniform(c.stream, num_instances, n, seed, instance_base, d->q64.as<double>());
+    pack(*d);
+    c.sync();
+    *out = reinterpret_cast<os_dense *>(d.release());
+  });
+}
+
 int os_dense_copy_q(os_dense *d_, double *q_out) {
   return dguard([&] {
     OS_REQUIRE(d_ && q_out, "null argument");

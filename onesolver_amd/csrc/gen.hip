@@ -1,5 +1,8 @@
 // gen.hip -- dense instances generated and packed on the GPU.
 //
+// Also the uniform instances of the dense configurations (Q_ij i.i.d.
+// U[-1,1) fp32 on and above the diagonal), restating instances.dense_uniform.
+//
 // SK instances of the dense benchmark distribution are produced in device
 // memory (no host generation or PCIe transfer), so that batches of thousands
 // of N=1024 instances for training and evaluation cost only HBM bandwidth.
@@ -13,6 +16,7 @@ namespace os {
 
 namespace {
 
+constexpr uint32_t TAG_GEN_UNIFORM = 0xC1;
 constexpr uint32_t TAG_GEN_GAUSS = 0xC2;
 
 __device__ __forceinline__ uint16_t bf16_rne(float f) {
@@ -42,6 +46,22 @@ __global__ __launch_bounds__(256) void gen_sk_offdiag_kernel(int n, uint32_t k0,
   const double z = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
   const float v = (float)(z / sqrt((double)n));
   qs[idx] = (double)(4.0f * v);
+}
+
+// Dense uniform instances (instances.py, dense_uniform): Q_ij, i <= j, =
+// fp32((w >> 8) * 2^-23 - 1) from counter (i*n + j, 0, 0, TAG_GEN_UNIFORM),
+// mirrored below the diagonal.
+__global__ __launch_bounds__(256) void gen_uniform_kernel(int n, uint32_t k0, uint32_t inst_base,
+                                                          double *__restrict__ q) {
+  const int s = blockIdx.y;
+  const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  const uint64_t a = (uint64_t)min(i, j), b = (uint64_t)max(i, j);
+  const uint64_t e = a * (uint64_t)n + b;
+  const uint4 o = philox4x32_10(
+      make_uint4((uint32_t)e, (uint32_t)(e >> 32), 0u, TAG_GEN_UNIFORM), k0, inst_base + (uint32_t)s);
+  q[(size_t)s * n * n + idx] = (double)((float)(o.x >> 8) * 0x1p-23f - 1.0f);
 }
 
 __global__ __launch_bounds__(256) void gen_sk_diag_kernel(int I, int n, double *__restrict__ q) {
@@ -88,6 +108,14 @@ void launch_gen_sk(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_ba
       n, (uint32_t)seed, inst_base, q);
   OS_HIP_CHECK(hipGetLastError());
   gen_sk_diag_kernel<<<(unsigned)(((size_t)I * n + 255) / 256), 256, 0, st>>>(I, n, q);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gen_uniform(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_base,
+                        double *q) {
+  const size_t nn = (size_t)n * n;
+  gen_uniform_kernel<<<dim3((unsigned)((nn + 255) / 256), I), 256, 0, st>>>(n, (uint32_t)seed,
+                                                                            inst_base, q);
   OS_HIP_CHECK(hipGetLastError());
 }
 

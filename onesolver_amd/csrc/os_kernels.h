@@ -59,6 +59,9 @@ void launch_stream_block(hipStream_t st, int I, int n, int np, int Rp, int b, in
 // gen.hip
 // q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed
 void launch_gen_sk(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_base, double *q);
+// q[s] = dense uniform instance inst_base + s (upper triangle incl. diagonal U[-1,1), mirrored)
+void launch_gen_uniform(hipStream_t st, int I, int n, uint64_t seed, uint32_t inst_base,
+                        double *q);
 // bf16 hi+lo operand packing and fp32 diagonal sub-blocks of I instances
 void launch_pack_q(hipStream_t st, int I, int n, int np, const double *q, uint16_t *qpack,
                    float *qdb);

@@ -175,3 +175,12 @@ def test_sk_instances_generated_on_the_gpu_match_the_host_generator(ctx):
     up = onesolver_amd.DenseBatch(ctx, qg).anneal(64, ts, seed=3, instance=20)
     assert (out["best_states"] == up["best_states"]).all()
     b.close()
+
+
+def test_uniform_instances_generated_on_the_gpu_match_the_host_generator(ctx):
+    """Bit for bit: the uniform generator uses only exact fp32 arithmetic."""
+    b = onesolver_amd.DenseBatch(ctx, uniform=(2, 70, 0, 5))
+    qg = b.q()
+    for s in range(2):
+        assert (qg[s] == instances.dense_uniform(70, seed=0, instance=5 + s)).all()
+    b.close()
