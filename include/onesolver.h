@@ -104,13 +104,18 @@ typedef struct os_sweep_stats {
   int32_t kernel_launches; /* sweep-kernel launches */
   double max_field_drift;  /* max |h_fp32 - h_fp64| over all refreshes */
   uint64_t attempts;       /* num_replicas x num_sweeps x n spin-flip attempts */
+  double pass_avg_ms;      /* dense n > 1024: average duration of one field pass
+                              (HIP events around a sample of the passes); else 0 */
+  int32_t pass_launches;   /* dense n > 1024: field passes; else 0 */
 } os_sweep_stats;
 
 typedef struct os_dense os_dense;
 
 /* Upload a dense QUBO (q_sym: n x n FP64 symmetric, the reference's flat
  * matrix) for sweep-mode annealing; keeps Q as FP64 and as bf16 hi+lo pairs
- * on the GPU.  n <= 1024 in this version. */
+ * on the GPU.  n <= 1024: fields in registers, one launch per segment;
+ * 1024 < n <= 16384: fields in HBM, blocks of 64 spins (os_sweep_stats
+ * reports the field passes). */
 int os_dense_create(os_ctx *ctx, int n, const double *q_sym, os_dense **out);
 /* A batch of num_instances same-size dense QUBOs (q_sym: num_instances
  * consecutive n x n matrices), annealed together in one launch per segment;

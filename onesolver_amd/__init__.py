@@ -78,7 +78,8 @@ class SweepStats(ctypes.Structure):
     """os_sweep_stats of include/onesolver.h."""
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("kernel_launches", ctypes.c_int32), ("max_field_drift", ctypes.c_double),
-                ("attempts", ctypes.c_uint64)]
+                ("attempts", ctypes.c_uint64), ("pass_avg_ms", ctypes.c_double),
+                ("pass_launches", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1,17 +1,25 @@
 // anneal_stream.hip -- sweep-mode annealing of large dense QUBOs (n > 1024).
 //
 // When R x N fp32 fields no longer fit the register file (N=8192, R=4096:
-// 128 MiB), they live in HBM and every block of k = 64 spins costs one pass:
-// launch g reads each 64x64 field tile, applies the previous block's flip
-// deltas (F += D_{g-1} * Qhi + D_{g-1} * Qlo over the 64 rows of block
-// g-1, bf16 v_mfma_f32_32x32x16_bf16, fp32 accumulation) and writes it back;
-// the work-groups holding block b = g mod NB then decide its 64 spins for
-// their 64 replicas (lane = replica, in order, in-block corrections with the
-// fp32 64x64 diagonal sub-block of Q) and publish D_g as +/- bit masks.  So
-// each launch moves the field array through HBM once (read + write) and the
-// decisions ride along in a few work-groups.  Thresholds are the sweep-mode
-// ones of the register-resident kernel: flip iff dE < neg_ln_u(w) *
-// tscale[t], w = Philox word of counter (j>>2, replica, sweep, tag).
+// 128 MiB), they live in HBM and every block of k = 64 spins g costs one pass
+// over them, in two launches that run side by side on two streams:
+//  - update(g) reads every 64x64 field tile except those of block b_g =
+//    g mod NB, adds the previous block's flips (D_{g-1} * Qhi + D_{g-1} * Qlo
+//    over the 64 rows of block b_{g-1}, bf16 v_mfma_f32_32x32x16_bf16, fp32
+//    accumulation) and writes it back; it uses no LDS, so the HBM stream runs
+//    at full occupancy;
+//  - decide(g) (one work-group per 64 replicas) does the same for the tiles
+//    of block b_g, then decides its 64 spins in order (lane = replica,
+//    in-block corrections with the fp32 64x64 diagonal sub-block of Q) and
+//    publishes D_g as +/- bit masks.
+// Both need only D_{g-1}, so the decisions leave the critical path: update(g+1)
+// waits for update(g) and decide(g), decide(g+1) for the same two.  Each
+// block moves the field array through HBM once (read + write).  Thresholds
+// are the sweep-mode ones of the register-resident kernel: flip iff dE <
+// neg_ln_u(w) * tscale[t], w = Philox word of counter (j>>2, replica, sweep,
+// tag).
+#include <algorithm>
+
 #include "os_kernels.h"
 
 namespace os {
@@ -29,121 +37,353 @@ __device__ __forceinline__ sf32x16 smfma(uint4 a, uint4 b, sf32x16 c) {
                                                  __builtin_bit_cast(sbf16x8, b), c, 0, 0, 0);
 }
 
-// 8 spins (bits of `plus` / `minus` at offset sh) as 8 bf16: +1, -1 or 0
+}  // namespace
+
+constexpr int kChunk = 2;     // decide launch: blocks whose operands load in one round trip
+constexpr int kPRows = 256;   // pass launch: replicas per tile (4 waves x 64)
+constexpr int kBStride = 272; // pass launch: bytes per staged operand column (256 + pad)
+constexpr int kQ4 = 20;       // decide launch: floats per (spin, lane-in-quad) row of qd4
+
+// 8 spins (bits sh..sh+7 of the +1 / -1 flip masks) as 8 bf16 +1 / -1 / 0:
+// with y = a | a << 15, (y >> 2q) & 0x10001 puts bits 2q and 2q+1 of a at
+// bits 0 and 16, and 0x3F80 / 0x8000 are bf16 1.0 and the sign bit.
 __device__ __forceinline__ uint4 spins8(uint32_t plus, uint32_t minus, int sh) {
+  const uint32_t a = ((plus | minus) >> sh) & 0xFFu, m = (minus >> sh) & 0xFFu;
+  const uint32_t ya = a | (a << 15), ym = m | (m << 15);
   uint32_t v[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const uint32_t p0 = (plus >> (sh + 2 * q)) & 1u, p1 = (plus >> (sh + 2 * q + 1)) & 1u;
-    const uint32_t m0 = (minus >> (sh + 2 * q)) & 1u, m1 = (minus >> (sh + 2 * q + 1)) & 1u;
-    v[q] = (p0 ? 0x3F80u : 0u) | (m0 ? 0xBF80u : 0u) | (p1 ? 0x3F800000u : 0u) |
-           (m1 ? 0xBF800000u : 0u);
+    const uint32_t xa = (ya >> (2 * q)) & 0x10001u, xm = (ym >> (2 * q)) & 0x10001u;
+    v[q] = xa * 0x3F80u + xm * 0x8000u;
   }
   return make_uint4(v[0], v[1], v[2], v[3]);
 }
 
-}  // namespace
+// Operands of up to K blocks for one wave's 32x32 part of a 64x64 field
+// tile: the flip masks (plus lo, plus hi, minus lo, minus hi) of its 32
+// replicas and the bf16 hi / lo rows of Q in the B layout of
+// v_mfma_f32_32x32x16_bf16.  qcol points at the wave's column of qpack for
+// block 0 of its instance; the flip masks of block g are at
+// dring[(g % ring) * T + drow].
+template <int K>
+struct TileOps {
+  uint4 dd[K];
+  uint4 B[K][8];
+};
 
-// Grid (np/64 column blocks, Rp/64 replica groups, I instances), 256 threads.
-// bprev >= 0: apply D (dprev) of block bprev to the tile; b >= 0: the tiles of
-// column block b decide block b of sweep `sweep` into dcur.
-__global__ __launch_bounds__(256) void stream_block_kernel(
-    int n, int np, int Rp, int b, int bprev, int sweep, float ts, uint32_t k0, uint32_t k1,
-    uint32_t inst_base, int last_of_sweep, const uint4 *__restrict__ qpack,
-    const float *__restrict__ qdb, float *__restrict__ F, uint32_t *__restrict__ xbits,
-    const uint4 *__restrict__ dprev, uint4 *__restrict__ dcur, double *__restrict__ energy,
-    double *__restrict__ ebest, uint32_t *__restrict__ bestbits) {
-  __shared__ __align__(16) float fl[kSB][kST];   // decided block: column-major fields
-  __shared__ __align__(16) float qd[kSB][kSB];   // its diagonal sub-block
-  __shared__ float thr[kSB][kSB + 1];            // thresholds [replica][spin]
-  const int cx = blockIdx.x, ry = blockIdx.y, s = blockIdx.z;
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int lc = lane & 31, kh = lane >> 5;
-  const int mt = w >> 1, nt = w & 1;
-  const int NB = np / kSB, npw = np / 32;
-  const size_t row0 = (size_t)s * Rp + (size_t)ry * 64;  // first state row of the group
-  const int col = cx * kSB + 32 * nt + lc;
-  const bool decide = b >= 0 && cx == b;
-  float *Fp = F + (row0 + 32 * mt + 4 * kh) * np + col;
+template <int K>
+__device__ __forceinline__ void ops_load(TileOps<K> &o, int g0, int cnt, int np, int NB, int ring,
+                                         size_t T, size_t drow, const uint4 *qcol,
+                                         const uint4 *__restrict__ dring, int kh) {
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    if (t < cnt) {
+      const int g = g0 + t;
+      o.dd[t] = dring[(size_t)(g % ring) * T + drow];
+      const uint4 *qb = qcol + (size_t)(g % NB) * np * 16;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        o.B[t][k] = qb[2 * k + kh];
+        o.B[t][4 + k] = qb[8 + 2 * k + kh];
+      }
+    }
+  }
+}
+
+// C += D_g * Qhi + D_g * Qlo for the (up to K) blocks in `o`, in order.
+template <int K>
+__device__ __forceinline__ void ops_apply(sf32x16 &C, const TileOps<K> &o, int cnt, int kh) {
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    if (t < cnt) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const uint32_t pw = ks < 2 ? o.dd[t].x : o.dd[t].y, mw = ks < 2 ? o.dd[t].z : o.dd[t].w;
+        const uint4 a = spins8(pw, mw, 16 * (ks & 1) + 8 * kh);
+        C = smfma(a, o.B[t][ks], C);
+        C = smfma(a, o.B[t][4 + ks], C);
+      }
+    }
+  }
+}
+
+// A 32x32 part of a field tile in the C layout of v_mfma_f32_32x32x16_bf16:
+// element i of lane l is row 8*(i/4) + 4*(l/32) + i%4, column l%32 (Fp is
+// already offset by 4*(l/32) rows and l%32 columns).
+__device__ __forceinline__ sf32x16 tile_load(const float *Fp, int np) {
   sf32x16 C;
 #pragma unroll
   for (int i = 0; i < 16; ++i) C[i] = Fp[(size_t)(8 * (i >> 2) + (i & 3)) * np];
-  if (bprev >= 0) {
-    const uint4 dd = dprev[row0 + 32 * mt + lc];  // (plus lo, plus hi, minus lo, minus hi)
-    const uint4 *qb = qpack + (((size_t)s * NB + bprev) * np + col) * 16;
+  return C;
+}
+
+__device__ __forceinline__ void tile_store(float *Fp, int np, const sf32x16 &C) {
 #pragma unroll
-    for (int ks = 0; ks < 4; ++ks) {
-      const uint32_t pw = ks < 2 ? dd.x : dd.y, mw = ks < 2 ? dd.z : dd.w;
-      const uint4 a = spins8(pw, mw, 16 * (ks & 1) + 8 * kh);
+  for (int i = 0; i < 16; ++i) Fp[(size_t)(8 * (i >> 2) + (i & 3)) * np] = C[i];
+}
+
+// Pass launch: Fout = Fin + the flips of the cnt blocks g0...  Persistent
+// grid over tiles of 256 replicas x 64 columns (column block fastest; with
+// the grid size and np/64 multiples of 8, every tile of one column block is
+// served by one XCD and its L2).  Wave w holds rows 64w.. of the tile as 2 x
+// 2 accumulators of 32x32; the work-group stages the Q rows of K blocks for
+// its 64 columns in LDS once and all four waves read them from there.
+template <int K>
+__global__ __launch_bounds__(256) void stream_pass_kernel(int np, int Rp, int I, int g0, int cnt,
+                                                          int ring, size_t T,
+                                                          const uint4 *__restrict__ qpack,
+                                                          const float *__restrict__ Fin,
+                                                          float *__restrict__ Fout,
+                                                          const uint4 *__restrict__ dring) {
+  __shared__ __align__(16) unsigned char bst[K][64 * kBStride];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int lc = lane & 31, kh = lane >> 5;
+  const int NB = np / kSB, RG = (Rp + kPRows - 1) / kPRows;
+  const int ntiles = NB * RG * I;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int cx = tile % NB, rg = (tile / NB) % RG, s = tile / (NB * RG);
+    const int r64 = rg * 4 + w;  // the wave's 64-replica group
+    const bool active = r64 * 64 < Rp;
+    const size_t row0 = (size_t)s * Rp + (size_t)r64 * 64;
+    sf32x16 acc[2][2];
+    if (active) {
 #pragma unroll
-      for (int p = 0; p < 2; ++p) C = smfma(a, qb[p * 8 + 2 * ks + kh], C);
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          acc[rt][ct] =
+              tile_load(Fin + (row0 + 32 * rt + 4 * kh) * np + cx * kSB + 32 * ct + lc, np);
     }
+    for (int c0 = 0; c0 < cnt; c0 += K) {
+      const int kc = min(K, cnt - c0);
+      __syncthreads();  // the previous chunk's operands are consumed
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Fp[(size_t)(8 * (i >> 2) + (i & 3)) * np] = C[i];
+      for (int t = 0; t < K; ++t) {
+        if (t < kc) {
+          const int g = g0 + c0 + t;
+          const uint4 *src = qpack + ((size_t)(s * NB + g % NB) * np + cx * kSB) * 16;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int idx = tid + 256 * i, col = idx >> 4, part = idx & 15;
+            *reinterpret_cast<uint4 *>(&bst[t][col * kBStride + part * 16]) = src[idx];
+          }
+        }
+      }
+      uint4 dd[K][2];
+      if (active) {
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+          if (t < kc) {
+            const int g = g0 + c0 + t;
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+              dd[t][rt] = dring[(size_t)(g % ring) * T + row0 + 32 * rt + lc];
+          }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+          if (t < kc) {
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+              uint4 a[2];
+#pragma unroll
+              for (int rt = 0; rt < 2; ++rt) {
+                const uint32_t pw = ks < 2 ? dd[t][rt].x : dd[t][rt].y;
+                const uint32_t mw = ks < 2 ? dd[t][rt].z : dd[t][rt].w;
+                a[rt] = spins8(pw, mw, 16 * (ks & 1) + 8 * kh);
+              }
+#pragma unroll
+              for (int ct = 0; ct < 2; ++ct) {
+                const unsigned char *bc = &bst[t][(32 * ct + lc) * kBStride];
+                const uint4 bh = *reinterpret_cast<const uint4 *>(bc + (2 * ks + kh) * 16);
+                const uint4 bl = *reinterpret_cast<const uint4 *>(bc + 128 + (2 * ks + kh) * 16);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                  acc[rt][ct] = smfma(a[rt], bh, acc[rt][ct]);
+                  acc[rt][ct] = smfma(a[rt], bl, acc[rt][ct]);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (active) {
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+          tile_store(Fout + (row0 + 32 * rt + 4 * kh) * np + cx * kSB + 32 * ct + lc, np,
+                     acc[rt][ct]);
+    }
   }
-  if (!decide) return;
-  // ---- decisions of block b for the 64 replicas of this group ----
+}
+
+// Value of lane (lane & ~3) | ow of each quad (ow compile-time).
+template <int ow>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_mov_dpp(v, ow * 0x55, 0xF, 0xF, false);
+}
+
+// Decide launch for block g (block b = g mod NB of sweep `sweep`): grid
+// (Rp/64, I), 256 threads.  The work-group reads its 64x64 tile of block b
+// from Fin (the fields after blocks .. gb-1) and adds the flips of blocks gb
+// .. g-1 in registers; everything it reads is requested up front, so the
+// decisions wait for one memory round trip per kChunk pending blocks.  The
+// 64 decisions of a replica are split over a quad of lanes: wave w takes
+// replicas 16w + lane/4, lane r4 = lane%4 owns spins j = r4 (mod 4) -- it
+// decides them and keeps their in-block corrections c[j] -- and each
+// decision reaches the quad by a DPP broadcast, after which every lane
+// corrects its own later spins with the fp32 diagonal sub-block of Q.  The
+// waves run at raised priority: the launch shares the CUs with the pass
+// launch and is on the critical path.
+__global__ __launch_bounds__(256) void stream_decide_kernel(
+    int n, int np, int Rp, int g, int gb, int ring, size_t T, int sweep, float ts, uint32_t k0,
+    uint32_t k1, uint32_t inst_base, int last_of_sweep, const uint4 *__restrict__ qpack,
+    const float *__restrict__ qdb, const float *__restrict__ Fin, uint32_t *__restrict__ xbits,
+    uint4 *__restrict__ dring, double *__restrict__ energy, double *__restrict__ ebest,
+    uint32_t *__restrict__ bestbits) {
+  __shared__ __align__(16) float fl[kSB][kST];        // fields of the block, [spin][replica]
+  __shared__ __align__(16) float qd4[kSB][4][kQ4];    // Q[j][4i + r4] at [j][r4][i]
+  __shared__ float thr[kSB][kSB + 1];                 // thresholds [replica][spin]
+  __shared__ uint32_t newx[kSB][2];                   // the block's new state words
+  __shared__ int improved[kSB];                       // replica's best energy improved
+  __builtin_amdgcn_s_setprio(3);
+  const int ry = blockIdx.x, s = blockIdx.y;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int lc = lane & 31, kh = lane >> 5, mt = w >> 1, nt = w & 1;
+  const int NB = np / kSB, npw = np / 32, b = g % NB, cnt = g - gb;
+  const size_t row0 = (size_t)s * Rp + (size_t)ry * 64, drow = row0 + 32 * mt + lc;
   const uint32_t tag = ((inst_base + (uint32_t)s) << 8) | TAG_SWEEP_THR;
+  // requests: diagonal sub-block, field tile, pending flips and Q rows, and
+  // the decided replica's state words and energies
+  const float4 *qsrc = reinterpret_cast<const float4 *>(qdb + ((size_t)s * NB + b) * kSB * kSB);
+  float4 qv[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) qv[i] = qsrc[tid + 256 * i];
+  const size_t off = (row0 + 32 * mt + 4 * kh) * np + b * kSB + 32 * nt + lc;
+  const uint4 *qcol = qpack + ((size_t)s * NB * np + b * kSB + 32 * nt + lc) * 16;
+  sf32x16 C = tile_load(Fin + off, np);
+  TileOps<kChunk> o;
+  ops_load(o, gb, cnt, np, NB, ring, T, drow, qcol, dring, kh);
+  const int rep = 16 * w + (lane >> 2), r4 = lane & 3;
+  const size_t rr = row0 + rep;
+  const uint2 xw0 = *reinterpret_cast<const uint2 *>(xbits + rr * npw + 2 * b);
+  const double e_old = energy[rr];
+  const double e_best = last_of_sweep ? ebest[rr] : 0.0;
+  // thresholds: 64 replicas x 16 Philox calls (4 spins each), 4 per thread
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int it = tid + 256 * i, r = it >> 4, q4 = it & 15;
+    const int j0 = b * kSB + 4 * q4;
+    const uint4 ph = philox4x32_10(make_uint4((uint32_t)(j0 >> 2), (uint32_t)(ry * 64 + r),
+                                              (uint32_t)sweep, tag), k0, k1);
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const float x = neg_ln_u(pick(ph, jj)) * ts;
+      thr[r][4 * q4 + jj] = j0 + jj < n ? x : -INFINITY;
+    }
+  }
+  ops_apply(C, o, cnt, kh);
+  for (int c0 = kChunk; c0 < cnt; c0 += kChunk) {
+    ops_load(o, gb + c0, cnt - c0, np, NB, ring, T, drow, qcol, dring, kh);
+    ops_apply(C, o, cnt - c0, kh);
+  }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
     *reinterpret_cast<float4 *>(&fl[32 * nt + lc][32 * mt + 8 * q + 4 * kh]) =
         make_float4(C[4 * q], C[4 * q + 1], C[4 * q + 2], C[4 * q + 3]);
-  const float *qsrc = qdb + ((size_t)s * NB + b) * kSB * kSB;
-  for (int k = tid; k < kSB * kSB; k += 256) qd[k >> 6][k & 63] = qsrc[k];
-  // thresholds: 64 replicas x 16 Philox calls (4 spins each), 4 per thread
-  for (int it = tid; it < 64 * 16; it += 256) {
-    const int r = it >> 4, q4 = it & 15;
-    const int j0 = b * kSB + 4 * q4;
-    const uint4 o = philox4x32_10(make_uint4((uint32_t)(j0 >> 2), (uint32_t)(ry * 64 + r),
-                                             (uint32_t)sweep, tag), k0, k1);
 #pragma unroll
-    for (int jj = 0; jj < 4; ++jj) {
-      const float x = neg_ln_u(pick(o, jj)) * ts;
-      thr[r][4 * q4 + jj] = j0 + jj < n ? x : -INFINITY;
-    }
+  for (int i = 0; i < 4; ++i) {
+    const int idx = tid + 256 * i, j = idx >> 4, c4 = idx & 15;  // Q[j][4*c4 .. 4*c4+3]
+    qd4[j][0][c4] = qv[i].x;
+    qd4[j][1][c4] = qv[i].y;
+    qd4[j][2][c4] = qv[i].z;
+    qd4[j][3][c4] = qv[i].w;
   }
   __syncthreads();
-  if (w != 0) return;
-  const size_t rr = row0 + lane;
-  uint32_t *xw = xbits + rr * npw + 2 * b;
-  const uint32_t w0 = xw[0], w1 = xw[1];
-  uint32_t x0 = w0, x1 = w1;
-  float c[kSB];
+  uint32_t x0 = xw0.x, x1 = xw0.y;
+  // this lane's spins 4i + r4: fields (corrected in place as spins are
+  // decided) and thresholds, in registers
+  float f[kSB / 4], th[kSB / 4];
 #pragma unroll
-  for (int j = 0; j < kSB; ++j) c[j] = 0.0f;
+  for (int i = 0; i < kSB / 4; ++i) {
+    f[i] = fl[4 * i + r4][rep];
+    th[i] = thr[rep][4 * i + r4];
+  }
   float desum = 0.0f;
+  // row j of the diagonal sub-block, this lane's columns 4i + r4 (i >= q0*4
+  // only), loaded one step ahead
+  float4 qc[4], qn[4];
 #pragma unroll
+  for (int q = 0; q < 4; ++q) qc[q] = reinterpret_cast<const float4 *>(qd4[0][r4])[q];
+#pragma clang loop unroll(full)
   for (int j = 0; j < kSB; ++j) {
-    const float h = fl[j][lane] + c[j];
+    if (j + 1 < kSB) {
+#pragma unroll
+      for (int q = ((j + 1) >> 2) >> 2; q < 4; ++q)
+        qn[q] = reinterpret_cast<const float4 *>(qd4[j + 1][r4])[q];
+    }
     uint32_t &xwd = j < 32 ? x0 : x1;
     const int bit = j & 31;
     const bool xj = (xwd >> bit) & 1u;
+    // owner lane (r4 == j % 4): field and decision, broadcast to the quad
+    const float h = f[j >> 2];
     const float de = xj ? -h : h;
-    const bool acc = de < thr[lane][j];
+    const int acc_own = de < th[j >> 2] ? 1 : 0;
+    int acc, de_bits;
+    switch (j & 3) {
+      case 0: acc = quad_bcast<0>(acc_own); de_bits = quad_bcast<0>(__float_as_int(de)); break;
+      case 1: acc = quad_bcast<1>(acc_own); de_bits = quad_bcast<1>(__float_as_int(de)); break;
+      case 2: acc = quad_bcast<2>(acc_own); de_bits = quad_bcast<2>(__float_as_int(de)); break;
+      default: acc = quad_bcast<3>(acc_own); de_bits = quad_bcast<3>(__float_as_int(de)); break;
+    }
     const float d = acc ? (xj ? -1.0f : 1.0f) : 0.0f;
     xwd ^= acc ? (1u << bit) : 0u;
-    desum += acc ? de : 0.0f;
-    const float4 *qrow = reinterpret_cast<const float4 *>(qd[j]);
+    // every lane of the quad sums the accepted dE in spin order
+    desum += acc ? __int_as_float(de_bits) : 0.0f;
+    // this lane's later spins 4i + r4 (i >= j/4; the decided ones among them
+    // are not read again, and Q's diagonal is zero)
 #pragma unroll
-    for (int q = (j + 1) / 4; q < kSB / 4; ++q) {
-      const float4 v = qrow[q];
-      if (4 * q > j) c[4 * q] = fmaf(v.x, d, c[4 * q]);
-      if (4 * q + 1 > j) c[4 * q + 1] = fmaf(v.y, d, c[4 * q + 1]);
-      if (4 * q + 2 > j) c[4 * q + 2] = fmaf(v.z, d, c[4 * q + 2]);
-      if (4 * q + 3 > j) c[4 * q + 3] = fmaf(v.w, d, c[4 * q + 3]);
+    for (int q = (j >> 2) >> 2; q < 4; ++q) {
+      if (4 * q >= (j >> 2)) f[4 * q] = fmaf(qc[q].x, d, f[4 * q]);
+      if (4 * q + 1 >= (j >> 2)) f[4 * q + 1] = fmaf(qc[q].y, d, f[4 * q + 1]);
+      if (4 * q + 2 >= (j >> 2)) f[4 * q + 2] = fmaf(qc[q].z, d, f[4 * q + 2]);
+      if (4 * q + 3 >= (j >> 2)) f[4 * q + 3] = fmaf(qc[q].w, d, f[4 * q + 3]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) qc[q] = qn[q];
+  }
+  if (r4 == 0) {
+    const uint32_t w0 = xw0.x, w1 = xw0.y;
+    *reinterpret_cast<uint2 *>(xbits + rr * npw + 2 * b) = make_uint2(x0, x1);
+    const uint32_t f0 = x0 ^ w0, f1 = x1 ^ w1;
+    dring[(size_t)(g % ring) * T + rr] = make_uint4(f0 & ~w0, f1 & ~w1, f0 & w0, f1 & w1);
+    const double e_now = e_old + (double)desum;
+    energy[rr] = e_now;
+    if (last_of_sweep) {
+      const bool better = e_now < e_best;
+      if (better) ebest[rr] = e_now;
+      improved[rep] = better ? 1 : 0;
+      newx[rep][0] = x0;
+      newx[rep][1] = x1;
     }
   }
-  xw[0] = x0;
-  xw[1] = x1;
-  const uint32_t f0 = x0 ^ w0, f1 = x1 ^ w1;
-  dcur[rr] = make_uint4(f0 & ~w0, f1 & ~w1, f0 & w0, f1 & w1);
-  const double e_now = energy[rr] + (double)desum;
-  energy[rr] = e_now;
-  if (last_of_sweep && e_now < ebest[rr]) {
-    ebest[rr] = e_now;
-    const uint32_t *src = xbits + rr * npw;
-    uint32_t *dst = bestbits + rr * npw;
-    for (int q = 0; q < npw; ++q) dst[q] = src[q];
+  if (!last_of_sweep) return;
+  // end of a sweep: the improved replicas' states become their best states,
+  // copied by the whole work-group (the block's words from LDS)
+  __syncthreads();
+  for (int k = tid; k < kSB * npw; k += 256) {
+    const int r = k / npw, q = k - r * npw;
+    if (!improved[r]) continue;
+    const size_t row = row0 + r;
+    uint32_t v;
+    if (q == 2 * b) v = newx[r][0];
+    else if (q == 2 * b + 1) v = newx[r][1];
+    else v = xbits[row * npw + q];
+    bestbits[row * npw + q] = v;
   }
 }
 
@@ -187,16 +427,32 @@ void launch_pack_q64(hipStream_t st, int I, int n, int np, const double *q, uint
   OS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_stream_block(hipStream_t st, int I, int n, int np, int Rp, int b, int bprev,
-                         int sweep, float ts, uint64_t seed, uint32_t inst_base,
-                         bool last_of_sweep, const uint4 *qpack, const float *qdb, float *F,
-                         uint32_t *xbits, const uint4 *dprev, uint4 *dcur, double *energy,
-                         double *ebest, uint32_t *bestbits) {
-  const dim3 grid(np / kSB, Rp / 64, I);
-  stream_block_kernel<<<grid, 256, 0, st>>>(n, np, Rp, b, bprev, sweep, ts, (uint32_t)seed,
-                                            (uint32_t)(seed >> 32), inst_base,
-                                            last_of_sweep ? 1 : 0, qpack, qdb, F, xbits, dprev,
-                                            dcur, energy, ebest, bestbits);
+void launch_stream_pass(hipStream_t st, int I, int np, int Rp, int g0, int cnt, int ring,
+                        int chunk, int grid, const uint4 *qpack, const float *Fin, float *Fout,
+                        const uint4 *dring) {
+  const size_t T = (size_t)I * Rp;
+  const int ntiles = (np / kSB) * ((Rp + kPRows - 1) / kPRows) * I;
+  grid = std::max(1, std::min(grid, ntiles));
+  if (chunk >= 4)
+    stream_pass_kernel<4><<<grid, 256, 0, st>>>(np, Rp, I, g0, cnt, ring, T, qpack, Fin, Fout,
+                                                dring);
+  else if (chunk >= 2)
+    stream_pass_kernel<2><<<grid, 256, 0, st>>>(np, Rp, I, g0, cnt, ring, T, qpack, Fin, Fout,
+                                                dring);
+  else
+    stream_pass_kernel<1><<<grid, 256, 0, st>>>(np, Rp, I, g0, cnt, ring, T, qpack, Fin, Fout,
+                                                dring);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, int gb, int ring,
+                          int sweep, float ts, uint64_t seed, uint32_t inst_base,
+                          bool last_of_sweep, const uint4 *qpack, const float *qdb,
+                          const float *Fin, uint32_t *xbits, uint4 *dring, double *energy,
+                          double *ebest, uint32_t *bestbits) {
+  stream_decide_kernel<<<dim3(Rp / 64, I), 256, 0, st>>>(
+      n, np, Rp, g, gb, ring, (size_t)I * Rp, sweep, ts, (uint32_t)seed, (uint32_t)(seed >> 32),
+      inst_base, last_of_sweep ? 1 : 0, qpack, qdb, Fin, xbits, dring, energy, ebest, bestbits);
   OS_HIP_CHECK(hipGetLastError());
 }
 

@@ -391,6 +391,8 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
       stats->kernel_launches = 1;
       stats->max_field_drift = 0.0;
       stats->attempts = (uint64_t)b.total_spins * (uint64_t)R * (uint64_t)S;
+      stats->pass_avg_ms = 0.0;
+      stats->pass_launches = 0;
     }
   });
 }

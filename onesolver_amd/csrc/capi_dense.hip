@@ -14,6 +14,7 @@
 // are recomputed in FP64 and the largest fp32 field error is recorded.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -155,7 +156,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     const size_t nn = (size_t)n * n;
     const int S = p->num_sweeps;
     const int seg = p->refresh_interval > 0 ? p->refresh_interval : std::max(S, 1);
-    d.fields.ensure(sizeof(float) * T * np);
+    d.fields.ensure(sizeof(float) * T * np * (d.stream ? 2 : 1));
     d.xbits.ensure(sizeof(uint32_t) * T * npw);
     d.bestbits.ensure(sizeof(uint32_t) * T * npw);
     d.energy.ensure(sizeof(double) * T);
@@ -219,7 +220,43 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
                                 hipMemcpyDeviceToDevice, st));
     OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), st));
     int launches = 0;
-    if (d.stream) d.dblk.ensure(sizeof(uint4) * 2 * T);
+    // stream path: the up-to-date field copy (0 or 1)
+    int cur = 0;
+    auto fbuf = [&](int i) { return d.fields.as<float>() + (size_t)(i & 1) * T * np; };
+    // blocks per field pass: OS_STREAM_GROUP overrides the default (tuning)
+    const char *genv = std::getenv("OS_STREAM_GROUP");
+    d.group = genv ? std::min(16, std::max(1, std::atoi(genv))) : os::Dense::kGroup;
+    const int m = d.group, ring = 3 * d.group;
+    // pass launches: blocks whose operands load in one round trip
+    // (OS_STREAM_PASS_CHUNK overrides the default, for tuning)
+    const char *cenv = std::getenv("OS_STREAM_PASS_CHUNK");
+    const int chunk = cenv ? std::atoi(cenv) : os::Dense::kPassChunk;
+    // pass launches: persistent work-groups per CU (OS_STREAM_PASS_WGS_PER_CU
+    // overrides the default); the grid leaves room for the decide launches
+    const char *wenv = std::getenv("OS_STREAM_PASS_WGS_PER_CU");
+    const int pgrid = c.num_cus * (wenv ? std::max(1, std::atoi(wenv)) : os::Dense::kPassWgsPerCu);
+    // stream path, with stats: HIP events around every pass_stride-th pass
+    // (at most kPassSamples of them)
+    constexpr int kPassSamples = 256;
+    int passes = 0, timed = 0, pass_stride = 1;
+    if (d.stream) {
+      const long long nb = np / os::stream_block(), total = ((long long)S * nb + m - 1) / m;
+      pass_stride = (int)std::max(1LL, (total + kPassSamples - 1) / kPassSamples);
+      if (stats)
+        while (d.pass_ev.size() < 2 * (size_t)kPassSamples) {
+          hipEvent_t e;
+          OS_HIP_CHECK(hipEventCreate(&e));
+          d.pass_ev.push_back(e);
+        }
+    }
+    if (d.stream) {
+      d.dblk.ensure(sizeof(uint4) * ring * T);
+      if (!d.side) {
+        OS_HIP_CHECK(hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
+        for (hipEvent_t *e : {&d.ev_seg, &d.ev_dec, &d.ev_pass[0], &d.ev_pass[1]})
+          OS_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      }
+    }
     for (int s0 = 0; s0 < S; s0 += seg) {
       const int ns = std::min(seg, S - s0);
       hipEvent_t a, b;
@@ -229,23 +266,45 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       ev.push_back(b);
       OS_HIP_CHECK(hipEventRecord(a, st));
       if (d.stream) {
-        // one pass over the fields per block of 64 spins, then one more that
-        // applies the last block's flips
+        // Block g (0 .. steps-1) of group H = g / m is decided on d.side from
+        // the fields after pass H-2 plus the flips of blocks m(H-1) .. g-1;
+        // pass H (context's stream) adds the flips of group H to the fields
+        // after pass H-1, in the other copy, once the group is decided.  So
+        // the passes run beside the decisions of the next group.  Pass H
+        // lands in copy cur + H + 1; the flip masks of block g stay in the
+        // ring slot g % 3m until every reader (the decides of groups H and
+        // H+1, pass H) is done: block g + 3m is decided after pass H+1.
         const int nb = np / os::stream_block();
-        uint4 *db = d.dblk.as<uint4>();
+        uint4 *ring_d = d.dblk.as<uint4>();
         const int steps = ns * nb;
-        for (int g = 0; g <= steps; ++g) {
-          const int blk = g < steps ? g % nb : -1;
-          const int bprev = g > 0 ? (g - 1) % nb : -1;
-          const int sw = s0 + (g < steps ? g / nb : 0);
-          os::launch_stream_block(st, I, n, np, Rp, blk, bprev, sw, g < steps ? p->tscale[sw] : 0.f,
-                                  p->seed, p->instance, blk == nb - 1, d.qpack.as<uint4>(),
-                                  d.qdb.as<float>(), d.fields.as<float>(),
-                                  d.xbits.as<uint32_t>(), db + (size_t)((g + 1) & 1) * T,
-                                  db + (size_t)(g & 1) * T, d.energy.as<double>(),
-                                  d.ebest.as<double>(), d.bestbits.as<uint32_t>());
+        OS_HIP_CHECK(hipEventRecord(d.ev_seg, st));
+        OS_HIP_CHECK(hipStreamWaitEvent(d.side, d.ev_seg, 0));
+        for (int g = 0; g < steps; ++g) {
+          const int H = g / m, sw = s0 + g / nb, blk = g % nb;
+          if (g % m == 0 && H >= 2) OS_HIP_CHECK(hipStreamWaitEvent(d.side, d.ev_pass[H & 1], 0));
+          const int gb = std::max(0, m * (H - 1));
+          os::launch_stream_decide(d.side, I, n, np, Rp, g, gb, ring, sw, p->tscale[sw], p->seed,
+                                   p->instance, blk == nb - 1, d.qpack.as<uint4>(),
+                                   d.qdb.as<float>(), fbuf(cur + std::max(0, H - 1)),
+                                   d.xbits.as<uint32_t>(), ring_d, d.energy.as<double>(),
+                                   d.ebest.as<double>(), d.bestbits.as<uint32_t>());
           ++launches;
+          if (g % m == m - 1 || g == steps - 1) {
+            OS_HIP_CHECK(hipEventRecord(d.ev_dec, d.side));
+            OS_HIP_CHECK(hipStreamWaitEvent(st, d.ev_dec, 0));
+            const bool timing = stats && passes % pass_stride == 0 && timed < kPassSamples;
+            if (timing) OS_HIP_CHECK(hipEventRecord(d.pass_ev[2 * timed], st));
+            os::launch_stream_pass(st, I, np, Rp, m * H, g - m * H + 1, ring, chunk, pgrid,
+                                   d.qpack.as<uint4>(), fbuf(cur + H), fbuf(cur + H + 1),
+                                   ring_d);
+            if (timing) OS_HIP_CHECK(hipEventRecord(d.pass_ev[2 * timed++ + 1], st));
+            ++passes;
+            OS_HIP_CHECK(hipEventRecord(d.ev_pass[H & 1], st));
+          }
         }
+        // the last pass waited for the last decide: the context's stream
+        // holds every launch of the segment again
+        if (steps > 0) cur = (cur + (steps - 1) / m + 1) & 1;
       } else {
         os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
                                 d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
@@ -256,7 +315,7 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       }
       OS_HIP_CHECK(hipEventRecord(b, st));
       // FP64 refresh of fields and running energies; records the fp32 drift
-      energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), d.fields.as<float>());
+      energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), fbuf(cur));
     }
     // FP64 energies of the best states (the returned energies)
     energies(d.bestbits.as<uint32_t>(), d.ebest.as<double>(), nullptr);
@@ -315,6 +374,14 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       stats->kernel_launches = launches;
       stats->max_field_drift = drift;
       stats->attempts = (uint64_t)I * (uint64_t)R * (uint64_t)S * (uint64_t)n;
+      double pms = 0.0;
+      for (int k = 0; k < timed; ++k) {
+        float ms = 0.0f;
+        OS_HIP_CHECK(hipEventElapsedTime(&ms, d.pass_ev[2 * k], d.pass_ev[2 * k + 1]));
+        pms += ms;
+      }
+      stats->pass_avg_ms = timed > 0 ? pms / timed : 0.0;
+      stats->pass_launches = passes;
     }
   });
 }

@@ -1,6 +1,8 @@
 // os_dense.h -- the dense batch behind the C ABI's os_dense handle.
 #pragma once
 
+#include <vector>
+
 #include "os_ctx.h"
 
 namespace os {
@@ -13,15 +15,30 @@ struct Dense {
   int I = 1;  // instances (same n), stacked in every array
   int n = 0, np = 0, npw = 0;
   // n > dense_max_n(): fields in HBM, blocks of 64 spins (anneal_stream.hip);
-  // qpack is then [I][n/64][np][128] and qdb [I][n/64][64][64]
+  // qpack is then [I][n/64][np][128], qdb [I][n/64][64][64], and `fields`
+  // holds two copies of the field array (passes read one and write the other)
   bool stream = false;
   DevBuf q64, qpack, qdb;
   DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
-  DevBuf dblk;  // stream path: flip masks of the last two blocks
+  // stream path: flip masks of the last 3 * group blocks (ring), and the
+  // decide launches on `side`, beside the field passes on the context's
+  // stream; ev_dec marks the last decide of a group, ev_pass[G % 2] pass G
+  DevBuf dblk;
+  static constexpr int kGroup = 4;      // default blocks per field pass
+  static constexpr int kPassChunk = 2;  // default operand blocks per round trip (pass)
+  static constexpr int kPassWgsPerCu = 1;  // default persistent pass work-groups per CU (room for decides)
+  int group = kGroup;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_seg = nullptr, ev_dec = nullptr, ev_pass[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> pass_ev;  // timing pairs around sampled passes
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
                     &scratch, &drift, &bytes, &marg, &status, &dblk})
       b->release();
+    for (hipEvent_t e : {ev_seg, ev_dec, ev_pass[0], ev_pass[1]})
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : pass_ev) (void)hipEventDestroy(e);
+    if (side) (void)hipStreamDestroy(side);
   }
 };
 

@@ -48,13 +48,22 @@ void launch_warm_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint
 int stream_block();
 void launch_pack_q64(hipStream_t st, int I, int n, int np, const double *q, uint16_t *qpack,
                      float *qdb);
-// one pass: apply D (dprev) of block bprev (if >= 0) to all fields, then decide
-// block b (if >= 0) of sweep `sweep` into dcur
-void launch_stream_block(hipStream_t st, int I, int n, int np, int Rp, int b, int bprev,
-                         int sweep, float ts, uint64_t seed, uint32_t inst_base,
-                         bool last_of_sweep, const uint4 *qpack, const float *qdb, float *F,
-                         uint32_t *xbits, const uint4 *dprev, uint4 *dcur, double *energy,
-                         double *ebest, uint32_t *bestbits);
+// Blocks g of a segment are numbered from 0 (block g covers spins 64*(g mod
+// NB).. of sweep g / NB); the flip masks of block g live in dring[g % ring]
+// (dring: ring x I*Rp uint4).
+// pass launch: Fout = Fin + the flips of blocks g0 .. g0+cnt-1, the operands
+// of `chunk` (1, 2 or 4) blocks staged per round trip, `grid` persistent
+// work-groups
+void launch_stream_pass(hipStream_t st, int I, int np, int Rp, int g0, int cnt, int ring,
+                        int chunk, int grid, const uint4 *qpack, const float *Fin, float *Fout,
+                        const uint4 *dring);
+// decide launch: block g, from the fields Fin (after blocks .. gb-1) plus the
+// flips of blocks gb .. g-1
+void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, int gb, int ring,
+                          int sweep, float ts, uint64_t seed, uint32_t inst_base,
+                          bool last_of_sweep, const uint4 *qpack, const float *qdb,
+                          const float *Fin, uint32_t *xbits, uint4 *dring, double *energy,
+                          double *ebest, uint32_t *bestbits);
 
 // gen.hip
 // q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed

@@ -111,6 +111,30 @@ def test_dense_streaming_path_matches_oracle_bit_for_bit(ctx, n, R, S, refresh):
     assert out["stats"]["max_field_drift"] == 0.0
 
 
+@pytest.mark.parametrize("group,refresh", [(1, 0), (3, 1), (7, 0)])
+def test_dense_streaming_pass_grouping(ctx, monkeypatch, group, refresh):
+    """Any number of blocks per field pass (OS_STREAM_GROUP), with partial
+    last groups and segment ends, gives the oracle's chains, for a batch of
+    two instances."""
+    n, R, S = 1100, 64, 3
+    qs = np.stack([_int_instance(n, 5), _int_instance(n, 6)])
+    ts = _tscale(S)
+    monkeypatch.setenv("OS_STREAM_GROUP", str(group))
+    print(f"[gpu] dense streaming n={n} group={group} refresh={refresh}", flush=True)
+    b = onesolver_amd.DenseBatch(ctx, qs)
+    out = b.anneal(R, ts, seed=3, instance=4, refresh_interval=refresh, final_states=True)
+    b.close()
+    nb = -(-n // 64)
+    segs = 1 if refresh == 0 else -(-S // refresh)
+    per_seg = S * nb if refresh == 0 else refresh * nb
+    assert out["stats"]["pass_launches"] == segs * -(-per_seg // group)
+    for i in range(2):
+        fin, bs, be = oracle.sweep_dense(qs[i], n, ts, R, seed=3, instance=4 + i)
+        assert (out["final_states"][i] == fin).all()
+        assert (out["best_states"][i] == bs).all()
+        np.testing.assert_allclose(out["best_energies"][i], oracle.energy(qs[i], bs), rtol=1e-12)
+
+
 def test_dense_streaming_sk_energies_and_drift(ctx):
     n, R, S = 2048, 64, 6
     b = onesolver_amd.DenseBatch(ctx, sk=(1, n, 4, 0))
