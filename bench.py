@@ -23,6 +23,14 @@ sys.path.insert(0, ROOT)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) on the default configurations,
+# calibrated on a 1 GiB device copy (read bytes = 2 x FETCH_SIZE, written
+# bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
+TRAFFIC_DENSE_512 = (2 * 2962442.1 + 245838.8) * 1024   # dense_sweep_kernel, 512 sweeps
+TRAFFIC_CSR_C3 = (2 * 20552008.9 + 7267549.8) * 1024    # csr_sweep_kernel, 2048 sweeps
+TRAFFIC_PASS_C5 = (2 * 98064.8 + 131072.0) * 1024       # stream_pass_kernel, 4 blocks
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -114,7 +122,9 @@ def run_dense(args):
                    "n": n, "replicas": R, "sweeps": S, "block_k": 16,
                    "refresh_interval": args.refresh, "parallelism": "replicas x columns on 1 GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS,
+                     "traffic": TRAFFIC_DENSE_512 if (n, R, args.refresh) == (1024, 16384, 512)
+                     else None,
                      "kernel": f"dense_sweep_kernel<{-(-n // 128)}>",
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_flops_per_launch": flops_per_launch},
@@ -183,7 +193,8 @@ def run_dense_c5(args):
                                f"256 sweeps", "n": n, "replicas": R, "sweeps": S, "block_k": 64},
         "roofline": {"bound": "hbm", "achieved": field_bytes / pass_s / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": field_bytes / pass_s / 1e9 / PEAK_HBM_GBS, "traffic": None,
+                     "frac": field_bytes / pass_s / 1e9 / PEAK_HBM_GBS,
+                     "traffic": TRAFFIC_PASS_C5 if (R, group) == (4096, 4.0) else None,
                      "kernel": "stream_pass_kernel", "avg_launch_ms": pass_s * 1e3,
                      "algorithmic_bytes_per_launch": field_bytes,
                      "blocks_per_pass": group, "mfma_tflops": flops / pass_s / 1e12},
@@ -290,7 +301,8 @@ def run_csr_c3(args):
                    "replicas": R, "sweeps": S, "sum_n": int(ns.sum()),
                    "launch_shape": list(b.launch_shape())},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": TRAFFIC_CSR_C3 if S == 2048 else None,
                      "kernel": "csr_sweep_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_alg},
     }
