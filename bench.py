@@ -274,12 +274,12 @@ def run_csr_c3(args):
     log(f"device {ctx.device_name()}")
     b = onesolver_amd.CsrBatch(ctx, insts)
     for i in range(args.warmup):
-        out = b.anneal(R, ts, seed=1234, best_states=False)
+        out = b.anneal(R, ts, seed=1234, best_states=False, tune=not args.c3_no_tune)
         log(f"warmup {i}: graph {out['stats']['kernel_ms']:.1f} ms shape {b.launch_shape()}")
     kernel_ms = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        out = b.anneal(R, ts, seed=1234 + i, best_states=False)
+        out = b.anneal(R, ts, seed=1234 + i, best_states=False, tune=not args.c3_no_tune)
         kernel_ms += out["stats"]["kernel_ms"]
         log(f"step {i}: graph {out['stats']['kernel_ms']:.1f} ms")
     elapsed = time.perf_counter() - t0
@@ -299,7 +299,8 @@ def run_csr_c3(args):
                                f"d-regular d in 3..8, weights +-1, {R} replicas, {S} sweeps, "
                                f"persistent kernel + hipGraph", "instances": 512,
                    "replicas": R, "sweeps": S, "sum_n": int(ns.sum()),
-                   "launch_shape": list(b.launch_shape())},
+                   "launch_shape": list(b.launch_shape()[:2]),
+                   "row_prefetch": bool(b.launch_shape()[2])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": TRAFFIC_CSR_C3 if S == 2048 else None,
@@ -392,6 +393,8 @@ def main():
     ap.add_argument("--c4-epochs", type=int, default=2)
     ap.add_argument("--c4-eval", type=int, default=32)
     ap.add_argument("--sweeps-c3", type=int, default=2048)
+    ap.add_argument("--c3-no-tune", action="store_true",
+                    help="skip the launch-shape tuner (library default shape)")
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--replicas", type=int, default=16384)
     ap.add_argument("--sweeps", type=int, default=4096)

@@ -211,8 +211,11 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
                         double *best_energies, double *run_best_energies, uint8_t *best_states,
                         uint8_t *final_states, double *inst_best_energy,
                         int32_t *inst_best_replica, os_sweep_stats *stats);
-/* The launch shape in use (0 before the first run). */
-int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups);
+/* The launch shape in use (0 before the first run); prefetch: 1 if the
+ * kernel variant that requests the next site's row ahead runs (batches whose
+ * rows have at most 8 couplings), 0 for the general one. */
+int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
+                              int32_t *prefetch);
 
 #ifdef __cplusplus
 }

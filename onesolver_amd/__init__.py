@@ -309,7 +309,7 @@ def _csr_lib():
         L.os_csr_batch_destroy.argtypes = [_vp]
         L.os_csr_batch_anneal.argtypes = [_vp, ctypes.POINTER(CsrParams), _f64p, _f64p, _u8p,
                                           _u8p, _f64p, _i32p, ctypes.POINTER(SweepStats)]
-        L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p]
+        L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
         L._csr_ready = True
     return L
 
@@ -346,9 +346,12 @@ class CsrBatch:
             pass
 
     def launch_shape(self):
-        w, g = ctypes.c_int32(0), ctypes.c_int32(0)
-        _check(_csr_lib().os_csr_batch_launch_shape(self._h, ctypes.byref(w), ctypes.byref(g)))
-        return w.value, g.value
+        """(waves per work-group, resident work-groups, 1 if the
+        row-prefetching kernel runs else 0); zeros before the first run."""
+        w, g, r = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(_csr_lib().os_csr_batch_launch_shape(self._h, ctypes.byref(w), ctypes.byref(g),
+                                                    ctypes.byref(r)))
+        return w.value, g.value, r.value
 
     def anneal(self, num_replicas, tscale, seed=1, instance_base=0, init_states=None,
                best_states=True, final_states=False, use_graph=True, tune=True):

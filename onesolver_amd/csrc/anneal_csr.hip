@@ -143,6 +143,147 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
   }
 }
 
+// One ELL row of at most 8 entries (chunks 1 or 2; the second chunk of a
+// one-chunk row is padding) and the site's diagonal, uniform across a wave.
+struct SiteRow {
+  int4 c0, c1;
+  float4 v0, v1;
+  float dj;
+
+  // field in the oracle's order: diag, then + w for each set neighbour
+  __device__ __forceinline__ float field(const uint32_t *xs, int lane, int chunks) const {
+    float h = dj;
+    h = plane_bit(xs, c0.x, lane) ? h + v0.x : h;
+    h = plane_bit(xs, c0.y, lane) ? h + v0.y : h;
+    h = plane_bit(xs, c0.z, lane) ? h + v0.z : h;
+    h = plane_bit(xs, c0.w, lane) ? h + v0.w : h;
+    if (chunks > 1) {
+      h = plane_bit(xs, c1.x, lane) ? h + v1.x : h;
+      h = plane_bit(xs, c1.y, lane) ? h + v1.y : h;
+      h = plane_bit(xs, c1.z, lane) ? h + v1.z : h;
+      h = plane_bit(xs, c1.w, lane) ? h + v1.w : h;
+    }
+    return h;
+  }
+};
+
+__device__ __forceinline__ SiteRow load_row(const int4 *ecol, const float4 *eval, const float *dg,
+                                            int j, int chunks, int n) {
+  SiteRow rw;
+  rw.c0 = ecol[(size_t)j * chunks];
+  rw.v0 = eval[(size_t)j * chunks];
+  rw.c1 = make_int4(n, n, n, n);
+  rw.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (chunks > 1) {
+    rw.c1 = ecol[(size_t)j * chunks + 1];
+    rw.v1 = eval[(size_t)j * chunks + 1];
+  }
+  rw.dj = dg[j];
+  return rw;
+}
+
+// Prefetching variant for batches whose ELL rows have at most 8 entries
+// (1 or 2 chunks): the same kernel, but the row, weights and diagonal of site
+// j+1 (uniform, scalar loads) are requested before site j is decided, so the
+// per-site chain no longer waits for a row load.  Same arithmetic, order and
+// results as csr_sweep_kernel.
+template <int kW>
+__global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
+    const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
+    const float4 *__restrict__ ell_val, const float *__restrict__ diag,
+    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
+    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
+    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
+    int max_words) {
+  extern __shared__ uint32_t lds[];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  uint32_t *xs = lds + (size_t)w * max_words * 64;
+  const int num_items = run->num_items, S = run->num_sweeps, Rp = run->Rp;
+  const uint32_t k0 = run->k0, k1 = run->k1;
+  const uint32_t base = run->instance_base;
+  const int init_mode = run->init_mode;
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= num_items) break;
+    const int2 wi = items[it];
+    const CsrInst d = inst[wi.x];
+    const int n = d.n, words = d.words, chunks = d.width >> 2;
+    const uint32_t r = (uint32_t)(wi.y * 64 + lane);
+    const uint32_t id = (base + (uint32_t)wi.x) << 8;
+    const int4 *ecol = ell_col + (d.ell_off >> 2);
+    const float4 *eval = ell_val + (d.ell_off >> 2);
+    const float *dg = diag + d.spin_off;
+    uint32_t *bb = best_bits + d.state_off + r;
+    for (int wd = 0; wd < words; ++wd) {
+      uint32_t v;
+      if (init_mode) {
+        v = init_bits[d.state_off + (size_t)wd * Rp + r];
+      } else {
+        const uint4 o = philox4x32_10(make_uint4((uint32_t)(wd >> 2), r, 0u, id | TAG_SWEEP_INIT),
+                                      k0, k1);
+        v = pick(o, (uint32_t)wd & 3u);
+      }
+      const int valid = n - 32 * wd;
+      if (valid < 32) v = valid <= 0 ? 0u : (v & ((1u << valid) - 1u));
+      xs[wd * 64 + lane] = v;
+    }
+    // FP64 energy in the oracle's order: sum over set j of diag_j + off_j / 2
+    double e = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double off = 0.0;
+      for (int c = 0; c < chunks; ++c) {
+        const int4 cc = ecol[(size_t)j * chunks + c];
+        const float4 vv = eval[(size_t)j * chunks + c];
+        off = plane_bit(xs, cc.x, lane) ? off + (double)vv.x : off;
+        off = plane_bit(xs, cc.y, lane) ? off + (double)vv.y : off;
+        off = plane_bit(xs, cc.z, lane) ? off + (double)vv.z : off;
+        off = plane_bit(xs, cc.w, lane) ? off + (double)vv.w : off;
+      }
+      if (plane_bit(xs, j, lane)) e += (double)dg[j] + 0.5 * off;
+    }
+    double eb = e;
+    for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+    for (int t = 0; t < S; ++t) {
+      const float ts = tscale[t];
+      SiteRow row = load_row(ecol, eval, dg, 0, chunks, n);
+      for (int j4 = 0; j4 < n; j4 += 4) {
+        const uint4 o = philox4x32_10(
+            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int j = j4 + q;
+          if (j >= n) break;
+          // the next site's row, requested before this site is decided
+          const SiteRow nrow = load_row(ecol, eval, dg, min(j + 1, n - 1), chunks, n);
+          const float thr = neg_ln_u(pick(o, (uint32_t)q)) * ts;
+          const float h = row.field(xs, lane, chunks);
+          uint32_t *xw = &xs[(j >> 5) * 64 + lane];
+          const uint32_t word = *xw;
+          const bool xj = (word >> (j & 31)) & 1u;
+          const float de = xj ? -h : h;
+          if (de < thr) {
+            *xw = word ^ (1u << (j & 31));
+            e += (double)de;
+          }
+          row = nrow;
+        }
+      }
+      if (e < eb) {
+        eb = e;
+        for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+      }
+    }
+    uint32_t *fb = final_bits + d.state_off + r;
+    for (int wd = 0; wd < words; ++wd) fb[(size_t)wd * Rp] = xs[wd * 64 + lane];
+    e_best[d.rep_off + r] = eb;
+    e_final[d.rep_off + r] = e;
+  }
+}
+
 // FP64 energy of states given as planes: E = sum_j x_j (Q_jj + sum_{l>j} Q_jl x_l),
 // the reference's energy function (upper triangle incl. diagonal), with the
 // double-precision couplings of the CSR input.  Thread = (item, lane).
@@ -182,15 +323,28 @@ void launch_csr_queue_reset(hipStream_t st, int *queue) {
   OS_HIP_CHECK(hipGetLastError());
 }
 
-void launch_csr_sweeps(hipStream_t st, int waves, int groups, int max_words, const CsrInst *inst,
-                       const int2 *items, const CsrRun *run, const int4 *ell_col,
-                       const float4 *ell_val, const float *diag, const float *tscale,
-                       const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
-                       double *e_best, double *e_final, int *queue) {
+bool csr_pf_eligible(int max_width) { return max_width <= 8; }
+
+void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_words,
+                       const CsrInst *inst, const int2 *items, const CsrRun *run,
+                       const int4 *ell_col, const float4 *ell_val, const float *diag,
+                       const float *tscale, const uint32_t *init_bits, uint32_t *best_bits,
+                       uint32_t *final_bits, double *e_best, double *e_final, int *queue) {
   const size_t lds = (size_t)csr_lds_bytes(waves, max_words);
   switch (waves) {
 #define OS_CSR_CASE(W)                                                                       \
   case W:                                                                                    \
+    if (pf) {                                                                                \
+      if (lds > 65536)                                                                       \
+        OS_HIP_CHECK(hipFuncSetAttribute((const void *)csr_sweep_pf_kernel<W>,               \
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,         \
+                                         (int)lds));                                         \
+      csr_sweep_pf_kernel<W><<<groups, 64 * W, lds, st>>>(inst, items, run, ell_col, ell_val, \
+                                                          diag, tscale, init_bits, best_bits, \
+                                                          final_bits, e_best, e_final, queue, \
+                                                          max_words);                         \
+      break;                                                                                 \
+    }                                                                                        \
     if (lds > 65536)                                                                         \
       OS_HIP_CHECK(hipFuncSetAttribute((const void *)csr_sweep_kernel<W>,                    \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
@@ -208,10 +362,27 @@ void launch_csr_sweeps(hipStream_t st, int waves, int groups, int max_words, con
   OS_HIP_CHECK(hipGetLastError());
 }
 
-int csr_max_groups_per_cu(int waves, int max_words) {
+int csr_max_groups_per_cu(bool pf, int waves, int max_words) {
   const size_t lds = (size_t)csr_lds_bytes(waves, max_words);
   int blocks = 0;
   hipError_t e = hipSuccess;
+  if (pf) {
+    switch (waves) {
+      case 1:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_pf_kernel<1>, 64, lds);
+        break;
+      case 2:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_pf_kernel<2>, 128, lds);
+        break;
+      case 4:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_pf_kernel<4>, 256, lds);
+        break;
+      default:
+        throw Error(-3, "csr sweep kernel: unsupported waves per group");
+    }
+    OS_HIP_CHECK(e);
+    return blocks;
+  }
   switch (waves) {
     case 1:
       e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, csr_sweep_kernel<1>, 64, lds);

@@ -33,12 +33,15 @@ struct CsrBatch {
   int Rp = 0;
   DevBuf items, run, tscale, init, best, fin, eb, ef, e64, queue;
   int num_items = 0;
-  // launch shape chosen by the tuner (0: not tuned)
+  // launch shape chosen by the tuner (0: not tuned); reg: row-prefetching kernel
   int waves = 0, groups = 0;
+  bool reg = false;
+  int max_width = 0;
   // captured graph and what it was captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_waves = 0, g_groups = 0, g_Rp = 0;
+  bool g_reg = false;
   const void *g_tscale = nullptr;
 
   void drop_graph() {
@@ -54,10 +57,10 @@ struct CsrBatch {
       b->release();
   }
 
-  void enqueue(int w, int g, bool energy) {
+  void enqueue(bool rg, int w, int g, bool energy) {
     hipStream_t st = ctx->stream;
     launch_csr_queue_reset(st, queue.as<int>());
-    launch_csr_sweeps(st, w, g, max_words, d_inst.as<CsrInst>(), items.as<int2>(),
+    launch_csr_sweeps(st, rg, w, g, max_words, d_inst.as<CsrInst>(), items.as<int2>(),
                       run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(), diag.as<float>(),
                       tscale.as<float>(), init.as<uint32_t>(), best.as<uint32_t>(),
                       fin.as<uint32_t>(), eb.as<double>(), ef.as<double>(), queue.as<int>());
@@ -116,11 +119,11 @@ void shape_for(CsrBatch &b, int Rp) {
   b.drop_graph();
 }
 
-float time_launch(CsrBatch &b, int w, int g) {
+float time_launch(CsrBatch &b, bool rg, int w, int g) {
   Ctx &c = *b.ctx;
-  b.enqueue(w, g, false);  // warm-up
+  b.enqueue(rg, w, g, false);  // warm-up
   OS_HIP_CHECK(hipEventRecord(c.ev_start, c.stream));
-  b.enqueue(w, g, false);
+  b.enqueue(rg, w, g, false);
   OS_HIP_CHECK(hipEventRecord(c.ev_stop, c.stream));
   OS_HIP_CHECK(hipEventSynchronize(c.ev_stop));
   float ms = 0.f;
@@ -135,18 +138,22 @@ void tune(CsrBatch &b, const CsrRun &full) {
   probe.num_sweeps = std::min(full.num_sweeps, 2);
   OS_HIP_CHECK(hipMemcpyAsync(b.run.ptr, &probe, sizeof(CsrRun), hipMemcpyHostToDevice, c.stream));
   float best_ms = 0.f;
-  for (int w : {1, 2, 4}) {
-    const int maxb = csr_max_groups_per_cu(w, b.max_words);
-    if (maxb < 1) continue;
-    std::vector<int> gpcs{maxb};
-    if (maxb > 1) gpcs.push_back(std::max(1, maxb / 2));
-    for (int gpc : gpcs) {
-      const int g = std::max(1, std::min(c.num_cus * gpc, (b.num_items + w - 1) / w));
-      const float ms = time_launch(b, w, g);
-      if (b.waves == 0 || ms < best_ms) {
-        best_ms = ms;
-        b.waves = w;
-        b.groups = g;
+  for (const bool rg : {false, true}) {
+    if (rg && !csr_pf_eligible(b.max_width)) continue;
+    for (int w : {1, 2, 4}) {
+      const int maxb = csr_max_groups_per_cu(rg, w, b.max_words);
+      if (maxb < 1) continue;
+      std::vector<int> gpcs{maxb};
+      if (maxb > 1) gpcs.push_back(std::max(1, maxb / 2));
+      for (int gpc : gpcs) {
+        const int g = std::max(1, std::min(c.num_cus * gpc, (b.num_items + w - 1) / w));
+        const float ms = time_launch(b, rg, w, g);
+        if (b.waves == 0 || ms < best_ms) {
+          best_ms = ms;
+          b.reg = rg;
+          b.waves = w;
+          b.groups = g;
+        }
       }
     }
   }
@@ -213,6 +220,7 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
         }
       for (int j = 0; j < ni; ++j) dg.push_back((float)diag[spins + j]);
       b->max_words = std::max(b->max_words, d.words);
+      b->max_width = std::max(b->max_width, d.width);
       b->total_words += d.words;
       b->inst.push_back(d);
       ell += (int64_t)ni * d.width;
@@ -308,18 +316,20 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
       if (p->tune) {
         os::tune(b, run);
       } else {
+        b.reg = os::csr_pf_eligible(b.max_width);
         b.waves = 1;
-        b.groups = std::max(1, std::min(c.num_cus * os::csr_max_groups_per_cu(1, b.max_words),
+        b.groups = std::max(1, std::min(c.num_cus * os::csr_max_groups_per_cu(b.reg, 1, b.max_words),
                                         b.num_items));
       }
     }
     OS_HIP_CHECK(hipMemcpyAsync(b.run.ptr, &run, sizeof(os::CsrRun), hipMemcpyHostToDevice, s));
-    if (p->use_graph && (!b.gexec || b.g_waves != b.waves || b.g_groups != b.groups ||
+    if (p->use_graph && (!b.gexec || b.g_reg != b.reg || b.g_waves != b.waves ||
+                         b.g_groups != b.groups ||
                          b.g_Rp != Rp)) {
       b.drop_graph();
       OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        b.enqueue(b.waves, b.groups, true);
+        b.enqueue(b.reg, b.waves, b.groups, true);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s, &g);
@@ -328,6 +338,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
       }
       OS_HIP_CHECK(hipStreamEndCapture(s, &b.graph));
       OS_HIP_CHECK(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
+      b.g_reg = b.reg;
       b.g_waves = b.waves;
       b.g_groups = b.groups;
       b.g_Rp = Rp;
@@ -337,7 +348,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     if (p->use_graph)
       OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
     else
-      b.enqueue(b.waves, b.groups, true);
+      b.enqueue(b.reg, b.waves, b.groups, true);
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
     // results
     std::vector<double> e64((size_t)b.I * Rp);
@@ -397,12 +408,14 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
   });
 }
 
-int os_csr_batch_launch_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_t *groups) {
+int os_csr_batch_launch_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_t *groups,
+                              int32_t *prefetch) {
   return os::guarded([&] {
     OS_REQUIRE(b_, "null argument");
     const CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
     if (waves_per_group) *waves_per_group = b.waves;
     if (groups) *groups = b.groups;
+    if (prefetch) *prefetch = b.reg ? 1 : 0;
   });
 }
 

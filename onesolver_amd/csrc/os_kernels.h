@@ -101,13 +101,15 @@ struct CsrRun {
   int32_t pad_;
 };
 int csr_lds_bytes(int waves, int max_words);
-int csr_max_groups_per_cu(int waves, int max_words);
+// pf: the row-prefetching kernel (batches within csr_pf_eligible)
+bool csr_pf_eligible(int max_width);
+int csr_max_groups_per_cu(bool pf, int waves, int max_words);
 void launch_csr_queue_reset(hipStream_t st, int *queue);
-void launch_csr_sweeps(hipStream_t st, int waves, int groups, int max_words, const CsrInst *inst,
-                       const int2 *items, const CsrRun *run, const int4 *ell_col,
-                       const float4 *ell_val, const float *diag, const float *tscale,
-                       const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
-                       double *e_best, double *e_final, int *queue);
+void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_words,
+                       const CsrInst *inst, const int2 *items, const CsrRun *run,
+                       const int4 *ell_col, const float4 *ell_val, const float *diag,
+                       const float *tscale, const uint32_t *init_bits, uint32_t *best_bits,
+                       uint32_t *final_bits, double *e_best, double *e_final, int *queue);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
                        const double *diag64, const uint32_t *bits, double *out);

@@ -81,8 +81,48 @@ def test_csr_batch_matches_oracle_bit_for_bit(ctx, use_graph):
     b = onesolver_amd.CsrBatch(ctx, insts)
     out = b.anneal(R, ts, seed=11, instance_base=4, final_states=True, use_graph=use_graph)
     _check_against_oracle(out, insts, ts, R, 11, 4)
-    w, g = b.launch_shape()
+    w, g, _ = b.launch_shape()
     assert w in (1, 2, 4) and g >= 1
+    b.close()
+
+
+def _partly_isolated(n, m, seed):
+    """A 3-regular graph on the first m spins, the other n - m spins isolated,
+    real weights."""
+    rp0, cl0, vl0, dg0 = instances.regular_csr(m, 3, seed=seed, instance=0)
+    rng = np.random.default_rng(seed)
+    vl = np.asarray(vl0, dtype=np.float64).copy()
+    # symmetric real weights: one draw per unordered pair
+    w = {}
+    for j in range(m):
+        for k in range(rp0[j], rp0[j + 1]):
+            key = (min(j, cl0[k]), max(j, cl0[k]))
+            if key not in w:
+                w[key] = float(np.float32(rng.normal()))
+            vl[k] = w[key]
+    rp = np.concatenate([rp0, np.full(n - m, rp0[-1], dtype=np.int32)]).astype(np.int32)
+    dg = np.concatenate([np.asarray(dg0, np.float64),
+                         (rng.random(n - m, dtype=np.float32) - np.float32(0.5)).astype(np.float64)])
+    return rp, np.asarray(cl0, np.int32), vl, dg
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_csr_prefetch_kernel_matches_oracle(ctx, tune):
+    """Batches whose rows all have at most 8 couplings run the row-prefetching
+    kernel (the default when untuned): n up to 2048, n != 0 mod 32, one- and
+    two-chunk rows, isolated spins, R != 0 mod 64."""
+    insts = [instances.regular_csr(n, d, seed=7, instance=i)
+             for i, (n, d) in enumerate([(6, 3), (33, 4), (64, 8), (2048, 5), (1999, 6)])]
+    insts.append(_partly_isolated(45, 20, 3))
+    R, S = 100, 6
+    ts = _tscale(S)
+    print(f"[gpu] csr prefetch-kernel batch, tune={tune}", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=21, instance_base=2, final_states=True, tune=tune)
+    _check_against_oracle(out, insts, ts, R, 21, 2)
+    w, g, pf = b.launch_shape()
+    if not tune:
+        assert pf == 1
     b.close()
 
 
