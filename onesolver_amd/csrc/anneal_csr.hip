@@ -167,18 +167,25 @@ struct SiteRow {
   }
 };
 
+// Row of site j, read with vector loads: the (uniform) index is passed
+// through an empty asm statement so the compiler treats it as per-lane and
+// does not use scalar loads, whose completion counter LDS reads share -- a
+// row requested one site ahead would otherwise hold up the current site's
+// LDS gathers.
 __device__ __forceinline__ SiteRow load_row(const int4 *ecol, const float4 *eval, const float *dg,
                                             int j, int chunks, int n) {
+  int e0 = j * chunks, jv = j;
+  asm volatile("" : "+v"(e0), "+v"(jv));
   SiteRow rw;
-  rw.c0 = ecol[(size_t)j * chunks];
-  rw.v0 = eval[(size_t)j * chunks];
+  rw.c0 = ecol[e0];
+  rw.v0 = eval[e0];
   rw.c1 = make_int4(n, n, n, n);
   rw.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
   if (chunks > 1) {
-    rw.c1 = ecol[(size_t)j * chunks + 1];
-    rw.v1 = eval[(size_t)j * chunks + 1];
+    rw.c1 = ecol[e0 + 1];
+    rw.v1 = eval[e0 + 1];
   }
-  rw.dj = dg[j];
+  rw.dj = dg[jv];
   return rw;
 }
 
@@ -250,9 +257,14 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
     for (int t = 0; t < S; ++t) {
       const float ts = tscale[t];
       SiteRow row = load_row(ecol, eval, dg, 0, chunks, n);
+      // the state word of the current sites (j >> 5), kept in a register:
+      // flips update it and are written through to LDS for the gathers
+      uint32_t word = 0u;
       for (int j4 = 0; j4 < n; j4 += 4) {
         const uint4 o = philox4x32_10(
             make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+        uint32_t *xw = &xs[(j4 >> 5) * 64 + lane];
+        if ((j4 & 31) == 0) word = *xw;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int j = j4 + q;
@@ -261,12 +273,11 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
           const SiteRow nrow = load_row(ecol, eval, dg, min(j + 1, n - 1), chunks, n);
           const float thr = neg_ln_u(pick(o, (uint32_t)q)) * ts;
           const float h = row.field(xs, lane, chunks);
-          uint32_t *xw = &xs[(j >> 5) * 64 + lane];
-          const uint32_t word = *xw;
           const bool xj = (word >> (j & 31)) & 1u;
           const float de = xj ? -h : h;
           if (de < thr) {
-            *xw = word ^ (1u << (j & 31));
+            word ^= 1u << (j & 31);
+            *xw = word;
             e += (double)de;
           }
           row = nrow;
