@@ -28,7 +28,7 @@ PEAK_HBM_GBS = 8000.0
 # calibrated on a 1 GiB device copy (read bytes = 2 x FETCH_SIZE, written
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2962442.1 + 245838.8) * 1024   # dense_sweep_kernel, 512 sweeps
-TRAFFIC_CSR_C3 = (2 * 20552008.9 + 7267549.8) * 1024    # csr_sweep_kernel, 2048 sweeps
+TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
 TRAFFIC_PASS_C5 = (2 * 98064.8 + 131072.0) * 1024       # stream_pass_kernel, 4 blocks
 
 
@@ -303,7 +303,7 @@ def run_csr_c3(args):
                    "row_prefetch": bool(b.launch_shape()[2])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": TRAFFIC_CSR_C3 if S == 2048 else None,
+                     "traffic": TRAFFIC_CSR_C3 if S == 2048 and b.launch_shape()[2] else None,
                      "kernel": "csr_sweep_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_alg},
     }

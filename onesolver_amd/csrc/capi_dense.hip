@@ -234,7 +234,10 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     // pass launches: persistent work-groups per CU (OS_STREAM_PASS_WGS_PER_CU
     // overrides the default); the grid leaves room for the decide launches
     const char *wenv = std::getenv("OS_STREAM_PASS_WGS_PER_CU");
-    const int pgrid = c.num_cus * (wenv ? std::max(1, std::atoi(wenv)) : os::Dense::kPassWgsPerCu);
+    const char *grenv = std::getenv("OS_STREAM_PASS_GRID");
+    const int pgrid = grenv ? std::max(1, std::atoi(grenv))
+                           : c.num_cus * (wenv ? std::max(1, std::atoi(wenv))
+                                               : os::Dense::kPassWgsPerCu);
     // stream path, with stats: HIP events around every pass_stride-th pass
     // (at most kPassSamples of them)
     constexpr int kPassSamples = 256;
