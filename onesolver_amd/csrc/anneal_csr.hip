@@ -143,6 +143,8 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
   }
 }
 
+namespace {
+
 // One ELL row of at most 8 entries (chunks 1 or 2; the second chunk of a
 // one-chunk row is padding) and the site's diagonal, uniform across a wave.
 struct SiteRow {
@@ -188,6 +190,8 @@ __device__ __forceinline__ SiteRow load_row(const int4 *ecol, const float4 *eval
   rw.dj = dg[jv];
   return rw;
 }
+
+}  // namespace
 
 // Prefetching variant for batches whose ELL rows have at most 8 entries
 // (1 or 2 chunks): the same kernel, but the row, weights and diagonal of site

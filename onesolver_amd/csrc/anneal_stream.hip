@@ -1,23 +1,24 @@
 // anneal_stream.hip -- sweep-mode annealing of large dense QUBOs (n > 1024).
 //
 // When R x N fp32 fields no longer fit the register file (N=8192, R=4096:
-// 128 MiB), they live in HBM and every block of k = 64 spins g costs one pass
-// over them, in two launches that run side by side on two streams:
-//  - update(g) reads every 64x64 field tile except those of block b_g =
-//    g mod NB, adds the previous block's flips (D_{g-1} * Qhi + D_{g-1} * Qlo
-//    over the 64 rows of block b_{g-1}, bf16 v_mfma_f32_32x32x16_bf16, fp32
-//    accumulation) and writes it back; it uses no LDS, so the HBM stream runs
-//    at full occupancy;
-//  - decide(g) (one work-group per 64 replicas) does the same for the tiles
-//    of block b_g, then decides its 64 spins in order (lane = replica,
-//    in-block corrections with the fp32 64x64 diagonal sub-block of Q) and
-//    publishes D_g as +/- bit masks.
-// Both need only D_{g-1}, so the decisions leave the critical path: update(g+1)
-// waits for update(g) and decide(g), decide(g+1) for the same two.  Each
-// block moves the field array through HBM once (read + write).  Thresholds
-// are the sweep-mode ones of the register-resident kernel: flip iff dE <
-// neg_ln_u(w) * tscale[t], w = Philox word of counter (j>>2, replica, sweep,
-// tag).
+// 128 MiB), they live in HBM, in two copies.  Spins are decided in blocks of
+// k = 64; block g of a segment covers spins 64*(g mod NB).. of sweep g / NB
+// (NB = N/64), and its flips are kept as +/- bit masks in a ring of 3m
+// slots.  Two kinds of launch run side by side on two streams:
+//  - decide (one work-group per 64 replicas): reads the 64x64 field tile of
+//    block g from the last field copy, adds the flips of the blocks not yet
+//    in that copy (bf16 hi + lo v_mfma_f32_32x32x16_bf16, fp32 accumulation),
+//    decides the 64 spins in order (each replica's decisions split over a
+//    lane quad, in-block corrections with the fp32 64x64 diagonal sub-block
+//    of Q) and publishes the flip masks;
+//  - pass (persistent, one work-group per CU by default): adds the flips of a
+//    group of m blocks to every tile of the field array, from one copy into
+//    the other -- the field array moves through HBM once per m blocks.
+// Decide of block g in group H reads the copy after pass H-2, so pass H runs
+// while group H+1 is decided (the host side, capi_dense.hip, orders them
+// with events).  Thresholds are the sweep-mode ones of the register-resident
+// kernel: flip iff dE < neg_ln_u(w) * tscale[t], w = Philox word of counter
+// (j>>2, replica, sweep, tag).
 #include <algorithm>
 
 #include "os_kernels.h"
@@ -36,8 +37,6 @@ __device__ __forceinline__ sf32x16 smfma(uint4 a, uint4 b, sf32x16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(sbf16x8, a),
                                                  __builtin_bit_cast(sbf16x8, b), c, 0, 0, 0);
 }
-
-}  // namespace
 
 constexpr int kChunk = 2;     // decide launch: blocks whose operands load in one round trip
 constexpr int kPRows = 256;   // pass launch: replicas per tile (4 waves x 64)
@@ -121,6 +120,14 @@ __device__ __forceinline__ void tile_store(float *Fp, int np, const sf32x16 &C) 
 #pragma unroll
   for (int i = 0; i < 16; ++i) Fp[(size_t)(8 * (i >> 2) + (i & 3)) * np] = C[i];
 }
+
+// Value of lane (lane & ~3) | ow of each quad (ow compile-time).
+template <int ow>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_mov_dpp(v, ow * 0x55, 0xF, 0xF, false);
+}
+
+}  // namespace
 
 // Pass launch: Fout = Fin + the flips of the cnt blocks g0...  Persistent
 // grid over tiles of 256 replicas x 64 columns (column block fastest; with
@@ -219,12 +226,6 @@ __global__ __launch_bounds__(256) void stream_pass_kernel(int np, int Rp, int I,
                      acc[rt][ct]);
     }
   }
-}
-
-// Value of lane (lane & ~3) | ow of each quad (ow compile-time).
-template <int ow>
-__device__ __forceinline__ int quad_bcast(int v) {
-  return __builtin_amdgcn_mov_dpp(v, ow * 0x55, 0xF, 0xF, false);
 }
 
 // Decide launch for block g (block b = g mod NB of sweep `sweep`): grid
