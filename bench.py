@@ -7,7 +7,12 @@ bf16 hi+lo MFMA delayed field updates with block k=16, FP64 field refresh
 every 512 sweeps.  One step = one full anneal of that batch with Q already
 resident in HBM.  Prints one JSON line (see the keys below).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|copy]
+
+Other workloads: c1 dense N=28 in reference semantics (1000 iterations x
+1024 tries); c3 a batch of 512 sparse instances; c4 the learned warm start
+(time to 99% of the best known energy); c5 dense uniform N=8192 with the
+fields in HBM; copy the 1 GiB device copy that calibrates the PMC traffic.
 """
 import argparse
 import json

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -121,3 +122,11 @@ def test_cli_prefix_and_equals_forms(tmp_path):
     assert "Schedule type: linear" in r.stdout
     r = _run(["--beta", "1"])
     assert r.returncode == 1 and "ambiguous" in r.stderr
+
+
+def test_bench_rejects_more_than_one_gpu():
+    """bench.py runs on one GPU; --gpus N > 1 exits with status 2 and says so."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
