@@ -41,13 +41,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline_dense(q, n, budget_s=12.0):
-    """The reference's host path restated: random-site attempts with an O(N^2)
-    FP64 energy recompute each (or_anneal_literal), one thread, on a bounded
-    sample; plus the O(N) local-field port on all cores for comparison."""
+def _host_threads():
+    threads = os.cpu_count() or 1
+    try:
+        threads = min(threads, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    return min(threads, 16)  # the host share of one GPU
+
+
+def cpu_baseline_dense(q, n, budget_s=8.0):
+    """The reference's host paths restated: random-site attempts with an O(N^2)
+    FP64 energy recompute each (or_anneal_literal) on one thread (the
+    reference's --device-type host) and one replica per thread on all host
+    cores (--device-type cpu), on bounded samples; plus the O(N) local-field
+    restatements (reference semantics, and the sweep-mode port) on all cores."""
     import oracle
 
     beta = oracle.geometric_schedule(0.1, 3.0, 64)
+    threads = _host_threads()
     # calibrate on a short run, then time about budget_s seconds of work
     t0 = time.perf_counter()
     oracle.anneal(q, n, beta[:8], 1, literal=True, threads=1)
@@ -58,14 +70,24 @@ def cpu_baseline_dense(q, n, budget_s=12.0):
     oracle.anneal(q, n, sched, 1, literal=True, threads=1)
     lit_s = time.perf_counter() - t0
     lit_rate = iters / lit_s
-    threads = os.cpu_count() or 1
-    try:
-        threads = min(threads, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        pass
-    threads = min(threads, 16)  # the host share of one GPU
-    ts = (1.0 / np.geomspace(0.1, 3.0, 8)).astype(np.float32)
+    t0 = time.perf_counter()
+    oracle.anneal(q, n, sched, threads, literal=True, threads=threads)
+    litp_s = time.perf_counter() - t0
+    # O(N) restatements: calibrated to about 2 s each
+    t0 = time.perf_counter()
+    oracle.anneal(q, n, np.full(2000, 1.0), threads, literal=False, threads=threads)
+    per_f = (time.perf_counter() - t0) / 2000
+    fld_iters = int(max(2000, min(20_000_000, 2.0 / max(per_f, 1e-12))))
+    t0 = time.perf_counter()
+    oracle.anneal(q, n, np.full(fld_iters, 1.0), threads, literal=False, threads=threads)
+    fld_s = time.perf_counter() - t0
     reps = max(threads, 1) * 4
+    t0 = time.perf_counter()
+    oracle.sweep_dense(q, n, (1.0 / np.geomspace(0.1, 3.0, 1)).astype(np.float32), reps,
+                       threads=threads)
+    per_s = time.perf_counter() - t0
+    nsw = int(max(1, min(256, 2.0 / max(per_s, 1e-9))))
+    ts = (1.0 / np.geomspace(0.1, 3.0, nsw)).astype(np.float32)
     t0 = time.perf_counter()
     oracle.sweep_dense(q, n, ts, reps, threads=threads)
     f_s = time.perf_counter() - t0
@@ -73,6 +95,13 @@ def cpu_baseline_dense(q, n, budget_s=12.0):
         "value": lit_rate, "unit": "attempts/s", "cores": 1, "kind": "port",
         "sample": f"reference algorithm restated (O(N^2) FP64 energy per attempt, random site), "
                   f"1 replica x {iters} attempts on the N={n} instance, {lit_s:.1f} s",
+        "all_cores": {"value": threads * iters / litp_s, "unit": "attempts/s", "cores": threads,
+                      "sample": f"the same, {threads} replicas (one per thread) x {iters} "
+                                f"attempts, {litp_s:.1f} s"},
+        "reference_semantics_local_fields": {
+            "value": threads * fld_iters / fld_s, "unit": "attempts/s", "cores": threads,
+            "sample": f"reference semantics with O(N) FP64 local fields, {threads} replicas x "
+                      f"{fld_iters} attempts, {fld_s:.1f} s"},
         "local_field_port": {"value": reps * len(ts) * n / f_s, "unit": "attempts/s",
                              "cores": threads,
                              "sample": f"sweep-mode O(N) local fields, {reps} replicas x "
@@ -132,7 +161,9 @@ def run_dense(args):
                      else None,
                      "kernel": f"dense_sweep_kernel<{-(-n // 128)}>",
                      "avg_launch_ms": avg_launch_s * 1e3,
-                     "algorithmic_flops_per_launch": flops_per_launch},
+                     "algorithmic_flops_per_launch": flops_per_launch,
+                     # the same work counted as FP32 (2N flop per attempt: one field update)
+                     "fp32_equivalent_tflops": achieved / 2.0},
         "kernel_attempts_per_s": float(R) * S * n * args.steps / (kernel_ms / 1e3),
         "best_energy": best, "best_energy_ising": best + const,
         "max_field_drift": drift,
@@ -203,6 +234,10 @@ def run_dense_c5(args):
                      "kernel": "stream_pass_kernel", "avg_launch_ms": pass_s * 1e3,
                      "algorithmic_bytes_per_launch": field_bytes,
                      "blocks_per_pass": group, "mfma_tflops": flops / pass_s / 1e12},
+        # the dense counting rule over the whole job: 4N bf16 flop per attempt
+        "attempt_mfma": {"achieved": attempts / elapsed * 4.0 * n / 1e12,
+                         "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                         "frac": attempts / elapsed * 4.0 * n / 1e12 / PEAK_BF16_TFLOPS},
         "ms_per_block": per_block_s * 1e3,
         "best_energy": best, "max_field_drift": drift,
     }, None
@@ -244,12 +279,7 @@ def cpu_baseline_csr(inst, ts_full, budget_s=10.0):
     import oracle
 
     rp, cl, vl, dg = inst
-    threads = os.cpu_count() or 1
-    try:
-        threads = min(threads, len(os.sched_getaffinity(0)))
-    except AttributeError:
-        pass
-    threads = min(threads, 16)  # the host share of one GPU
+    threads = _host_threads()
     reps = max(threads, 1) * 8
     t0 = time.perf_counter()
     oracle.sweep_csr(rp, cl, vl, dg, ts_full[:2], reps, threads=threads)
