@@ -42,7 +42,7 @@ namespace {
 
 constexpr int kWaves = 4;
 constexpr int kRT = 64;    // replicas per work-group
-constexpr int kRing = 32;  // D blocks kept in LDS
+constexpr int kRing = 8;   // D blocks kept in LDS
 constexpr int kTRing = 4;  // threshold blocks kept in LDS
 constexpr int kLA = 3;     // thresholds are computed this many blocks ahead
 constexpr int kHelpers = kWaves - 1;
@@ -50,6 +50,11 @@ constexpr int kFT = kRT + 4;  // row stride (floats) of the transposed field til
 constexpr int kSpinLimit = 1 << 24;
 
 static_assert(kLA < kTRing, "a threshold slot is rewritten only after its block was decided");
+// The owner of step g writes ring slot g % kRing after it has seen D_{g-1}
+// published.  The owners of steps g-4 .. g-1 -- all four waves -- published
+// only after their own load_a of those steps, so every wave has read
+// D_{g-5} and older; slot g % kRing last held D_{g-kRing}.
+static_assert(kRing >= kWaves + 1, "a ring slot is rewritten only after every wave read it");
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
 
@@ -97,15 +102,17 @@ __device__ __forceinline__ void lds_add(int *p) {
 
 // Waits until *p >= target, then orders later LDS reads after the wait.
 // Returns false (and raises *abort) if the wait exceeds kSpinLimit sleeps or
-// another wave has aborted.
+// another wave has aborted (checked every 64 sleeps).
 __device__ __forceinline__ bool wait_ge(int *p, int target, int *abort) {
   int spins = 0;
   while (lds_load(p) < target) {
-    if (lds_load(abort)) return false;
     __builtin_amdgcn_s_sleep(1);
-    if (++spins > kSpinLimit) {
-      lds_store(abort, 1);
-      return false;
+    if ((++spins & 63) == 0) {
+      if (lds_load(abort)) return false;
+      if (spins > kSpinLimit) {
+        lds_store(abort, 1);
+        return false;
+      }
     }
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
@@ -200,10 +207,10 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   __shared__ uint32_t xs[kRT][NPW + 1];
   __shared__ double es[kRT];
   __shared__ double ebs[kRT];
-  __shared__ __align__(16) uint4 qds[64];  // the owner's fp32 16x16 diagonal sub-block
+  // fp32 16x16 diagonal sub-blocks of the owners of even and odd steps
+  __shared__ __align__(16) uint4 qds[2][64];
   __shared__ int d_count;
   __shared__ int thr_cnt[kTRing];
-  __shared__ int wnext[kWaves];
   __shared__ int abort_flag;
 
   const int tid = threadIdx.x;
@@ -239,7 +246,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     ebs[tid] = ebest[rbase + tid];
   }
   if (tid < kTRing) thr_cnt[tid] = kHelpers;
-  if (tid < kWaves) wnext[tid] = 0;
+  if (tid < 64) qds[0][tid] = reinterpret_cast<const uint4 *>(qdb)[tid];  // block 0 (step 0)
   if (tid == 0) {
     d_count = 0;
     abort_flag = 0;
@@ -275,10 +282,9 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         Bn[tt][p] = qpack[((size_t)x * NP + tile_col(w, tt, lc)) * 4 + 2 * p + kh];
   };
   // fetched a step early for this wave's next owner step: its tile's operands
-  // and its diagonal sub-block
+  // and its diagonal sub-block (stored to qds at the end of that step)
   uint4 Bo[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
   uint4 qdn = make_uint4(0u, 0u, 0u, 0u);
-  if (w == 0) qdn = reinterpret_cast<const uint4 *>(qdb)[lane];  // block 0 (step 0)
   load_ops(0);
 
   bool ok = true;
@@ -292,19 +298,14 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       if (!ok) break;
       load_a(dring[(g - 1) % kRing], lutp, lutn, lc, kh, A);
     }
-    if ((b1 & 3) == w && g + 1 < G) {
-      // next step is this wave's: its tile of block b1 needs D_g (rows of b)
-      const int t1 = (b1 >> 2) >> 1;
-#pragma unroll
-      for (int p = 0; p < 2; ++p)
-        Bo[p] = qpack[((size_t)b * NP + tile_col(w, t1, lc)) * 4 + 2 * p + kh];
-      qdn = reinterpret_cast<const uint4 *>(qdb)[(size_t)b1 * 64 + lane];
-    }
+    // the owner of the next step is one of this step's helpers; its loads sit
+    // in the helper branch, so the owner branch waits for none of them
+    const bool next_owner = (b1 & 3) == w && g + 1 < G;
     if (owner == w) {
       const int k = b >> 2;
       const int t = k >> 1;     // this wave's tile holding block b
       const int half = k & 1;   // its lanes 16*half .. +15
-      qds[lane] = qdn;
+      const double e_prev = es[lane];
       // (a) bring tile t up to date and transpose it (static indices, uniform branch)
 #pragma unroll
       for (int tt = 0; tt < TPW; ++tt) {
@@ -326,8 +327,6 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
                             C[m][tt][4 * q + 3]);
       }
       wave_sync();
-      ok = wait_ge(&thr_cnt[g % kTRing], kHelpers * (g / kTRing + 1), &abort_flag);
-      if (!ok) break;
       // ---- sequential decisions of block b, lane = replica ----
       const float(*tb_thr)[17] = thr[g % kTRing];
       const float *fcol = &fblk[16 * half][lane];
@@ -343,7 +342,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       // (row == column: the diagonal block of a symmetric Q is symmetric) are
       // read from LDS during step jj; sched_barrier keeps the compiler from
       // hoisting all 16 steps' reads.
-      const float4 *qdv = reinterpret_cast<const float4 *>(qds);
+      const float4 *qdv = reinterpret_cast<const float4 *>(qds[g & 1]);
       float fn = fcol[0], tn = tb_thr[lane][0];
       float4 qn[4];
 #pragma unroll
@@ -380,14 +379,8 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       xs[lane][b >> 1] = (word & ~(0xFFFFu << sh)) | (xb << sh);
       const uint32_t flips = xb ^ x0;
       const uint32_t dmask = (flips & ~x0) | ((flips & x0) << 16);
-      // ring slot g % kRing: every other wave must have consumed D_{g - kRing}
-      if (g >= kRing) {
-        for (int v = 0; v < kWaves && ok; ++v)
-          if (v != w) ok = wait_ge(&wnext[v], g - kRing + 1, &abort_flag);
-        if (!ok) break;
-      }
       dring[g % kRing][lane] = dmask;
-      const double e_now = es[lane] + (double)desum;
+      const double e_now = e_prev + (double)desum;
       es[lane] = e_now;
       // end of a sweep: best-state tracking (before the next owner may change xs)
       if (b == NB - 1 && e_now < ebs[lane]) {
@@ -408,50 +401,70 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
             for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
           }
       }
-    } else if (g > 0) {
-      // D_{g-1} into every tile, with the thresholds of block g+kLA computed
-      // under the MFMAs (their slot was last read in step g+kLA-kTRing < g)
-      const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
-      const int g2 = min(g + kLA, G - 1);
-      const float ts2 = tscale[g2 / NB];
-      // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
-      // the 10 Philox rounds and 4 logarithms are spread over the MFMA pairs
-      const int item = hw * 64 + lane;
-      const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
-      uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
-                            (uint32_t)(sweep0 + g2 / NB), tag);
-      uint32_t pk0 = k0, pk1 = k1;
-      float tv[4];
+    } else {
+      if (next_owner) {
+        // next step is this wave's: its tile of block b1 needs D_g (rows of b)
+        const int t1 = (b1 >> 2) >> 1;
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p)
+          Bo[p] = qpack[((size_t)b * NP + tile_col(w, t1, lc)) * 4 + 2 * p + kh];
+        qdn = reinterpret_cast<const uint4 *>(qdb)[(size_t)b1 * 64 + lane];
+      }
+      if (g > 0) {
+        // D_{g-1} into every tile, with the thresholds of block g+kLA computed
+        // under the MFMAs (their slot was last read in step g+kLA-kTRing < g)
+        const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
+        const int g2 = min(g + kLA, G - 1);
+        const float ts2 = tscale[g2 / NB];
+        // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
+        // the 10 Philox rounds and 4 logarithms are spread over the MFMA pairs
+        const int item = hw * 64 + lane;
+        const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
+        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
+                              (uint32_t)(sweep0 + g2 / NB), tag);
+        uint32_t pk0 = k0, pk1 = k1;
+        float tv[4];
 #pragma unroll
-        for (int tt = 0; tt < TPW; ++tt) {
-          const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
+        for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
-          const int slot = p * TPW + tt;
+          for (int tt = 0; tt < TPW; ++tt) {
+            const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
 #pragma unroll
-          for (int u = 0; u < 14; ++u) {
-            if ((u * 2 * TPW) / 14 != slot) continue;
-            if (u < 10) {
-              philox_round(ph, pk0, pk1, u);
-            } else {
-              const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
-              tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
+            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+            const int slot = p * TPW + tt;
+#pragma unroll
+            for (int u = 0; u < 14; ++u) {
+              if ((u * 2 * TPW) / 14 != slot) continue;
+              if (u < 10) {
+                philox_round(ph, pk0, pk1, u);
+              } else {
+                const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
+                tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
+              }
             }
           }
+        if (g + kLA < G) {
+          const int sl = (g + kLA) % kTRing;
+          store_thresholds(item, tv, thr[sl]);
+          if (hw == g % kHelpers)  // the fourth quarter rotates
+            block_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
+                             thr[sl]);
+          if (lane == 0) lds_add(&thr_cnt[sl]);
         }
-      if (g + kLA < G) {
-        const int sl = (g + kLA) % kTRing;
-        store_thresholds(item, tv, thr[sl]);
-        if (hw == g % kHelpers)  // the fourth quarter rotates
-          block_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
-                           thr[sl]);
-        if (lane == 0) lds_add(&thr_cnt[sl]);
       }
+      // Bo and qdn have arrived long since.  The role branches are divergent
+      // to the compiler, which lays the owner branch out after this one; this
+      // wait, on every path through this branch, keeps it from waiting for
+      // them (and for the next operands behind them) at the owner's MFMAs.
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    }
+    if (next_owner) {
+      // off the critical path: the next owner step's sub-block and thresholds
+      qds[(g + 1) & 1][lane] = qdn;
+      ok = wait_ge(&thr_cnt[(g + 1) % kTRing], kHelpers * ((g + 1) / kTRing + 1), &abort_flag);
+      if (!ok) break;
     }
     // D_0 .. D_{g-1} are in every tile of this wave; fetch D_g's operands
-    if (lane == 0) lds_store(&wnext[w], g);
     load_ops(b);
   }
   // drain: D_{G-1} into every tile
