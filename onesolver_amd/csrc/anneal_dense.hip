@@ -37,6 +37,7 @@ namespace os {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -46,6 +47,9 @@ constexpr int kRing = 8;   // D blocks kept in LDS
 constexpr int kTRing = 4;  // threshold blocks kept in LDS
 constexpr int kLA = 3;     // thresholds are computed this many blocks ahead
 constexpr int kHelpers = kWaves - 1;
+// Helpers 1 and 2 of a step compute the next thresholds; helper 0 is the next
+// step's owner, whose work in this step delays the decision chain.
+constexpr int kThrWriters = 2;
 constexpr int kFT = kRT + 4;  // row stride (floats) of the transposed field tile
 constexpr int kSpinLimit = 1 << 24;
 
@@ -207,8 +211,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   __shared__ uint32_t xs[kRT][NPW + 1];
   __shared__ double es[kRT];
   __shared__ double ebs[kRT];
-  // fp32 16x16 diagonal sub-blocks of the owners of even and odd steps
-  __shared__ __align__(16) uint4 qds[2][64];
+  __shared__ __align__(16) uint4 qds[64];  // the owner's fp32 16x16 diagonal sub-block
   __shared__ int d_count;
   __shared__ int thr_cnt[kTRing];
   __shared__ int abort_flag;
@@ -245,8 +248,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     es[tid] = energy[rbase + tid];
     ebs[tid] = ebest[rbase + tid];
   }
-  if (tid < kTRing) thr_cnt[tid] = kHelpers;
-  if (tid < 64) qds[0][tid] = reinterpret_cast<const uint4 *>(qdb)[tid];  // block 0 (step 0)
+  if (tid < kTRing) thr_cnt[tid] = kThrWriters;
   if (tid == 0) {
     d_count = 0;
     abort_flag = 0;
@@ -282,9 +284,10 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         Bn[tt][p] = qpack[((size_t)x * NP + tile_col(w, tt, lc)) * 4 + 2 * p + kh];
   };
   // fetched a step early for this wave's next owner step: its tile's operands
-  // and its diagonal sub-block (stored to qds at the end of that step)
+  // and its diagonal sub-block
   uint4 Bo[2] = {make_uint4(0u, 0u, 0u, 0u), make_uint4(0u, 0u, 0u, 0u)};
   uint4 qdn = make_uint4(0u, 0u, 0u, 0u);
+  if (w == 0) qdn = reinterpret_cast<const uint4 *>(qdb)[lane];  // block 0 (step 0)
   load_ops(0);
 
   bool ok = true;
@@ -305,6 +308,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       const int k = b >> 2;
       const int t = k >> 1;     // this wave's tile holding block b
       const int half = k & 1;   // its lanes 16*half .. +15
+      qds[lane] = qdn;
       const double e_prev = es[lane];
       // (a) bring tile t up to date and transpose it (static indices, uniform branch)
 #pragma unroll
@@ -330,48 +334,59 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       // ---- sequential decisions of block b, lane = replica ----
       const float(*tb_thr)[17] = thr[g % kTRing];
       const float *fcol = &fblk[16 * half][lane];
-      float c[16];
-#pragma unroll
-      for (int jj = 0; jj < 16; ++jj) c[jj] = 0.0f;
       const uint32_t word = xs[lane][b >> 1];
       const int sh = (b & 1) * 16;
       uint32_t xb = (word >> sh) & 0xFFFFu;
       const uint32_t x0 = xb;
       float desum = 0.0f;
-      // Software-pipelined: step jj+1's field, threshold and sub-block row
-      // (row == column: the diagonal block of a symmetric Q is symmetric) are
-      // read from LDS during step jj; sched_barrier keeps the compiler from
-      // hoisting all 16 steps' reads.
-      const float4 *qdv = reinterpret_cast<const float4 *>(qds[g & 1]);
-      float fn = fcol[0], tn = tb_thr[lane][0];
-      float4 qn[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) qn[q] = qdv[q];
+      // The block's 16 fields and thresholds of this lane's replica are read
+      // at once; c[j] starts as the field of spin j and gains the in-block
+      // corrections.  Sub-block rows (row == column: the diagonal block of a
+      // symmetric Q is symmetric) are read two steps ahead; sched_barrier
+      // keeps the compiler from hoisting all 16 steps' reads.
+      float c[16], th[16];
 #pragma unroll
       for (int jj = 0; jj < 16; ++jj) {
-        const float fc = fn, tc = tn;
+        c[jj] = fcol[jj * kFT];
+        th[jj] = tb_thr[lane][jj];
+      }
+      // qds through a base register, so every row read is one ds_read with an
+      // immediate offset (a constant address costs a v_mov per read)
+      uint32_t qa = lds_addr(qds);
+      asm volatile("" : "+v"(qa));
+      const __attribute__((address_space(3))) f32x4 *qdv =
+          (const __attribute__((address_space(3))) f32x4 *)(uintptr_t)qa;
+      f32x4 qn[2][4];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (4 * q + 3 > r) qn[r][q] = qdv[r * 4 + q];
+#pragma unroll
+      for (int jj = 0; jj < 16; ++jj) {
         float qc[16];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          qc[4 * q] = qn[q].x;
-          qc[4 * q + 1] = qn[q].y;
-          qc[4 * q + 2] = qn[q].z;
-          qc[4 * q + 3] = qn[q].w;
-        }
-        if (jj < 15) {
-          fn = fcol[(jj + 1) * kFT];
-          tn = tb_thr[lane][jj + 1];
+        for (int q = 0; q < 4; ++q)
+          if (4 * q + 3 > jj) {
+            qc[4 * q] = qn[jj & 1][q][0];
+            qc[4 * q + 1] = qn[jj & 1][q][1];
+            qc[4 * q + 2] = qn[jj & 1][q][2];
+            qc[4 * q + 3] = qn[jj & 1][q][3];
+          }
+        if (jj < 14) {
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            if (4 * q + 3 > jj + 1) qn[q] = qdv[(jj + 1) * 4 + q];
+            if (4 * q + 3 > jj + 2) qn[jj & 1][q] = qdv[(jj + 2) * 4 + q];
         }
-        const float h = fc + c[jj];
-        const bool xj = (xb >> jj) & 1u;
-        const float de = xj ? -h : h;
-        const bool acc = de < tc;
-        const float d = acc ? (xj ? -1.0f : 1.0f) : 0.0f;
+        // de = (1 - 2 x_j) h and d = +-1 (or 0) by the sign bit of x_j; d * h
+        // is exact, so desum gains exactly de on a flip
+        const float h = c[jj];
+        const uint32_t sg = (x0 << (31 - jj)) & 0x80000000u;
+        const float de = __uint_as_float(__float_as_uint(h) ^ sg);
+        const bool acc = de < th[jj];
+        const float d = acc ? __uint_as_float(0x3F800000u | sg) : 0.0f;
         xb ^= acc ? (1u << jj) : 0u;
-        desum += acc ? de : 0.0f;
+        desum = fmaf(d, h, desum);
 #pragma unroll
         for (int mm = jj + 1; mm < 16; ++mm) c[mm] = fmaf(qc[mm], d, c[mm]);
         __builtin_amdgcn_sched_barrier(0);
@@ -411,45 +426,67 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         qdn = reinterpret_cast<const uint4 *>(qdb)[(size_t)b1 * 64 + lane];
       }
       if (g > 0) {
-        // D_{g-1} into every tile, with the thresholds of block g+kLA computed
-        // under the MFMAs (their slot was last read in step g+kLA-kTRing < g)
         const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
-        const int g2 = min(g + kLA, G - 1);
-        const float ts2 = tscale[g2 / NB];
-        // thresholds of item hw*64 + lane (replica r, spins 16*bn + 4q .. +3):
-        // the 10 Philox rounds and 4 logarithms are spread over the MFMA pairs
-        const int item = hw * 64 + lane;
-        const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
-        uint4 ph = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)(item >> 2),
-                              (uint32_t)(sweep0 + g2 / NB), tag);
-        uint32_t pk0 = k0, pk1 = k1;
-        float tv[4];
+        if (hw == 0) {
+          // the next owner: D_{g-1} into every tile and nothing else
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+          for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int tt = 0; tt < TPW; ++tt) {
-            const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
+            for (int tt = 0; tt < TPW; ++tt) {
+              const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
 #pragma unroll
-            for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
-            const int slot = p * TPW + tt;
+              for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+            }
+        } else {
+          // D_{g-1} into every tile, with the thresholds of block g+kLA
+          // computed under the MFMAs (their slot was last read in step
+          // g+kLA-kTRing < g): items (hw-1)*128 + lane and + 64 (replica
+          // r = item/4, spins 16*bn + 4q .. +3, q = item%4); their 10 Philox
+          // rounds and 4 logarithms are spread over the MFMA pairs
+          const int g2 = min(g + kLA, G - 1);
+          const float ts2 = tscale[g2 / NB];
+          const int item = (hw - 1) * 128 + lane;
+          const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
+          uint4 ph[2];
 #pragma unroll
-            for (int u = 0; u < 14; ++u) {
-              if ((u * 2 * TPW) / 14 != slot) continue;
-              if (u < 10) {
-                philox_round(ph, pk0, pk1, u);
-              } else {
-                const float x = neg_ln_u(pick(ph, u - 10)) * ts2;
-                tv[u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
+          for (int i = 0; i < 2; ++i)
+            ph[i] = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)((item + 64 * i) >> 2),
+                               (uint32_t)(sweep0 + g2 / NB), tag);
+          uint32_t pk0 = k0, pk1 = k1;
+          float tv[2][4];
+#pragma unroll
+          for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int tt = 0; tt < TPW; ++tt) {
+              const bf16x8 Bt = as_bf16x8(Bn[tt][p]);
+#pragma unroll
+              for (int m = 0; m < 2; ++m) C[m][tt] = mfma32(A[m], Bt, C[m][tt]);
+              const int slot = p * TPW + tt;
+#pragma unroll
+              for (int u = 0; u < 14; ++u) {
+                if ((u * 2 * TPW) / 14 != slot) continue;
+                if (u < 10) {
+                  if (u) {
+                    pk0 += kPhiloxW0;
+                    pk1 += kPhiloxW1;
+                  }
+#pragma unroll
+                  for (int i = 0; i < 2; ++i) philox_round(ph[i], pk0, pk1, 0);
+                } else {
+#pragma unroll
+                  for (int i = 0; i < 2; ++i) {
+                    const float x = neg_ln_u(pick(ph[i], u - 10)) * ts2;
+                    tv[i][u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
+                  }
+                }
               }
             }
+          if (g + kLA < G) {
+            const int sl = (g + kLA) % kTRing;
+            store_thresholds(item, tv[0], thr[sl]);
+            store_thresholds(item + 64, tv[1], thr[sl]);
+            if (lane == 0) lds_add(&thr_cnt[sl]);
           }
-        if (g + kLA < G) {
-          const int sl = (g + kLA) % kTRing;
-          store_thresholds(item, tv, thr[sl]);
-          if (hw == g % kHelpers)  // the fourth quarter rotates
-            block_thresholds(192 + lane, g2 % NB, g2 / NB, n, rloc, sweep0, k0, k1, tag, ts2,
-                             thr[sl]);
-          if (lane == 0) lds_add(&thr_cnt[sl]);
         }
       }
       // Bo and qdn have arrived long since.  The role branches are divergent
@@ -459,9 +496,8 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     }
     if (next_owner) {
-      // off the critical path: the next owner step's sub-block and thresholds
-      qds[(g + 1) & 1][lane] = qdn;
-      ok = wait_ge(&thr_cnt[(g + 1) % kTRing], kHelpers * ((g + 1) / kTRing + 1), &abort_flag);
+      // off the critical path: the thresholds of the next owner step
+      ok = wait_ge(&thr_cnt[(g + 1) % kTRing], kThrWriters * ((g + 1) / kTRing + 1), &abort_flag);
       if (!ok) break;
     }
     // D_0 .. D_{g-1} are in every tile of this wave; fetch D_g's operands
