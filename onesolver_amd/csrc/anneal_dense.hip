@@ -47,9 +47,12 @@ constexpr int kRing = 8;   // D blocks kept in LDS
 constexpr int kTRing = 4;  // threshold blocks kept in LDS
 constexpr int kLA = 3;     // thresholds are computed this many blocks ahead
 constexpr int kHelpers = kWaves - 1;
-// Helpers 1 and 2 of a step compute the next thresholds; helper 0 is the next
-// step's owner, whose work in this step delays the decision chain.
-constexpr int kThrWriters = 2;
+// The last kThrWriters helpers of a step compute the thresholds of a later
+// block.  The first helpers own the next steps: their work in this step delays
+// the decision chain.
+constexpr int kThrWriters = 1;
+constexpr int kThrItems = 4 / kThrWriters;  // threshold items per lane and writer
+static_assert(kThrWriters >= 1 && kThrWriters <= kHelpers && 4 % kThrWriters == 0, "writers");
 constexpr int kFT = kRT + 4;  // row stride (floats) of the transposed field tile
 constexpr int kSpinLimit = 1 << 24;
 
@@ -427,8 +430,8 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
       }
       if (g > 0) {
         const int hw = (w - owner - 1 + kWaves) % kWaves;  // 0 .. kHelpers-1
-        if (hw == 0) {
-          // the next owner: D_{g-1} into every tile and nothing else
+        if (hw < kHelpers - kThrWriters) {
+          // an owner of one of the next steps: D_{g-1} into every tile only
 #pragma unroll
           for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -440,20 +443,20 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
         } else {
           // D_{g-1} into every tile, with the thresholds of block g+kLA
           // computed under the MFMAs (their slot was last read in step
-          // g+kLA-kTRing < g): items (hw-1)*128 + lane and + 64 (replica
+          // g+kLA-kTRing < g): items item + 64i, i < kThrItems (replica
           // r = item/4, spins 16*bn + 4q .. +3, q = item%4); their 10 Philox
           // rounds and 4 logarithms are spread over the MFMA pairs
           const int g2 = min(g + kLA, G - 1);
           const float ts2 = tscale[g2 / NB];
-          const int item = (hw - 1) * 128 + lane;
+          const int item = (hw - (kHelpers - kThrWriters)) * 64 * kThrItems + lane;
           const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
-          uint4 ph[2];
+          uint4 ph[kThrItems];
 #pragma unroll
-          for (int i = 0; i < 2; ++i)
+          for (int i = 0; i < kThrItems; ++i)
             ph[i] = make_uint4((uint32_t)(j0 >> 2), rloc + (uint32_t)((item + 64 * i) >> 2),
                                (uint32_t)(sweep0 + g2 / NB), tag);
           uint32_t pk0 = k0, pk1 = k1;
-          float tv[2][4];
+          float tv[kThrItems][4];
 #pragma unroll
           for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -471,10 +474,10 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
                     pk1 += kPhiloxW1;
                   }
 #pragma unroll
-                  for (int i = 0; i < 2; ++i) philox_round(ph[i], pk0, pk1, 0);
+                  for (int i = 0; i < kThrItems; ++i) philox_round(ph[i], pk0, pk1, 0);
                 } else {
 #pragma unroll
-                  for (int i = 0; i < 2; ++i) {
+                  for (int i = 0; i < kThrItems; ++i) {
                     const float x = neg_ln_u(pick(ph[i], u - 10)) * ts2;
                     tv[i][u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
                   }
@@ -483,8 +486,8 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
             }
           if (g + kLA < G) {
             const int sl = (g + kLA) % kTRing;
-            store_thresholds(item, tv[0], thr[sl]);
-            store_thresholds(item + 64, tv[1], thr[sl]);
+#pragma unroll
+            for (int i = 0; i < kThrItems; ++i) store_thresholds(item + 64 * i, tv[i], thr[sl]);
             if (lane == 0) lds_add(&thr_cnt[sl]);
           }
         }
