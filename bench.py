@@ -32,7 +32,7 @@ PEAK_HBM_GBS = 8000.0
 # (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) on the default configurations,
 # calibrated on a 1 GiB device copy (read bytes = 2 x FETCH_SIZE, written
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
-TRAFFIC_DENSE_512 = (2 * 2962442.1 + 245838.8) * 1024   # dense_sweep_kernel, 512 sweeps
+TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
 TRAFFIC_PASS_C5 = (2 * 98064.8 + 131072.0) * 1024       # stream_pass_kernel, 4 blocks
 
