@@ -34,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
-TRAFFIC_PASS_C5 = (2 * 98064.8 + 131072.0) * 1024       # stream_pass_kernel, 4 blocks
+TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
 
 
 def log(msg):
@@ -230,7 +230,7 @@ def run_dense_c5(args):
         "roofline": {"bound": "hbm", "achieved": field_bytes / pass_s / 1e9,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": field_bytes / pass_s / 1e9 / PEAK_HBM_GBS,
-                     "traffic": TRAFFIC_PASS_C5 if (R, group) == (4096, 4.0) else None,
+                     "traffic": TRAFFIC_PASS_C5 if R == 4096 and round(group) == 5 else None,
                      "kernel": "stream_pass_kernel", "avg_launch_ms": pass_s * 1e3,
                      "algorithmic_bytes_per_launch": field_bytes,
                      "blocks_per_pass": group, "mfma_tflops": flops / pass_s / 1e12},

@@ -24,7 +24,7 @@ struct Dense {
   // decide launches on `side`, beside the field passes on the context's
   // stream; ev_dec marks the last decide of a group, ev_pass[G % 2] pass G
   DevBuf dblk;
-  static constexpr int kGroup = 4;      // default blocks per field pass
+  static constexpr int kGroup = 5;      // default blocks per field pass
   static constexpr int kPassChunk = 2;  // default operand blocks per round trip (pass)
   static constexpr int kPassWgsPerCu = 1;  // default persistent pass work-groups per CU (room for decides)
   int group = kGroup;
