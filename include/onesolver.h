@@ -77,6 +77,20 @@ int os_anneal_reference(os_ctx *ctx, int n, const double *q_sym, const double *b
 int os_energy_fp64(os_ctx *ctx, int n, const double *q_sym, uint32_t count,
                    const uint8_t *states, double *energies);
 
+/* Exact minimum by enumeration of all 2^n states.  Replaces exhaustive::solve
+ * (include/exhaustive/exhaustive.hpp:29-167): E(x) = sum_{i <= j} q[i][j]
+ * x_i x_j over the upper triangle of q_upper (n x n FP64, row-major; the
+ * diagonal holds the linear terms, entries below it are ignored), summed in
+ * the reference's order (i ascending, then j >= i ascending); of equal
+ * energies the lowest state index wins, bit i of the index being variable i.
+ * n <= 40 (the reference: n <= 30, exhaustive.hpp:67; 32 for its state
+ * conversion, ulong_to_vec.hpp:23-26).  Non-finite coefficients are supported
+ * up to n = 30; a state whose energy is NaN or +inf never wins, as in
+ * exhaustive.hpp:108-132.  Outputs: best_state [n] (0/1 bytes) and
+ * best_energy. */
+int os_exhaustive(os_ctx *ctx, int n, const double *q_upper, uint8_t *best_state,
+                  double *best_energy);
+
 /* ---- sweep-mode annealing (configs quoted in "sweeps") -------------------
  * A sweep visits every site once in the fixed order j = 0..N-1; site j of
  * sweep t flips iff dE < -ln(u) * tscale[t] with u the open-interval 24-bit

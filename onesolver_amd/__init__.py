@@ -58,6 +58,7 @@ def lib():
                                           ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, _u8p,
                                           _f64p, _u8p, _f64p]
         L.os_energy_fp64.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.c_uint32, _u8p, _f64p]
+        L.os_exhaustive.argtypes = [_vp, ctypes.c_int, _f64p, _u8p, _f64p]
         L.os_dense_create.argtypes = [_vp, ctypes.c_int, _f64p, ctypes.POINTER(_vp)]
         L.os_dense_destroy.argtypes = [_vp]
         L.os_dense_anneal.argtypes = [_vp, ctypes.POINTER(SweepParams), _f64p, _u8p, _u8p, _u8p,
@@ -167,6 +168,17 @@ class Context:
         _check(lib().os_energy_fp64(self._h, n, _p(q, _f64p), st.shape[0], _p(st, _u8p),
                                     _p(out, _f64p)))
         return out
+
+
+    def exhaustive(self, q_upper):
+        """(state, energy) of the minimum over all 2^n states of the upper
+        triangle of q_upper, lowest state index on ties (os_exhaustive)."""
+        q = np.ascontiguousarray(q_upper, dtype=np.float64)
+        n = q.shape[0]
+        st = np.zeros(n, dtype=np.uint8)
+        e = ctypes.c_double(0.0)
+        _check(lib().os_exhaustive(self._h, n, _p(q, _f64p), _p(st, _u8p), ctypes.byref(e)))
+        return st, e.value
 
 
 class DenseProblem:

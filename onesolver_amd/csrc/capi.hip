@@ -5,6 +5,7 @@
 // message, mirroring the reference's exception types (std::invalid_argument
 // for bad input, sycl::exception for device failures).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -135,6 +136,35 @@ int os_energy_fp64(os_ctx *ctx_, int n, const double *q_sym, uint32_t count,
     OS_HIP_CHECK(hipMemcpyAsync(energies, c.b_energy.ptr, sizeof(double) * count,
                                 hipMemcpyDeviceToHost, c.stream));
     c.sync();
+  });
+}
+
+int os_exhaustive(os_ctx *ctx_, int n, const double *q_upper, uint8_t *best_state,
+                  double *best_energy) {
+  return guarded([&] {
+    OS_REQUIRE(ctx_ != nullptr, "context is null");
+    OS_REQUIRE(n >= 1, "n must be positive");
+    OS_REQUIRE(q_upper && best_state && best_energy, "null array");
+    if (n > os::exhaustive_max_n())
+      throw os::Error(OS_ERR_UNSUPPORTED,
+                      "exhaustive search supports n <= " + std::to_string(os::exhaustive_max_n()));
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    bool finite = true;
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j) finite = finite && std::isfinite(q_upper[(size_t)i * n + j]);
+    const size_t qbytes = sizeof(double) * (size_t)n * n;
+    const size_t parts = os::exhaustive_partials(n);
+    c.b_q.ensure(qbytes);
+    c.b_energy.ensure(sizeof(double) * parts);
+    c.b_scratch.ensure(sizeof(uint64_t) * parts);
+    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_upper, qbytes, hipMemcpyHostToDevice, c.stream));
+    double e = 0.0;
+    uint64_t x = 0;
+    os::exhaustive_solve(c.stream, n, c.b_q.as<double>(), finite, c.b_energy.as<double>(),
+                         c.b_scratch.as<uint64_t>(), parts, &e, &x);
+    for (int i = 0; i < n; ++i) best_state[i] = (uint8_t)((x >> i) & 1u);
+    *best_energy = e;
   });
 }
 
