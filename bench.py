@@ -7,12 +7,14 @@ bf16 hi+lo MFMA delayed field updates with block k=16, FP64 field refresh
 every 512 sweeps.  One step = one full anneal of that batch with Q already
 resident in HBM.  Prints one JSON line (see the keys below).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|exh|copy]
 
 Other workloads: c1 dense N=28 in reference semantics (1000 iterations x
 1024 tries); c3 a batch of 512 sparse instances; c4 the learned warm start
 (time to 99% of the best known energy); c5 dense uniform N=8192 with the
-fields in HBM; copy the 1 GiB device copy that calibrates the PMC traffic.
+fields in HBM; exh the exhaustive search (all 2^n states, dense U[-1,1] upper
+triangle, n=32 by default); copy the 1 GiB device copy that calibrates the PMC
+traffic.
 """
 import argparse
 import json
@@ -27,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 
 # HBM bytes per launch of the dominant kernel, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) on the default configurations,
@@ -410,12 +413,71 @@ def run_copy_calibration(args):
             "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes}
 
 
+def run_exhaustive(args):
+    """All 2^n states of a dense U[-1,1] instance (os_exhaustive), n=32 by
+    default.  One step = one search, Q uploaded by the call (0.01 ms)."""
+    import onesolver_amd
+    n = args.exh_n
+    rng = np.random.default_rng(0)
+    q = np.triu(rng.uniform(-1.0, 1.0, (n, n)))
+    ctx = onesolver_amd.Context(0)
+    for _ in range(args.warmup):
+        ctx.exhaustive(q)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, e = ctx.exhaustive(q)
+    elapsed = time.perf_counter() - t0
+    ctx.close()
+    states = float(2 ** n) * args.steps
+    # per state: one FP64 add for the energy and kLow - 1 = 4 for the low
+    # fields of the Gray-code walk (the second pass re-walks only the threads
+    # whose minimum is within the re-ranking tolerance)
+    flops = 5.0 * states
+    achieved = flops / elapsed / 1e12
+    res = {
+        "metric": "exhaustive states/s", "value": states / elapsed, "unit": "states/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp64", "data": "synthetic (numpy seed 0, U[-1,1] upper triangle)",
+        "config": {"workload": f"exhaustive search, dense n={n}, all 2^{n} states, lowest index "
+                               f"on ties", "n": n},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "exh_gray_kernel, both passes (whole os_exhaustive call)",
+                     "avg_launch_ms": elapsed / args.steps * 1e3,
+                     "algorithmic_flops_per_launch": 5.0 * 2 ** n},
+        "best_energy": e,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = _host_threads()
+        m = 31  # a full search of 2^31 states, Gray-code restatement on all cores
+        qs = np.triu(q[:m, :m]) + np.triu(q[:m, :m], 1).T
+        t0 = time.perf_counter()
+        oracle.exhaustive(qs, m, threads=threads)
+        port_s = time.perf_counter() - t0
+        ml = 22  # the reference's literal O(N^2) per-state loop on one thread
+        t0 = time.perf_counter()
+        oracle.exhaustive(qs[:ml, :ml], ml, literal=True)
+        lit_s = time.perf_counter() - t0
+        res["cpu_baseline"] = {
+            "value": 2.0 ** m / port_s, "unit": "states/s", "cores": threads, "kind": "port",
+            "sample": f"Gray-code restatement with literal re-ranking, all 2^{m} states of the "
+                      f"leading {m}x{m} block, {port_s:.1f} s",
+            "literal_one_thread": {"value": 2.0 ** ml / lit_s, "unit": "states/s", "cores": 1,
+                                   "sample": f"reference per-state O(N^2) loop, n={ml}, "
+                                             f"{lit_s:.2f} s"}}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "copy"])
+    ap.add_argument("--workload", default="c2",
+                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "copy"])
+    ap.add_argument("--exh-n", type=int, default=32)
     ap.add_argument("--c5-replicas", type=int, default=4096)
     ap.add_argument("--c5-sweeps", type=int, default=1024)
     ap.add_argument("--c5-group", type=int, default=0,
@@ -445,6 +507,9 @@ def main():
         return
     if args.workload == "c4":
         print(json.dumps(run_warmstart_c4(args)))
+        return
+    if args.workload == "exh":
+        print(json.dumps(run_exhaustive(args)))
         return
     if args.workload == "c5":
         res, _ = run_dense_c5(args)

@@ -23,6 +23,7 @@
 #include <string>
 #include <vector>
 
+#include "options.hpp"
 #include "onesolver/helpers/devices.hpp"
 #include "onesolver/model/qubo.hpp"
 #include "onesolver/model/solution.hpp"
@@ -50,83 +51,10 @@ const char *kHelp =
     "  --seed arg (=1234)                    seed of the Philox4x32-10 streams\n"
     "  --sweeps-per-beta arg (=1)            attempts per beta value\n";
 
-struct Options {
-  std::map<std::string, std::string> values;
-  bool help = false;
-};
-
 const std::vector<std::string> kNames = {"help",     "input",       "output",
                                          "num-iter", "num-tries",   "schedule-type",
                                          "beta-min", "beta-max",    "device-type",
                                          "seed",     "sweeps-per-beta"};
-
-std::string resolve(const std::string &given) {
-  for (const auto &n : kNames)
-    if (n == given) return n;
-  std::vector<std::string> hits;
-  for (const auto &n : kNames)
-    if (n.compare(0, given.size(), given) == 0) hits.push_back(n);
-  if (hits.empty()) throw std::runtime_error("unrecognised option '--" + given + "'");
-  if (hits.size() > 1) {
-    std::string m = "option '--" + given + "' is ambiguous and matches ";
-    for (size_t i = 0; i < hits.size(); ++i)
-      m += (i ? (i + 1 == hits.size() ? ", and " : ", ") : "") + ("'--" + hits[i] + "'");
-    throw std::runtime_error(m);
-  }
-  return hits[0];
-}
-
-Options parse(int argc, char *argv[]) {
-  Options o;
-  for (int i = 1; i < argc; ++i) {
-    std::string tok = argv[i];
-    if (tok.size() < 3 || tok.compare(0, 2, "--") != 0) {
-      if (tok.size() >= 2 && tok[0] == '-') throw std::runtime_error("unrecognised option '" + tok + "'");
-      throw std::runtime_error("too many positional options have been specified on the command line");
-    }
-    std::string body = tok.substr(2), val;
-    bool has_val = false;
-    const size_t eq = body.find('=');
-    if (eq != std::string::npos) {
-      val = body.substr(eq + 1);
-      body = body.substr(0, eq);
-      has_val = true;
-    }
-    const std::string name = resolve(body);
-    if (name == "help") {
-      if (has_val) throw std::runtime_error("option '--help' does not take any arguments");
-      o.help = true;
-      continue;
-    }
-    if (!has_val) {
-      if (i + 1 >= argc)
-        throw std::runtime_error("the required argument for option '--" + name + "' is missing");
-      val = argv[++i];
-    }
-    if (o.values.count(name))
-      throw std::runtime_error("option '--" + name + "' cannot be specified more than once");
-    o.values[name] = val;
-  }
-  return o;
-}
-
-template <typename T>
-T convert(const Options &o, const std::string &name, T def) {
-  auto it = o.values.find(name);
-  if (it == o.values.end()) return def;
-  std::istringstream is(it->second);
-  T v;
-  if (!(is >> v) || !is.eof() || (std::is_unsigned<T>::value && it->second.find('-') != std::string::npos))
-    throw std::runtime_error("the argument ('" + it->second + "') for option '--" + name +
-                             "' is invalid");
-  return v;
-}
-
-template <>
-std::string convert<std::string>(const Options &o, const std::string &name, std::string def) {
-  auto it = o.values.find(name);
-  return it == o.values.end() ? def : it->second;
-}
 
 }  // namespace
 
@@ -135,19 +63,19 @@ int main(int argc, char *argv[]) {
   unsigned num_iter, num_tries;
   double beta_min, beta_max;
   try {
-    Options opt = parse(argc, argv);
+    cli::Options opt = cli::parse(argc, argv, kNames);
     if (opt.help) {
       std::cout << kHelp << std::endl;
       return 0;
     }
-    num_iter = convert<unsigned>(opt, "num-iter", 100u);
-    num_tries = convert<unsigned>(opt, "num-tries", 100u);
-    schedule_type = convert<std::string>(opt, "schedule-type", "geometric");
-    beta_min = convert<double>(opt, "beta-min", 0.1);
-    beta_max = convert<double>(opt, "beta-max", 1.0);
-    device_type = convert<std::string>(opt, "device-type", "host");
-    const uint64_t seed = convert<uint64_t>(opt, "seed", 1234u);
-    const unsigned sweeps_per_beta = convert<unsigned>(opt, "sweeps-per-beta", 1u);
+    num_iter = cli::convert<unsigned>(opt, "num-iter", 100u);
+    num_tries = cli::convert<unsigned>(opt, "num-tries", 100u);
+    schedule_type = cli::convert<std::string>(opt, "schedule-type", "geometric");
+    beta_min = cli::convert<double>(opt, "beta-min", 0.1);
+    beta_max = cli::convert<double>(opt, "beta-max", 1.0);
+    device_type = cli::convert<std::string>(opt, "device-type", "host");
+    const uint64_t seed = cli::convert<uint64_t>(opt, "seed", 1234u);
+    const unsigned sweeps_per_beta = cli::convert<unsigned>(opt, "sweeps-per-beta", 1u);
 
     if (!opt.values.count("input")) {
       std::cerr << "No input file provided." << std::endl;

@@ -158,11 +158,12 @@ int os_exhaustive(os_ctx *ctx_, int n, const double *q_upper, uint8_t *best_stat
     c.b_q.ensure(qbytes);
     c.b_energy.ensure(sizeof(double) * parts);
     c.b_scratch.ensure(sizeof(uint64_t) * parts);
+    c.b_fields.ensure(sizeof(double) * std::max<size_t>(1, os::exhaustive_thread_slots(n)));
     OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_upper, qbytes, hipMemcpyHostToDevice, c.stream));
     double e = 0.0;
     uint64_t x = 0;
-    os::exhaustive_solve(c.stream, n, c.b_q.as<double>(), finite, c.b_energy.as<double>(),
-                         c.b_scratch.as<uint64_t>(), parts, &e, &x);
+    os::exhaustive_solve(c.stream, n, c.b_q.as<double>(), finite, c.b_fields.as<double>(),
+                         c.b_energy.as<double>(), c.b_scratch.as<uint64_t>(), parts, &e, &x);
     for (int i = 0; i < n; ++i) best_state[i] = (uint8_t)((x >> i) & 1u);
     *best_energy = e;
   });

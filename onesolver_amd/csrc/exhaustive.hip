@@ -17,10 +17,12 @@
 // their local fields h_j = q_jj + sum_{i != j} q_ij x_i in registers, so a
 // state costs kLow FMAs; a higher bit B flips once per 2^kLow states, with
 // its field summed from the LDS copy of q.
-//   pass 1: the minimum E_w of the walk energies;
-//   pass 2: the same walk; every state whose walk energy is within
-//           tol = 1e-9 (|E_w| + 1) of E_w gets its literal energy, and the
-//           (literal energy, index) minimum of those is the answer.
+//   pass 1: every thread's minimum of its walk energies, and their minimum
+//           E_w;
+//   pass 2: only the threads whose minimum is within tol = 1e-9 (|E_w| + 1)
+//           of E_w walk again; every state whose walk energy is within tol of
+//           E_w gets its literal energy, and the (literal energy, index)
+//           minimum of those is the answer.
 // A walk energy differs from the literal one by the rounding of at most 2^S
 // steps, far below tol, so the literal minimum is always among the
 // candidates and the result equals the reference's.
@@ -40,7 +42,7 @@ constexpr int kMaxN = 40;
 constexpr int kLiteralMaxN = 20;
 constexpr int kLow = 5;      // bits flipped in the unrolled order
 constexpr int kSegBits = 12;  // literal restart every 2^kSegBits states
-constexpr int kMaxTopBits = 18;
+constexpr int kMaxTopBits = 20;
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ bool better(double e, uint64_t x, double be, uint64_t bx) {
@@ -123,27 +125,41 @@ __global__ __launch_bounds__(kThreads) void exh_literal_kernel(int n, const doub
 }
 
 // Gray-code walks, 20 < n <= 40; L = n - P bits per thread, 2^P threads.
-// PASS 1 writes per-block minima of the walk energies (out_x unused);
-// PASS 2 writes per-block (literal energy, index) minima of the candidates
+// PASS 1 writes every thread's minimum walk energy to tmin and per-block
+// minima of them (out_x unused); PASS 2 walks the threads with tmin <= thr
+// and writes per-block (literal energy, index) minima of the candidates
 // e_walk <= thr.
 template <int PASS>
 __global__ __launch_bounds__(kThreads) void exh_gray_kernel(int n, int L, const double *__restrict__ q,
-                                                            double thr, double *__restrict__ out_e,
+                                                            double thr, double *__restrict__ tmin,
+                                                            double *__restrict__ out_e,
                                                             uint64_t *__restrict__ out_x) {
   __shared__ double qs[kMaxN * kMaxN];
-  load_q(n, q, qs);
-  __syncthreads();
   const int S = min(L, kSegBits);
   const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-  const bool active = t < (1ull << (n - L));
-  // the low kLow x kLow block of q, symmetric, in registers
+  bool active = t < (1ull << (n - L));
+  if (PASS == 2) {
+    if (active) active = tmin[t] <= thr;
+    if (!__syncthreads_or(active)) {  // no candidate in this block
+      if (threadIdx.x == 0) {
+        out_e[blockIdx.x] = DBL_MAX;
+        out_x[blockIdx.x] = 0;
+      }
+      return;
+    }
+  }
+  load_q(n, q, qs);
+  __syncthreads();
+  // the off-diagonal part of the low kLow x kLow block of q in registers:
+  // ql[i][j] = q_ij for i < j
   double ql[kLow][kLow];
 #pragma unroll
   for (int i = 0; i < kLow; ++i)
 #pragma unroll
-    for (int j = 0; j < kLow; ++j) ql[i][j] = qs[i * kMaxN + j];
+    for (int j = i + 1; j < kLow; ++j) ql[i][j] = qs[i * kMaxN + j];
   double emin = DBL_MAX;
   uint64_t xmin = 0;
+  double eg[8];  // PASS 1: the energies of 8 consecutive states
   const int segs = active ? 1 << (L - S) : 0;
   const int blocks = 1 << (S - kLow);
   for (int seg = 0; seg < segs; ++seg) {
@@ -166,11 +182,20 @@ __global__ __launch_bounds__(kThreads) void exh_gray_kernel(int n, int L, const 
           e = fma(s[b], h[b], e);
 #pragma unroll
           for (int j = 0; j < kLow; ++j)
-            if (j != b) h[j] = fma(s[b], ql[j][b], h[j]);
+            if (j != b) h[j] = fma(s[b], j < b ? ql[j][b] : ql[b][j], h[j]);
           s[b] = -s[b];
         }
         if (PASS == 1) {
-          emin = fmin(emin, e);
+          // minimum of each 8 states as a tree, then into emin; the barrier
+          // keeps the compiler from holding a whole block of energies
+          // (register pressure, occupancy)
+          eg[pos & 7] = e;
+          if ((pos & 7) == 7) {
+            const double m = fmin(fmin(fmin(eg[0], eg[1]), fmin(eg[2], eg[3])),
+                                  fmin(fmin(eg[4], eg[5]), fmin(eg[6], eg[7])));
+            emin = fmin(emin, m);
+            __builtin_amdgcn_sched_barrier(0);
+          }
         } else if (e <= thr) {
           const uint64_t x = xhi | (low0 ^ (uint32_t)(pos ^ (pos >> 1)));
           const double v = literal_energy(n, qs, x);
@@ -192,6 +217,7 @@ __global__ __launch_bounds__(kThreads) void exh_gray_kernel(int n, int L, const 
       }
     }
   }
+  if (PASS == 1 && t < (1ull << (n - L))) tmin[t] = emin;
   block_min(emin, xmin, out_e, out_x);
 }
 
@@ -203,8 +229,9 @@ static void gray_shape(int n, int *P, int *L) {
   *L = n - *P;
 }
 
-void exhaustive_solve(hipStream_t st, int n, const double *q_dev, bool finite, double *part_e,
-                      uint64_t *part_x, size_t part_cap, double *best_e, uint64_t *best_x) {
+void exhaustive_solve(hipStream_t st, int n, const double *q_dev, bool finite, double *tmin,
+                      double *part_e, uint64_t *part_x, size_t part_cap, double *best_e,
+                      uint64_t *best_x) {
   auto reduce = [&](int nb) {
     std::vector<double> e(nb);
     std::vector<uint64_t> x(nb);
@@ -236,14 +263,21 @@ void exhaustive_solve(hipStream_t st, int n, const double *q_dev, bool finite, d
   gray_shape(n, &P, &L);
   const int nb = (int)(((1ull << P) + kThreads - 1) / kThreads);
   OS_REQUIRE((size_t)nb <= part_cap, "partials buffer too small");
-  exh_gray_kernel<1><<<nb, kThreads, 0, st>>>(n, L, q_dev, 0.0, part_e, part_x);
+  exh_gray_kernel<1><<<nb, kThreads, 0, st>>>(n, L, q_dev, 0.0, tmin, part_e, part_x);
   OS_HIP_CHECK(hipGetLastError());
   reduce(nb);
   const double ew = *best_e;
   const double thr = ew + 1e-9 * (std::fabs(ew) + 1.0);
-  exh_gray_kernel<2><<<nb, kThreads, 0, st>>>(n, L, q_dev, thr, part_e, part_x);
+  exh_gray_kernel<2><<<nb, kThreads, 0, st>>>(n, L, q_dev, thr, tmin, part_e, part_x);
   OS_HIP_CHECK(hipGetLastError());
   reduce(nb);
+}
+
+size_t exhaustive_thread_slots(int n) {
+  if (n <= kLiteralMaxN) return 0;
+  int P, L;
+  gray_shape(std::min(n, kMaxN), &P, &L);
+  return (size_t)1 << P;
 }
 
 size_t exhaustive_partials(int n) {

@@ -28,13 +28,16 @@ void launch_ref_anneal(hipStream_t st, int n, const double *q, const double *bet
 
 // exhaustive.hip
 int exhaustive_max_n();
-// entries of the per-block partial buffers exhaustive_solve needs for n
+// entries of the per-block partial buffers, and of the per-thread minima,
+// that exhaustive_solve needs for n
 size_t exhaustive_partials(int n);
+size_t exhaustive_thread_slots(int n);
 // minimum over all 2^n states of the energy of the upper triangle of q_dev
 // (n x n FP64, device), lowest index on ties; `finite`: every used
 // coefficient is finite.  Synchronises the stream.
-void exhaustive_solve(hipStream_t st, int n, const double *q_dev, bool finite, double *part_e,
-                      uint64_t *part_x, size_t part_cap, double *best_e, uint64_t *best_x);
+void exhaustive_solve(hipStream_t st, int n, const double *q_dev, bool finite, double *tmin,
+                      double *part_e, uint64_t *part_x, size_t part_cap, double *best_e,
+                      uint64_t *best_x);
 
 // anneal_dense.hip
 int dense_max_n();

@@ -9,6 +9,7 @@ built with exact ties.
 """
 import json
 import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -118,3 +119,19 @@ def test_exhaustive_rejects_sizes(ctx):
     with pytest.raises(onesolver_amd.OneSolverError) as ei:
         ctx.exhaustive(np.zeros((41, 41)))
     assert ei.value.code == onesolver_amd.OS_ERR_UNSUPPORTED
+
+
+def test_exhaustive_cli_gpu_device(tmp_path):
+    """one-solver-exhaustive --device-type gpu on the config-1 instance."""
+    out = tmp_path / "c1.csv"
+    r = subprocess.run([onesolver_amd.EXHAUSTIVE_CLI_PATH, "--input",
+                        os.path.join(GOLDEN, "c1_dense28.qubo"), "--output", str(out),
+                        "--device-type", "gpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(os.path.join(GOLDEN, "c1_ground.json")) as f:
+        g = json.load(f)
+    head, row = out.read_text().splitlines()
+    assert head == ",".join(str(i) for i in range(28)) + ",energy"
+    vals = row.split(",")
+    assert [int(v) for v in vals[:28]] == g["state"]
+    assert abs(float(vals[28]) - g["energy"]) < 1e-4  # the CSV keeps 6 significant digits

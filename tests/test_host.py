@@ -130,3 +130,38 @@ def test_bench_rejects_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def _run_exh(args):
+    return subprocess.run([onesolver_amd.EXHAUSTIVE_CLI_PATH] + args, capture_output=True,
+                          text=True, timeout=120)
+
+
+def test_exhaustive_cli_help_and_validation():
+    r = _run_exh(["--help"])
+    assert r.returncode == 0
+    for opt in ["--input arg", "--output arg", "--device-type arg (=host)"]:
+        assert opt in r.stdout
+    for args, msg in [([], "No input file provided."),
+                      (["--input", "x"], "No output file provided."),
+                      (["--input", "x", "--output", "y", "--device-type", "tpu"],
+                       "Unknown device type: tpu"),
+                      (["--input", "/nonexistent.qubo", "--output", "y"],
+                       "can not open input file: /nonexistent.qubo")]:
+        r = _run_exh(args)
+        assert r.returncode == 255 and msg in r.stderr, (args, r.stderr)
+    r = _run_exh(["--bogus", "1"])
+    assert r.returncode == 1 and "error: unrecognised option '--bogus'" in r.stderr
+
+
+@pytest.mark.parametrize("device", ["host", "cpu"])
+def test_exhaustive_cli_host_devices_solve_test1(tmp_path, device):
+    out = tmp_path / "result.csv"
+    r = _run_exh(["--input", os.path.join(GOLDEN, "examples", "test1.qubo"), "--output",
+                  str(out), "--device-type", device])
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("Reading input from: ")
+    assert lines[1].startswith("Output will be saved to: ")
+    assert lines[2].startswith("Using device: ")
+    assert out.read_text().splitlines() == ["0,1,2,3,energy", "1,1,0,1,-12"]
