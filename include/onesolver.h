@@ -123,6 +123,29 @@ typedef struct os_sweep_stats {
   int32_t pass_launches;   /* dense n > 1024: field passes; else 0 */
 } os_sweep_stats;
 
+/* Replica exchange (parallel tempering) on a dense batch, n <= 1024.  Each
+ * instance runs num_ladders ladders of num_temperatures slots; replica
+ * r = l * num_temperatures + k starts in slot k of ladder l.  A replica sweeps
+ * at the temperature scale (1/beta) of its slot, with the thresholds of
+ * sweep-mode annealing.  After every exchange_interval sweeps, round t pairs
+ * the slots (k, k+1), k = t mod 2, t mod 2 + 2, ..., of every ladder; the two
+ * replicas swap slots with probability min(1, exp((beta_k - beta_{k+1})
+ * (E_k - E_{k+1}))), drawn as u < exp(.) with u = (w >> 8) 2^-24 and w word 0
+ * of Philox counter (k, l, t, ((instance + i) << 8) | 0x13).  The energies
+ * are the running energies of the launch (FP64 after a refresh).  No
+ * reference counterpart: the reference recommends replica exchange
+ * (benchmarks/annealing/performance.md:52-59) but anneals only. */
+typedef struct os_tempering_params {
+  uint32_t num_ladders;
+  uint32_t num_temperatures; /* >= 2 slots per ladder */
+  const float *tscale;       /* [num_temperatures] host array, > 0 */
+  int32_t num_sweeps;
+  int32_t exchange_interval; /* sweeps between exchange rounds, >= 1 */
+  uint64_t seed;             /* Philox key */
+  uint32_t instance;         /* stream id of this instance (< 2^24) */
+  int32_t refresh_interval;  /* a multiple of exchange_interval, or <= 0 (end only) */
+} os_tempering_params;
+
 typedef struct os_dense os_dense;
 
 /* Upload a dense QUBO (q_sym: n x n FP64 symmetric, the reference's flat
@@ -161,6 +184,12 @@ int os_dense_destroy(os_dense *dense);
 int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best_energies,
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats);
+/* Replica exchange on `dense` (see os_tempering_params).  Outputs as in
+ * os_dense_anneal, indexed by replica; final_slots (optional, [I][R]) holds
+ * the slot of every replica at the end. */
+int os_dense_tempering(os_dense *dense, const os_tempering_params *params, double *best_energies,
+                       uint8_t *best_states, uint8_t *final_states, uint32_t *final_slots,
+                       uint8_t *best_state, double *best_energy, os_sweep_stats *stats);
 
 /* ---- learned warm start --------------------------------------------------
  * A 3-layer message-passing network (hidden 64, bf16 weights, fp32

@@ -76,6 +76,14 @@ class SweepParams(ctypes.Structure):
                 ("init_marginals", _vp)]
 
 
+class TemperingParams(ctypes.Structure):
+    """os_tempering_params of include/onesolver.h."""
+    _fields_ = [("num_ladders", ctypes.c_uint32), ("num_temperatures", ctypes.c_uint32),
+                ("tscale", _vp), ("num_sweeps", ctypes.c_int32),
+                ("exchange_interval", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("instance", ctypes.c_uint32), ("refresh_interval", ctypes.c_int32)]
+
+
 class SweepStats(ctypes.Structure):
     """os_sweep_stats of include/onesolver.h."""
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
@@ -225,6 +233,42 @@ class DenseProblem:
         return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
                 "best_energy": e.value, "stats": st.as_dict()}
 
+    def tempering(self, ladders, tscale, num_sweeps, exchange_interval, seed=1, instance=0,
+                  refresh_interval=0, best_states=True, final_states=False):
+        """Replica exchange (os_dense_tempering): `ladders` ladders of
+        len(tscale) temperature slots; arrays indexed by replica."""
+        out = _tempering(self._h, 1, self.n, ladders, tscale, num_sweeps, exchange_interval, seed,
+                         instance, refresh_interval, best_states, final_states)
+        for k in ("best_energies", "best_states", "final_states", "final_slots", "best_state"):
+            if out[k] is not None:
+                out[k] = out[k][0]
+        out["best_energy"] = float(out["best_energy"][0])
+        return out
+
+
+def _tempering(h, I, n, ladders, tscale, num_sweeps, exchange_interval, seed, instance,
+               refresh_interval, best_states, final_states):
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    R = ladders * len(ts)
+    prm = TemperingParams(ladders, len(ts), ts.ctypes.data, num_sweeps, exchange_interval, seed,
+                          instance, refresh_interval)
+    be = np.zeros((I, R), dtype=np.float64)
+    bs = np.zeros((I, R, n), dtype=np.uint8) if best_states else None
+    fs = np.zeros((I, R, n), dtype=np.uint8) if final_states else None
+    sl = np.zeros((I, R), dtype=np.uint32)
+    best = np.zeros((I, n), dtype=np.uint8)
+    e = np.zeros(I, dtype=np.float64)
+    st = SweepStats()
+    L = lib()
+    L.os_dense_tempering.argtypes = [_vp, ctypes.POINTER(TemperingParams), _f64p, _u8p, _u8p,
+                                     ctypes.POINTER(ctypes.c_uint32), _u8p, _f64p,
+                                     ctypes.POINTER(SweepStats)]
+    _check(L.os_dense_tempering(h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p), _p(fs, _u8p),
+                                sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                _p(best, _u8p), _p(e, _f64p), ctypes.byref(st)))
+    return {"best_energies": be, "best_states": bs, "final_states": fs, "final_slots": sl,
+            "best_state": best, "best_energy": e, "stats": st.as_dict()}
+
 
 class DenseBatch:
     """Same-size dense QUBOs annealed together (os_dense_batch_create)."""
@@ -277,6 +321,13 @@ class DenseBatch:
             self.close()
         except Exception:
             pass
+
+    def tempering(self, ladders, tscale, num_sweeps, exchange_interval, seed=1, instance=0,
+                  refresh_interval=0, best_states=True, final_states=False):
+        """Replica exchange on every instance (os_dense_tempering); arrays are
+        instance-major, [I, R] and [I, R, n]."""
+        return _tempering(self._h, self.I, self.n, ladders, tscale, num_sweeps, exchange_interval,
+                          seed, instance, refresh_interval, best_states, final_states)
 
     def anneal(self, num_replicas, tscale, seed=1, instance=0, refresh_interval=0,
                init_states=None, init_marginals=None, best_states=True, final_states=False):

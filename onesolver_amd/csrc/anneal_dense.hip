@@ -30,7 +30,8 @@
 //
 // Acceptance: flip iff dE < neg_ln_u(w) * tscale[t] with w the Philox word of
 // counter (j>>2, replica, sweep, tag), i.e. with probability
-// min(1, exp(-dE / tscale^-1)).
+// min(1, exp(-dE / tscale^-1)).  With `tsr` (replica exchange) the scale is
+// tsr[row] of the replica instead, for the whole launch.
 #include "os_kernels.h"
 
 namespace os {
@@ -198,7 +199,8 @@ __device__ __forceinline__ int tile_col(int w, int t, int lc) {
 template <int TPW>
 __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     int n, const uint4 *__restrict__ qpack, const float *__restrict__ qdb,
-    const float *__restrict__ tscale, int sweep0, int nsweeps, uint32_t k0, uint32_t k1,
+    const float *__restrict__ tscale, const float *__restrict__ tsr, int sweep0, int nsweeps,
+    uint32_t k0, uint32_t k1,
     uint32_t inst_base, int gpi, float *__restrict__ fields, uint32_t *__restrict__ xbits,
     uint32_t *__restrict__ bestbits, double *__restrict__ energy, double *__restrict__ ebest,
     int *__restrict__ status) {
@@ -218,6 +220,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   __shared__ int d_count;
   __shared__ int thr_cnt[kTRing];
   __shared__ int abort_flag;
+  __shared__ float tsl[kRT];  // per-replica temperature scales (tsr), if given
 
   const int tid = threadIdx.x;
   const int w = tid >> 6;
@@ -250,6 +253,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
   if (tid < kRT) {
     es[tid] = energy[rbase + tid];
     ebs[tid] = ebest[rbase + tid];
+    tsl[tid] = tsr ? tsr[rbase + tid] : 0.0f;
   }
   if (tid < kTRing) thr_cnt[tid] = kThrWriters;
   if (tid == 0) {
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
     const int gblk = it >> 8;
     if (gblk < G)
       block_thresholds(it & 255, gblk % NB, gblk / NB, n, rloc, sweep0, k0, k1, tag,
-                       tscale[gblk / NB], thr[gblk]);
+                       tsr ? tsr[rbase + ((it & 255) >> 2)] : tscale[gblk / NB], thr[gblk]);
   }
   __syncthreads();
 
@@ -447,8 +451,13 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
           // r = item/4, spins 16*bn + 4q .. +3, q = item%4); their 10 Philox
           // rounds and 4 logarithms are spread over the MFMA pairs
           const int g2 = min(g + kLA, G - 1);
-          const float ts2 = tscale[g2 / NB];
           const int item = (hw - (kHelpers - kThrWriters)) * 64 * kThrItems + lane;
+          float tsi[kThrItems];
+          {
+            const float ts2 = tsr ? 0.0f : tscale[g2 / NB];
+#pragma unroll
+            for (int i = 0; i < kThrItems; ++i) tsi[i] = tsr ? tsl[(item + 64 * i) >> 2] : ts2;
+          }
           const int j0 = (g2 % NB) * 16 + 4 * (item & 3);
           uint4 ph[kThrItems];
 #pragma unroll
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(256, 1) void dense_sweep_kernel(
                 } else {
 #pragma unroll
                   for (int i = 0; i < kThrItems; ++i) {
-                    const float x = neg_ln_u(pick(ph[i], u - 10)) * ts2;
+                    const float x = neg_ln_u(pick(ph[i], u - 10)) * tsi[i];
                     tv[i][u - 10] = j0 + (u - 10) < n ? x : -INFINITY;
                   }
                 }
@@ -602,8 +611,8 @@ int dense_padded_n(int n) {
 }
 
 void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const uint4 *qpack,
-                         const float *qdb, const float *tscale, int sweep0, int nsweeps,
-                         uint64_t seed, uint32_t instance, int R_pad, float *fields,
+                         const float *qdb, const float *tscale, const float *tsr, int sweep0,
+                         int nsweeps, uint64_t seed, uint32_t instance, int R_pad, float *fields,
                          uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
                          int *status) {
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
@@ -612,7 +621,7 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const
   switch (np / (kWaves * 32)) {
 #define OS_DENSE_CASE(T)                                                                    \
   case T:                                                                                   \
-    dense_sweep_kernel<T><<<grid, block, 0, st>>>(n, qpack, qdb, tscale, sweep0, nsweeps, k0, \
+    dense_sweep_kernel<T><<<grid, block, 0, st>>>(n, qpack, qdb, tscale, tsr, sweep0, nsweeps, k0, \
                                                   k1, instance, gpi, fields, xbits, bestbits, \
                                                   energy, ebest, status);                   \
     break;

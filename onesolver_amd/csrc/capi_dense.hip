@@ -135,14 +135,29 @@ int os_dense_destroy(os_dense *d) {
   return dguard([&] { delete reinterpret_cast<Dense *>(d); });
 }
 
-int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energies,
-                    uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
-                    double *best_energy, os_sweep_stats *stats) {
+}  // extern "C"
+
+namespace {
+
+// Replica exchange: ladders of K temperature slots with scales tscale[K];
+// after every `interval` sweeps, adjacent slots (k, k+1), k of the round's
+// parity, swap their replicas with probability
+// min(1, exp((beta_k - beta_{k+1}) (E_k - E_{k+1}))).
+struct Exchange {
+  int K = 0;
+  const float *tscale = nullptr;
+  int interval = 0;
+  uint32_t *final_slots = nullptr;  // optional [I][R] slot of each replica at the end
+};
+
+int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double *best_energies,
+              uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
+              double *best_energy, os_sweep_stats *stats) {
   return dguard([&] {
     OS_REQUIRE(d_ && p, "null argument");
     OS_REQUIRE(p->num_replicas >= 1, "num_replicas must be positive");
     OS_REQUIRE(p->num_sweeps >= 0, "num_sweeps must be non-negative");
-    OS_REQUIRE(p->num_sweeps == 0 || p->tscale, "tscale is null");
+    OS_REQUIRE(p->num_sweeps == 0 || p->tscale || xc, "tscale is null");
     OS_REQUIRE(!(p->init_states && p->init_marginals), "give init_states or init_marginals, not both");
     Dense &d = *reinterpret_cast<Dense *>(d_);
     OS_REQUIRE((uint64_t)p->instance + (uint64_t)d.I <= (1u << 24),
@@ -155,7 +170,41 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     const size_t T = (size_t)I * Rp;  // state rows
     const size_t nn = (size_t)n * n;
     const int S = p->num_sweeps;
-    const int seg = p->refresh_interval > 0 ? p->refresh_interval : std::max(S, 1);
+    const int refresh = p->refresh_interval > 0 ? p->refresh_interval : std::max(S, 1);
+    const int seg = xc ? xc->interval : refresh;
+    if (xc) {
+      OS_REQUIRE(!d.stream, "replica exchange supports n <= 1024");
+      OS_REQUIRE(xc->K >= 2 && xc->tscale, "replica exchange needs >= 2 temperature slots");
+      OS_REQUIRE(R % xc->K == 0, "num_replicas must be a multiple of the temperature slots");
+      OS_REQUIRE(xc->interval >= 1, "exchange_interval must be positive");
+      OS_REQUIRE(p->refresh_interval <= 0 || p->refresh_interval % xc->interval == 0,
+                 "refresh_interval must be a multiple of exchange_interval");
+      for (int k = 0; k < xc->K; ++k)
+        OS_REQUIRE(xc->tscale[k] > 0.0f && std::isfinite(xc->tscale[k]),
+                   "temperature scales must be positive");
+    }
+    // replica exchange: the slot of every replica and the replica at every
+    // slot (per instance and ladder), and the per-row scales of the launch
+    const int K = xc ? xc->K : 0, L = xc ? R / K : 0;
+    std::vector<uint32_t> slot_of, rep_at;
+    std::vector<float> tsr_h;
+    if (xc) {
+      slot_of.resize((size_t)I * R);
+      rep_at.resize((size_t)I * R);
+      for (size_t i = 0; i < (size_t)I * R; ++i) {
+        slot_of[i] = (uint32_t)(i % R % K);
+        rep_at[i] = (uint32_t)(i % R);
+      }
+      tsr_h.assign(T, xc->tscale[0]);
+      d.tsr.ensure(sizeof(float) * T);
+    }
+    auto upload_tsr = [&]() {
+      for (int s = 0; s < I; ++s)
+        for (int r = 0; r < R; ++r)
+          tsr_h[(size_t)s * Rp + r] = xc->tscale[slot_of[(size_t)s * R + r]];
+      OS_HIP_CHECK(hipMemcpyAsync(d.tsr.ptr, tsr_h.data(), sizeof(float) * T,
+                                  hipMemcpyHostToDevice, c.stream));
+    };
     d.fields.ensure(sizeof(float) * T * np * (d.stream ? 2 : 1));
     d.xbits.ensure(sizeof(uint32_t) * T * npw);
     d.bestbits.ensure(sizeof(uint32_t) * T * npw);
@@ -189,9 +238,12 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
     };
     OS_HIP_CHECK(hipEventRecord(t0, st));
     OS_HIP_CHECK(hipMemsetAsync(d.status.ptr, 0, sizeof(int), st));
-    if (S > 0)
+    if (S > 0 && !xc)
       OS_HIP_CHECK(hipMemcpyAsync(d.tscale.ptr, p->tscale, sizeof(float) * S,
                                   hipMemcpyHostToDevice, st));
+    if (xc) upload_tsr();
+    std::vector<double> e_now(xc ? T : 0);
+    int round = 0;
     // initial states
     if (p->init_states) {
       d.bytes.ensure(T * n);
@@ -310,7 +362,8 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
         if (steps > 0) cur = (cur + (steps - 1) / m + 1) & 1;
       } else {
         os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
-                                d.tscale.as<float>() + s0, s0, ns, p->seed, p->instance, Rp,
+                                d.tscale.as<float>() + s0, xc ? d.tsr.as<float>() : nullptr, s0,
+                                ns, p->seed, p->instance, Rp,
                                 d.fields.as<float>(), d.xbits.as<uint32_t>(),
                                 d.bestbits.as<uint32_t>(), d.energy.as<double>(),
                                 d.ebest.as<double>(), d.status.as<int>());
@@ -318,7 +371,40 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       }
       OS_HIP_CHECK(hipEventRecord(b, st));
       // FP64 refresh of fields and running energies; records the fp32 drift
-      energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), fbuf(cur));
+      if (!xc || (s0 + ns) % refresh == 0 || s0 + ns >= S)
+        energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), fbuf(cur));
+      if (xc && s0 + ns < S) {
+        // exchange round on the running energies of the launch (or of the
+        // refresh), then the scales of the next segment
+        OS_HIP_CHECK(hipMemcpyAsync(e_now.data(), d.energy.ptr, sizeof(double) * T,
+                                    hipMemcpyDeviceToHost, st));
+        c.sync();
+        const uint32_t k0 = (uint32_t)p->seed, k1 = (uint32_t)(p->seed >> 32);
+        for (int s = 0; s < I; ++s) {
+          const uint32_t tag = ((p->instance + (uint32_t)s) << 8) | os::TAG_EXCHANGE;
+          for (int l = 0; l < L; ++l) {
+            uint32_t *ra = rep_at.data() + (size_t)s * R + (size_t)l * K;
+            for (int k = round & 1; k + 1 < K; k += 2) {
+              const uint32_t a = ra[k], b = ra[k + 1];  // replicas of the instance
+              const double ea = e_now[(size_t)s * Rp + a];
+              const double eb = e_now[(size_t)s * Rp + b];
+              const double delta = (1.0 / (double)xc->tscale[k] - 1.0 / (double)xc->tscale[k + 1]) *
+                                   (ea - eb);
+              const uint4 o = os::philox4x32_10(
+                  make_uint4((uint32_t)k, (uint32_t)l, (uint32_t)round, tag), k0, k1);
+              const double u = (double)(o.x >> 8) * 0x1p-24;
+              if (delta >= 0.0 || u < std::exp(delta)) {
+                ra[k] = b;
+                ra[k + 1] = a;
+                slot_of[(size_t)s * R + a] = (uint32_t)(k + 1);
+                slot_of[(size_t)s * R + b] = (uint32_t)k;
+              }
+            }
+          }
+        }
+        ++round;
+        upload_tsr();
+      }
     }
     // FP64 energies of the best states (the returned energies)
     energies(d.bestbits.as<uint32_t>(), d.ebest.as<double>(), nullptr);
@@ -386,7 +472,42 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
       stats->pass_avg_ms = timed > 0 ? pms / timed : 0.0;
       stats->pass_launches = passes;
     }
+    if (xc && xc->final_slots) std::memcpy(xc->final_slots, slot_of.data(), sizeof(uint32_t) * I * R);
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energies,
+                    uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
+                    double *best_energy, os_sweep_stats *stats) {
+  return dense_run(d_, p, nullptr, best_energies, best_states, final_states, best_state,
+                   best_energy, stats);
+}
+
+int os_dense_tempering(os_dense *d_, const os_tempering_params *t, double *best_energies,
+                       uint8_t *best_states, uint8_t *final_states, uint32_t *final_slots,
+                       uint8_t *best_state, double *best_energy, os_sweep_stats *stats) {
+  if (!t) return dguard([&] { OS_REQUIRE(false, "null argument"); });
+  if (t->num_ladders < 1 || t->num_temperatures < 2 ||
+      (uint64_t)t->num_ladders * t->num_temperatures > (1u << 24))
+    return dguard([&] { OS_REQUIRE(false, "invalid ladder shape"); });
+  os_sweep_params p{};
+  p.num_replicas = t->num_ladders * t->num_temperatures;
+  p.num_sweeps = t->num_sweeps;
+  p.tscale = nullptr;
+  p.seed = t->seed;
+  p.instance = t->instance;
+  p.refresh_interval = t->refresh_interval;
+  Exchange xc;
+  xc.K = (int)t->num_temperatures;
+  xc.tscale = t->tscale;
+  xc.interval = t->exchange_interval;
+  xc.final_slots = final_slots;
+  return dense_run(d_, &p, &xc, best_energies, best_states, final_states, best_state, best_energy,
+                   stats);
 }
 
 }  // extern "C"

@@ -14,6 +14,8 @@
 //   sweep-mode init bit j of replica r     : (j>>7, r, 0, TAG_SWEEP_INIT)
 //   sweep-mode threshold of spin j, sweep t: (j>>2, r, t, TAG_SWEEP_THR) word j&3
 //   warm-start draw of spin j, replica r   : (j>>2, r, 0, TAG_WARM)      word j&3
+//   replica exchange, slots (k, k+1) of ladder l in round t:
+//                                            (k, l, t, TAG_EXCHANGE)     word 0
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -35,6 +37,7 @@ enum : uint32_t {
   TAG_SWEEP_INIT = 0x10,
   TAG_SWEEP_THR = 0x11,
   TAG_WARM = 0x12,
+  TAG_EXCHANGE = 0x13,
 };
 
 __host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {

@@ -20,6 +20,7 @@ struct Dense {
   bool stream = false;
   DevBuf q64, qpack, qdb;
   DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
+  DevBuf tsr;  // replica exchange: per-row temperature scales
   // stream path: flip masks of the last 3 * group blocks (ring), and the
   // decide launches on `side`, beside the field passes on the context's
   // stream; ev_dec marks the last decide of a group, ev_pass[G % 2] pass G
@@ -33,7 +34,7 @@ struct Dense {
   std::vector<hipEvent_t> pass_ev;  // timing pairs around sampled passes
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
-                    &scratch, &drift, &bytes, &marg, &status, &dblk})
+                    &scratch, &drift, &bytes, &marg, &status, &dblk, &tsr})
       b->release();
     for (hipEvent_t e : {ev_seg, ev_dec, ev_pass[0], ev_pass[1]})
       if (e) (void)hipEventDestroy(e);

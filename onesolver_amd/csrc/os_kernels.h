@@ -45,9 +45,11 @@ int dense_padded_n(int n);
 // num_instances instances of n spins (qpack / qdb stacked per instance), R_pad
 // replicas each (instance i at rows [i*R_pad, (i+1)*R_pad), stream id
 // instance + i)
+// tsr: optional per-row temperature scales (replica exchange), else tscale
+// per sweep
 void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const uint4 *qpack,
-                         const float *qdb, const float *tscale, int sweep0, int nsweeps,
-                         uint64_t seed, uint32_t instance, int R_pad, float *fields,
+                         const float *qdb, const float *tscale, const float *tsr, int sweep0,
+                         int nsweeps, uint64_t seed, uint32_t instance, int R_pad, float *fields,
                          uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
                          int *status);
 // start states of num_instances instances (R rows each, stream ids instance + i;

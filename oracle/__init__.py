@@ -156,6 +156,28 @@ def sweep_dense(flat, n, tscale, num_replicas, seed=1, instance=0, init=None, th
     return fin, bs, be
 
 
+def tempering_dense(flat, n, tscale, ladders, num_sweeps, interval, seed=1, instance=0,
+                    threads=0):
+    """(final states, best states, best energies, final slots) of
+    or_tempering_dense; replicas = ladders * len(tscale)."""
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    R = ladders * len(ts)
+    fin = np.zeros((R, n), dtype=np.uint8)
+    bs = np.zeros((R, n), dtype=np.uint8)
+    be = np.zeros(R, dtype=np.float64)
+    sl = np.zeros(R, dtype=np.uint32)
+    L = lib()
+    L.or_tempering_dense.argtypes = [ctypes.c_int, _f64p, _f32p, ctypes.c_int, ctypes.c_uint32,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                     _u8p, _u8p, _f64p, ctypes.POINTER(ctypes.c_uint32),
+                                     ctypes.c_int]
+    L.or_tempering_dense(n, _p(flat, _f64p), _p(ts, _f32p), len(ts), ladders, num_sweeps,
+                         interval, seed, instance, _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p),
+                         sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), threads)
+    return fin, bs, be, sl
+
+
 def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0, init=None,
               threads=0):
     rp = np.ascontiguousarray(row_ptr, dtype=np.int32)

@@ -64,6 +64,7 @@ enum {
   TAG_SWEEP_INIT = 0x10,
   TAG_SWEEP_THR = 0x11,
   TAG_WARM = 0x12,
+  TAG_EXCHANGE = 0x13,
 };
 
 static void seed_key(uint64_t seed, uint32_t key[2]) {
@@ -542,6 +543,99 @@ void or_sweep_dense(int n, const double *flat, const float *tscale, int num_swee
     free(h);
     free(x);
   }
+}
+
+/* Replica exchange over sweep-mode replicas (os_dense_tempering, no reference
+ * counterpart).  Replica r = l*K + k of ladder l starts in slot k; it sweeps
+ * with the thresholds of or_sweep_dense at scale tscale[slot].  After every
+ * `interval` sweeps (but not after the last), round t pairs slots (k, k+1),
+ * k = t%2, t%2+2, ...: with E the running energies, the replicas swap slots
+ * iff d = (1/ts_k - 1/ts_{k+1}) (E_k - E_{k+1}) >= 0 or u < exp(d), u = (w>>8)
+ * 2^-24 from word 0 of Philox counter (k, l, t, (instance << 8) | 0x13).      */
+void or_tempering_dense(int n, const double *flat, const float *tscale, int K,
+                        uint32_t ladders, int num_sweeps, int interval, uint64_t seed,
+                        uint32_t instance, uint8_t *final_states, uint8_t *best_states,
+                        double *best_energies, uint32_t *final_slots, int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  const uint32_t R = ladders * (uint32_t)K;
+  uint8_t *xs = (uint8_t *)malloc((size_t)R * n);
+  double *hs = (double *)malloc(sizeof(double) * (size_t)R * n);
+  double *es = (double *)malloc(sizeof(double) * R);
+  uint32_t *slot = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  uint32_t *at = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  for (uint32_t r = 0; r < R; ++r) {
+    uint8_t *x = xs + (size_t)r * n;
+    double *h = hs + (size_t)r * n;
+    sweep_init_bits(n, r, instance, key, x);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && x[l]) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    es[r] = or_energy(flat, x, n);
+    best_energies[r] = es[r];
+    memcpy(best_states + (size_t)r * n, x, (size_t)n);
+    slot[r] = r % (uint32_t)K;
+    at[r] = r;
+  }
+  uint32_t round = 0;
+  for (int s0 = 0; s0 < num_sweeps; s0 += interval) {
+    const int ns = num_sweeps - s0 < interval ? num_sweeps - s0 : interval;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int64_t rr = 0; rr < (int64_t)R; ++rr) {
+      const uint32_t r = (uint32_t)rr;
+      uint8_t *x = xs + (size_t)r * n;
+      double *h = hs + (size_t)r * n;
+      const float ts = tscale[slot[r]];
+      uint32_t w[4];
+      for (int t = s0; t < s0 + ns; ++t) {
+        for (int j = 0; j < n; ++j) {
+          if ((j & 3) == 0) sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
+          float thr = or_neg_ln_u(w[j & 3]) * ts;
+          double de = x[j] ? -h[j] : h[j];
+          if (de < (double)thr) {
+            double d = x[j] ? -1.0 : 1.0;
+            x[j] = (uint8_t)(1 - x[j]);
+            for (int l = 0; l < n; ++l)
+              if (l != j) h[l] += flat[(size_t)l * n + j] * d;
+            es[r] += de;
+          }
+        }
+        if (es[r] < best_energies[r]) {
+          best_energies[r] = es[r];
+          memcpy(best_states + (size_t)r * n, x, (size_t)n);
+        }
+      }
+    }
+    if (s0 + ns >= num_sweeps) break;
+    for (uint32_t l = 0; l < ladders; ++l) {
+      uint32_t *ra = at + (size_t)l * K;
+      for (int k = (int)(round & 1u); k + 1 < K; k += 2) {
+        const uint32_t a = ra[k], b = ra[k + 1];
+        const double d = (1.0 / (double)tscale[k] - 1.0 / (double)tscale[k + 1]) * (es[a] - es[b]);
+        uint32_t ctr[4] = {(uint32_t)k, l, round, (instance << 8) | TAG_EXCHANGE};
+        uint32_t o[4];
+        or_philox4x32_10(ctr, key, o);
+        const double u = (double)(o[0] >> 8) * 0x1p-24;
+        if (d >= 0.0 || u < exp(d)) {
+          ra[k] = b;
+          ra[k + 1] = a;
+          slot[a] = (uint32_t)(k + 1);
+          slot[b] = (uint32_t)k;
+        }
+      }
+    }
+    ++round;
+  }
+  if (final_states) memcpy(final_states, xs, (size_t)R * n);
+  if (final_slots) memcpy(final_slots, slot, sizeof(uint32_t) * R);
+  free(xs);
+  free(hs);
+  free(es);
+  free(slot);
+  free(at);
 }
 
 /* Energy of a CSR instance in the fixed order used by the GPU kernels:

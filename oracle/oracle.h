@@ -93,6 +93,13 @@ void or_sweep_dense(int n, const double *flat, const float *tscale, int num_swee
                     uint32_t num_replicas, uint64_t seed, uint32_t instance,
                     const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
                     double *best_energies, int threads);
+/* Replica exchange (os_dense_tempering): num_ladders * K replicas, replica
+ * l*K + k starting in slot k, sweeping at tscale[slot]; exchange rounds of
+ * adjacent slots after every `interval` sweeps (not after the last).       */
+void or_tempering_dense(int n, const double *flat, const float *tscale, int K,
+                        uint32_t ladders, int num_sweeps, int interval, uint64_t seed,
+                        uint32_t instance, uint8_t *final_states, uint8_t *best_states,
+                        double *best_energies, uint32_t *final_slots, int threads);
 /* CSR over the off-diagonal couplings (symmetric), fp32 values, fp32 diag. */
 void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
                   const float *diag, const float *tscale, int num_sweeps,

@@ -316,3 +316,64 @@ def test_warm_init_follows_marginals():
     m = st.mean(axis=0)
     assert m[0] == 0.0 and m[1] == 1.0
     assert abs(m[2] - 0.25) < 0.015 and abs(m[3] - 0.9) < 0.01
+
+
+def test_tempering_with_equal_temperatures_is_plain_sweeps():
+    rng = np.random.default_rng(4)
+    n = 40
+    q = rng.integers(-3, 4, (n, n)).astype(np.float64)
+    q = np.triu(q) + np.triu(q, 1).T
+    K, L, S = 4, 8, 24
+    fin, bs, be, sl = oracle.tempering_dense(q, n, np.full(K, 1.5, np.float32), L, S, 4, seed=3,
+                                             instance=5)
+    fin2, bs2, be2 = oracle.sweep_dense(q, n, np.full(S, 1.5, np.float32), L * K, seed=3,
+                                        instance=5)
+    assert (fin == fin2).all() and (bs == bs2).all() and (be == be2).all()
+    for l in range(L):
+        assert sorted(sl[l * K:(l + 1) * K]) == list(range(K))
+
+
+def test_tempering_exchange_decisions():
+    # one exchange round after one sweep between a hot (beta 0.2) and a cold
+    # (beta 1) slot: with d = (beta_0 - beta_1)(E_0 - E_1), a pair must swap
+    # when d >= 0, keeps its slots when d < -15, and in between swaps with
+    # probability exp(d)
+    rng = np.random.default_rng(8)
+    n = 24
+    q = rng.integers(-4, 5, (n, n)).astype(np.float64)
+    q = np.triu(q) + np.triu(q, 1).T
+    ts = np.array([5.0, 1.0], np.float32)
+    L, iv = 512, 1
+    _, _, _, sl = oracle.tempering_dense(q, n, ts, L, 2 * iv, iv, seed=11, instance=2)
+    # the states at the exchange: plain sweeps of each slot's replicas
+    hot, _, _ = oracle.sweep_dense(q, n, np.full(iv, ts[0]), 2 * L, seed=11, instance=2)
+    cold, _, _ = oracle.sweep_dense(q, n, np.full(iv, ts[1]), 2 * L, seed=11, instance=2)
+    e_hot = oracle.energy(q, hot[0::2])
+    e_cold = oracle.energy(q, cold[1::2])
+    d = (1.0 / ts[0] - 1.0 / ts[1]) * (e_hot - e_cold)
+    swapped = sl[0::2] == 1
+    assert (swapped[d >= 0]).all() and (d >= 0).sum() > 50
+    assert not (swapped[d < -15]).any() and (d < -15).sum() > 50
+    mid = (d < 0) & (d >= -15)
+    p = np.exp(d[mid])
+    assert abs(swapped[mid].sum() - p.sum()) < 4 * np.sqrt((p * (1 - p)).sum()) + 1
+
+
+def test_tempering_keeps_boltzmann_marginals():
+    # replica exchange leaves each slot's states distributed as
+    # exp(-beta_k E(x)) at stationarity
+    q = np.array([[0.3, -1.0, 0.5, 0.0], [-1.0, -0.2, 0.7, 0.4], [0.5, 0.7, 0.1, -0.6],
+                  [0.0, 0.4, -0.6, 0.2]])
+    betas = np.array([0.4, 1.3])
+    ts = (1.0 / betas).astype(np.float32)
+    L = 60000
+    fin, _, _, sl = oracle.tempering_dense(q, 4, ts, L, 30, 1, seed=21)
+    states = ((np.arange(16)[:, None] >> np.arange(4)) & 1).astype(np.uint8)
+    e = oracle.energy(q.astype(np.float32).astype(np.float64), states)
+    for k, beta in enumerate(betas):
+        idx = fin[sl == k] @ (1 << np.arange(4))
+        counts = np.bincount(idx, minlength=16)
+        p = np.exp(-beta * (e - e.min()))
+        p /= p.sum()
+        chi2 = ((counts - L * p) ** 2 / (L * p)).sum()
+        assert chi2 < 45.0, (k, chi2)  # 15 dof: p ~ 1e-4
