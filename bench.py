@@ -7,14 +7,15 @@ bf16 hi+lo MFMA delayed field updates with block k=16, FP64 field refresh
 every 512 sweeps.  One step = one full anneal of that batch with Q already
 resident in HBM.  Prints one JSON line (see the keys below).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|exh|copy]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c3|c4|c5|exh|pt|copy]
 
 Other workloads: c1 dense N=28 in reference semantics (1000 iterations x
 1024 tries); c3 a batch of 512 sparse instances; c4 the learned warm start
 (time to 99% of the best known energy); c5 dense uniform N=8192 with the
 fields in HBM; exh the exhaustive search (all 2^n states, dense U[-1,1] upper
-triangle, n=32 by default); copy the 1 GiB device copy that calibrates the PMC
-traffic.
+triangle, n=32 by default); pt replica exchange on the config-2 instance
+(256 ladders x 64 temperatures, plain annealing at the same budget beside it);
+copy the 1 GiB device copy that calibrates the PMC traffic.
 """
 import argparse
 import json
@@ -413,6 +414,64 @@ def run_copy_calibration(args):
             "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes}
 
 
+def run_tempering(args):
+    """Replica exchange on the config-2 instance (SK N=1024, seed 0): 256
+    ladders of 64 temperature slots (beta 0.1 .. 3.0 geometric) = 16384
+    replicas, 4096 sweeps, an exchange round every 16 sweeps, FP64 refresh
+    every 512.  Plain annealing with the config-2 schedule and budget runs in
+    the same call for comparison (its best energy and time)."""
+    import onesolver_amd
+    from onesolver_amd import instances
+
+    n, S, K, L, iv = 1024, 4096, 64, 256, args.pt_interval
+    q32, const = instances.sk_qubo(n, seed=0, instance=0)
+    q = q32.astype(np.float64)
+    ts_slots = onesolver_amd.tscale_from_beta(np.geomspace(0.1, 3.0, K))
+    ctx = onesolver_amd.Context(0)
+    prob = onesolver_amd.DenseProblem(ctx, q)
+    kw = dict(seed=1234, refresh_interval=512, best_states=False)
+    for _ in range(args.warmup):
+        prob.tempering(L, ts_slots, S, iv, **kw)
+    t0 = time.perf_counter()
+    best, kernel_ms, launches = np.inf, 0.0, 0
+    for i in range(args.steps):
+        out = prob.tempering(L, ts_slots, S, iv, **kw)
+        best = min(best, out["best_energy"])
+        kernel_ms += out["stats"]["kernel_ms"]
+        launches += out["stats"]["kernel_launches"]
+        log(f"step {i}: tempering best {out['best_energy']:.4f}")
+    elapsed = time.perf_counter() - t0
+    # plain annealing, same instance, replicas and sweeps (config 2)
+    ts_sa = onesolver_amd.tscale_from_beta(np.geomspace(0.1, 3.0, S))
+    t1 = time.perf_counter()
+    sa = prob.anneal(K * L, ts_sa, **kw)
+    sa_s = time.perf_counter() - t1
+    prob.close()
+    ctx.close()
+    attempts = float(K * L) * S * n * args.steps
+    avg_launch_s = kernel_ms / launches / 1e3
+    flops_per_launch = 4.0 * n * (K * L * (S / (launches / args.steps)) * n)
+    achieved = flops_per_launch / avg_launch_s / 1e12
+    return {
+        "metric": "spin-flip attempts/s", "value": attempts / elapsed, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 hi+lo MFMA, fp32 fields, fp64 energies",
+        "data": "synthetic (Philox-generated SK instance, seed 0)",
+        "config": {"workload": f"replica exchange, dense SK QUBO N=1024, {L} ladders x {K} "
+                               f"slots (beta 0.1..3.0), {S} sweeps, exchange every {iv} sweeps",
+                   "n": n, "ladders": L, "slots": K, "sweeps": S, "exchange_interval": iv},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "dense_sweep_kernel<8>", "avg_launch_ms": avg_launch_s * 1e3,
+                     "algorithmic_flops_per_launch": flops_per_launch},
+        "best_energy": best, "best_energy_ising": best + const,
+        "annealing_same_budget": {"best_energy": sa["best_energy"],
+                                  "best_energy_ising": sa["best_energy"] + const,
+                                  "ms": sa_s * 1e3},
+    }
+
+
 def run_exhaustive(args):
     """All 2^n states of a dense U[-1,1] instance (os_exhaustive), n=32 by
     default.  One step = one search, Q uploaded by the call (0.01 ms)."""
@@ -476,7 +535,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "copy"])
+                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "pt", "copy"])
+    ap.add_argument("--pt-interval", type=int, default=16)
     ap.add_argument("--exh-n", type=int, default=32)
     ap.add_argument("--c5-replicas", type=int, default=4096)
     ap.add_argument("--c5-sweeps", type=int, default=1024)
@@ -510,6 +570,9 @@ def main():
         return
     if args.workload == "exh":
         print(json.dumps(run_exhaustive(args)))
+        return
+    if args.workload == "pt":
+        print(json.dumps(run_tempering(args)))
         return
     if args.workload == "c5":
         res, _ = run_dense_c5(args)
