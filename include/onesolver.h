@@ -260,6 +260,48 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
 int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
                               int32_t *prefetch);
 
+/* ---- .qubo instances: text and binary ingest on the GPU ----
+ *
+ * An os_qubo holds a parsed model on the GPU: its distinct terms (i <= j, the
+ * first value of a repeated key, in order of first appearance), n = largest
+ * index + 1, and the header's max_nodes. */
+typedef struct os_qubo os_qubo;
+
+/* Parse `.qubo` text[0, len) -- replaces qubo::QUBOModel::load / parse_qubo
+ * (include/model/qubo.hpp:392-430) and its QUBOBuilder checks (:282-379).
+ * Accepts and rejects exactly what that grammar does; on rejection returns
+ * OS_ERR_INVALID and os_last_error() is the reference's exception message
+ * ("Parsing failed. Incorrect file format.", "An empty input, no coefficients
+ * defined.", ...).  Numbers convert as strtod does (correctly rounded). */
+int os_qubo_parse(os_ctx *ctx, const char *text, size_t len, os_qubo **out);
+/* Load the binary form written by os_qubo_write_binary (magic "OSQB",
+ * version 1, little-endian: int32 n, int32 max_nodes, int64 num_linear,
+ * int64 num_quadratic, then int32 i[T], int32 j[T], float64 v[T] with
+ * T = num_linear + num_quadratic).  Rejects terms outside 0 <= i <= j < n,
+ * repeated keys, counts that do not match, and n != largest index + 1.  No
+ * reference counterpart (the reference reads text only). */
+int os_qubo_read_binary(os_ctx *ctx, const void *data, size_t len, os_qubo **out);
+int os_qubo_destroy(os_qubo *q);
+/* Each output may be NULL.  n: QUBOModel::get_nodes (qubo.hpp:424 of the
+ * builder: largest index + 1); num_linear / num_quadratic: the distinct
+ * terms. */
+int os_qubo_info(os_qubo *q, int32_t *n, int32_t *max_nodes, int64_t *num_linear,
+                 int64_t *num_quadratic);
+/* GPU time of the last text parse in ms: ms[0] line index, ms[1] line
+ * parse, ms[2] dedupe and compaction. */
+int os_qubo_parse_times(os_qubo *q, float *ms);
+/* The terms (each output may be NULL; num_linear + num_quadratic entries). */
+int os_qubo_terms(os_qubo *q, int32_t *i, int32_t *j, double *v);
+/* helpers::flatten_qubo (include/helpers/qubo_helpers.hpp:26-44): the n x n
+ * FP64 symmetric matrix into q_sym. */
+int os_qubo_flatten(os_qubo *q, double *q_sym);
+/* A dense sweep-mode problem built on the GPU from the terms (the flattened
+ * matrix never passes through the host); n <= 16384. */
+int os_qubo_dense(os_qubo *q, os_dense **out);
+/* Bytes of the binary form (returns the size, not a status; -1 for NULL). */
+int64_t os_qubo_binary_size(os_qubo *q);
+int os_qubo_write_binary(os_qubo *q, void *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,10 +162,11 @@ struct QUBOBuilder {
 
 namespace detail {
 
-template <typename It>
+// Builder: QUBOBuilder, or any type with its four set/add member functions.
+template <typename It, typename Builder = QUBOBuilder>
 class QuboReader {
  public:
-  QuboReader(It first, It last, QUBOBuilder &b) : p_(first), end_(last), b_(b) {}
+  QuboReader(It first, It last, Builder &b) : p_(first), end_(last), b_(b) {}
 
   // true iff the whole input matched the grammar
   bool run() {
@@ -180,7 +181,7 @@ class QuboReader {
 
  private:
   It p_, end_;
-  QUBOBuilder &b_;
+  Builder &b_;
 
   static bool blank(char c) { return c == ' ' || c == '\t'; }
   void skip_blanks() {

@@ -36,6 +36,7 @@ class OneSolverError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
         self.code = code
+        self.message = msg
 
 
 class InvalidArgument(OneSolverError, ValueError):
@@ -356,6 +357,114 @@ class DenseBatch:
                                      ctypes.byref(st)))
         return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
                 "best_energy": e, "stats": st.as_dict()}
+
+
+def _qubo_lib():
+    L = lib()
+    if not getattr(L, "_qubo_ready", False):
+        i64 = ctypes.c_int64
+        L.os_qubo_parse.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.os_qubo_read_binary.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t,
+                                          ctypes.POINTER(_vp)]
+        L.os_qubo_destroy.argtypes = [_vp]
+        L.os_qubo_info.argtypes = [_vp, _i32p, _i32p, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.os_qubo_parse_times.argtypes = [_vp, _f32p]
+        L.os_qubo_terms.argtypes = [_vp, _i32p, _i32p, _f64p]
+        L.os_qubo_flatten.argtypes = [_vp, _f64p]
+        L.os_qubo_dense.argtypes = [_vp, ctypes.POINTER(_vp)]
+        L.os_qubo_binary_size.argtypes = [_vp]
+        L.os_qubo_binary_size.restype = i64
+        L.os_qubo_write_binary.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t]
+        L._qubo_ready = True
+    return L
+
+
+class Qubo:
+    """A `.qubo` model parsed on the GPU (os_qubo): the grammar and checks of
+    qubo::QUBOModel::load (include/model/qubo.hpp:282-430).  Rejected input
+    raises InvalidArgument carrying the reference's exception message."""
+
+    def __init__(self, ctx: Context, text=None, *, binary=None):
+        if (text is None) == (binary is None):
+            raise ValueError("give text or binary")
+        L = _qubo_lib()
+        h = _vp()
+        if text is not None:
+            data = text.encode() if isinstance(text, str) else bytes(text)
+            _check(L.os_qubo_parse(ctx.handle, data, len(data), ctypes.byref(h)))
+        else:
+            data = bytes(binary)
+            _check(L.os_qubo_read_binary(ctx.handle, data, len(data), ctypes.byref(h)))
+        self._h = h
+        self.ctx = ctx
+        n, mx = ctypes.c_int32(), ctypes.c_int32()
+        nl, nq = ctypes.c_int64(), ctypes.c_int64()
+        _check(L.os_qubo_info(h, ctypes.byref(n), ctypes.byref(mx), ctypes.byref(nl),
+                              ctypes.byref(nq)))
+        self.n, self.max_nodes = n.value, mx.value
+        self.num_linear, self.num_quadratic = nl.value, nq.value
+
+    @classmethod
+    def load(cls, ctx: Context, path):
+        """Text or binary file, told apart by the binary magic "OSQB"."""
+        with open(path, "rb") as f:
+            data = f.read()
+        if data[:4] == b"OSQB":
+            return cls(ctx, binary=data)
+        return cls(ctx, data)
+
+    def close(self):
+        if self._h:
+            _qubo_lib().os_qubo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def parse_times(self):
+        """GPU ms of the last parse: line index, line parse, dedupe + compaction."""
+        ms = np.zeros(3, np.float32)
+        _check(_qubo_lib().os_qubo_parse_times(self._h, _p(ms, _f32p)))
+        return ms
+
+    def terms(self):
+        """(i, j, v): the distinct terms in order of first appearance, i <= j."""
+        T = self.num_linear + self.num_quadratic
+        i = np.empty(T, np.int32)
+        j = np.empty(T, np.int32)
+        v = np.empty(T, np.float64)
+        _check(_qubo_lib().os_qubo_terms(self._h, _p(i, _i32p), _p(j, _i32p), _p(v, _f64p)))
+        return i, j, v
+
+    def flatten(self):
+        """helpers::flatten_qubo: the n x n FP64 symmetric matrix."""
+        q = np.empty((self.n, self.n), np.float64)
+        _check(_qubo_lib().os_qubo_flatten(self._h, _p(q, _f64p)))
+        return q
+
+    def to_binary(self) -> bytes:
+        L = _qubo_lib()
+        size = L.os_qubo_binary_size(self._h)
+        buf = ctypes.create_string_buffer(size)
+        _check(L.os_qubo_write_binary(self._h, buf, size))
+        return buf.raw
+
+    def dense(self) -> "DenseProblem":
+        """A DenseProblem built on the GPU from the terms (os_qubo_dense)."""
+        h = _vp()
+        _check(_qubo_lib().os_qubo_dense(self._h, ctypes.byref(h)))
+        d = DenseProblem.__new__(DenseProblem)
+        d.n, d.ctx, d._h = self.n, self.ctx, h
+        return d
 
 
 class CsrParams(ctypes.Structure):

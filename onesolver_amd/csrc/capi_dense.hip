@@ -127,6 +127,21 @@ int os_dense_copy_q(os_dense *d_, double *q_out) {
   });
 }
 
+}  // extern "C"
+
+os::Dense *os::dense_create_device(Ctx &c, int n, void (*fill)(void *arg, double *q64),
+                                   void *arg) {
+  c.activate();
+  std::unique_ptr<Dense> d(dense_alloc(c, 1, n));
+  OS_HIP_CHECK(hipMemsetAsync(d->q64.ptr, 0, sizeof(double) * (size_t)n * n, c.stream));
+  fill(arg, d->q64.as<double>());
+  pack(*d);
+  c.sync();
+  return d.release();
+}
+
+extern "C" {
+
 int os_dense_create(os_ctx *ctx_, int n, const double *q_sym, os_dense **out) {
   return os_dense_batch_create(ctx_, 1, n, q_sym, out);
 }

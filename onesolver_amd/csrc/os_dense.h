@@ -43,4 +43,9 @@ struct Dense {
   }
 };
 
+// A one-instance dense problem of size n whose FP64 matrix `fill` writes on
+// the context's stream (it receives the zeroed n x n device array); packs the
+// MFMA operands and waits for the stream.  Defined in capi_dense.hip.
+Dense *dense_create_device(Ctx &c, int n, void (*fill)(void *arg, double *q64), void *arg);
+
 }  // namespace os

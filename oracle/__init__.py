@@ -219,3 +219,40 @@ def csr_energy(row_ptr, col, val, diag, states):
     e = st2 @ np.asarray(diag, dtype=np.float64)
     e += (st2[:, rows[upper]] * st2[:, col[upper]]) @ np.asarray(val, dtype=np.float64)[upper]
     return e if st.ndim > 1 else float(e[0])
+
+
+QP_MESSAGES = {
+    1: "Parsing failed. Incorrect file format.",
+    2: "Incorrect file, encountered coefficient from lower triangle of QUBO matrix.",
+    3: "An empty input, no coefficients defined.",
+    4: "No header line or header line misformatted.",
+    5: "Number of linear terms is greater than num nodes.",
+    6: "Number of quadratic terms is not equal to the declared one.",
+    7: "Number of linear terms is not equal to the declared one.",
+}
+
+
+def qubo_parse(text: bytes):
+    """or_qubo_parse: returns (code, message, terms) where terms is None on
+    error and otherwise a dict with the distinct terms in order of first
+    appearance (i, j, v arrays), n, num_linear and num_quadratic."""
+    L = lib()
+    i64 = ctypes.c_int64
+    L.or_qubo_parse.argtypes = [ctypes.c_char_p, i64, i64, _i32p, _i32p, _f64p,
+                                ctypes.POINTER(i64), _i32p, ctypes.POINTER(i64),
+                                ctypes.POINTER(i64)]
+    text = bytes(text)
+    cap = text.count(b"\n") + text.count(b"\r") + 2
+    ti = np.zeros(cap, np.int32)
+    tj = np.zeros(cap, np.int32)
+    tv = np.zeros(cap, np.float64)
+    nt, nl, nq = i64(), i64(), i64()
+    n = ctypes.c_int32()
+    code = L.or_qubo_parse(text, len(text), cap, _p(ti, _i32p), _p(tj, _i32p), _p(tv, _f64p),
+                           ctypes.byref(nt), ctypes.byref(n), ctypes.byref(nl),
+                           ctypes.byref(nq))
+    if code:
+        return code, QP_MESSAGES[code], None
+    k = nt.value
+    return 0, None, {"i": ti[:k].copy(), "j": tj[:k].copy(), "v": tv[:k].copy(), "n": n.value,
+                     "num_linear": nl.value, "num_quadratic": nq.value}

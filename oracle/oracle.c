@@ -697,3 +697,234 @@ void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float
     free(x);
   }
 }
+
+/* ------------------------------------------------------------------------
+ * .qubo reader: parse_qubo + QUBOBuilder, include/model/qubo.hpp:282-417.
+ *
+ *   file        := comment* header? (comment | coefficient)*   (:406)
+ *   comment     := 'c' print* (eol | eoi)                       (:399)
+ *   header      := "p qubo" uint uint uint uint (eol | eoi)     (:400-404)
+ *   coefficient := index index real (eol | eoi)                 (:396-398)
+ *   index       := lexeme[uint ' ']                             (:395)
+ * with blanks (space, tab) skipped before every token (qi::blank, :407) and
+ * the whole input consumed (:410).  add_element runs as soon as the three
+ * numbers are read, before the end of the line is checked (:397), and
+ * throws for i > j; numbers are converted by strtod (correctly rounded).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  const char *p, *end;
+  /* builder state, qubo.hpp:284-289 */
+  int num_quadratic, num_linear, max_nodes;
+  int have_used, max_used;
+  int64_t cap, count, nlin, nquad;
+  int32_t *ti, *tj;
+  double *tv;
+  uint64_t *hkeys; /* open addressing over (i, j), first value kept */
+  uint64_t hmask;
+  int lower;       /* add_element threw */
+} QP;
+
+static int qp_blank(int c) { return c == ' ' || c == '\t'; }
+static int qp_at(const QP *s, const char *q) { return q < s->end ? (unsigned char)*q : -1; }
+static void qp_skip(QP *s) {
+  while (s->p < s->end && qp_blank((unsigned char)*s->p)) ++s->p;
+}
+static int qp_eol_or_eoi(QP *s) {
+  qp_skip(s);
+  int c = qp_at(s, s->p);
+  if (c < 0) return 1;
+  if (c == '\n') { ++s->p; return 1; }
+  if (c == '\r') {
+    ++s->p;
+    if (qp_at(s, s->p) == '\n') ++s->p;
+    return 1;
+  }
+  return 0;
+}
+/* qi::uint_: decimal digits, no sign, fails beyond 2^32 - 1 */
+static int qp_uint(QP *s, const char **q, unsigned *out) {
+  const char *t = *q;
+  int c = qp_at(s, t);
+  if (c < '0' || c > '9') return 0;
+  uint64_t v = 0;
+  while ((c = qp_at(s, t)) >= '0' && c <= '9') {
+    v = v * 10 + (uint64_t)(c - '0');
+    if (v > 0xFFFFFFFFull) return 0;
+    ++t;
+  }
+  *out = (unsigned)v;
+  *q = t;
+  return 1;
+}
+static int qp_lower(int c) { return c >= 'A' && c <= 'Z' ? c + 32 : c; }
+static int qp_word_ci(QP *s, const char **q, const char *w) {
+  const char *t = *q;
+  for (; *w; ++w, ++t)
+    if (qp_lower(qp_at(s, t)) != *w) return 0;
+  *q = t;
+  return 1;
+}
+/* qi::double_: sign? (nan ("(" ... ")")? | inf (inity)? | digits ["." digits]
+ * | "." digits) (e sign? digits)? -- the exponent only when it has digits. */
+static int qp_real(QP *s, double *out) {
+  const char *t = s->p, *start = s->p;
+  int neg = 0, c = qp_at(s, t);
+  if (c == '+' || c == '-') { neg = c == '-'; ++t; c = qp_at(s, t); }
+  if (c == 'n' || c == 'N') {
+    if (!qp_word_ci(s, &t, "nan")) return 0;
+    if (qp_at(s, t) == '(') {
+      const char *u = t + 1;
+      while (u < s->end && *u != ')') ++u;
+      if (u < s->end) t = u + 1;
+    }
+    *out = neg ? -(double)NAN : (double)NAN;
+    s->p = t;
+    return 1;
+  }
+  if (c == 'i' || c == 'I') {
+    if (!qp_word_ci(s, &t, "inf")) return 0;
+    qp_word_ci(s, &t, "inity");
+    *out = neg ? -(double)INFINITY : (double)INFINITY;
+    s->p = t;
+    return 1;
+  }
+  int digits = 0;
+  while ((c = qp_at(s, t)) >= '0' && c <= '9') { ++t; digits = 1; }
+  if (qp_at(s, t) == '.') {
+    const char *u = t + 1;
+    int frac = 0;
+    while ((c = qp_at(s, u)) >= '0' && c <= '9') { ++u; frac = 1; }
+    if (digits || frac) { t = u; digits = 1; }
+  }
+  if (!digits) return 0;
+  c = qp_at(s, t);
+  if (c == 'e' || c == 'E') {
+    const char *u = t + 1;
+    c = qp_at(s, u);
+    if (c == '+' || c == '-') ++u;
+    int ed = 0;
+    while ((c = qp_at(s, u)) >= '0' && c <= '9') { ++u; ed = 1; }
+    if (ed) t = u;
+  }
+  size_t L = (size_t)(t - start);
+  char buf[64], *tok = L < sizeof buf ? buf : (char *)malloc(L + 1);
+  memcpy(tok, start, L);
+  tok[L] = 0;
+  *out = strtod(tok, NULL);
+  if (tok != buf) free(tok);
+  s->p = t;
+  return 1;
+}
+static int qp_comment(QP *s) {
+  const char *save = s->p;
+  qp_skip(s);
+  if (qp_at(s, s->p) != 'c') { s->p = save; return 0; }
+  ++s->p;
+  for (;;) {
+    qp_skip(s);
+    int c = qp_at(s, s->p);
+    if (c >= 0x20 && c < 0x7F) ++s->p; else break;   /* qi::print */
+  }
+  if (!qp_eol_or_eoi(s)) { s->p = save; return 0; }
+  return 1;
+}
+static void qp_header(QP *s) {
+  const char *save = s->p;
+  qp_skip(s);
+  const char *q = s->p;
+  for (const char *w = "p qubo"; *w; ++w, ++q)
+    if (qp_at(s, q) != *w) { s->p = save; return; }
+  s->p = q;
+  for (int k = 0; k < 4; ++k) {
+    unsigned v;
+    qp_skip(s);
+    if (!qp_uint(s, &s->p, &v)) { s->p = save; return; }
+    if (k == 1) s->max_nodes = (int)v;      /* the actions run as each number is read */
+    if (k == 2) s->num_linear = (int)v;
+    if (k == 3) s->num_quadratic = (int)v;
+  }
+  if (!qp_eol_or_eoi(s)) s->p = save;
+}
+static uint64_t qp_hash(uint64_t k) { return (k * 0x9E3779B97F4A7C15ull) >> 17; }
+/* QUBOBuilder::add_element, qubo.hpp:304-318 */
+static int qp_add(QP *s, int i, int j, double v) {
+  if (i > j) { s->lower = 1; return 0; }
+  uint64_t key = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+  uint64_t h = qp_hash(key) & s->hmask;
+  for (;;) {
+    if (s->hkeys[h] == key + 1) break;   /* repeated key: the first value stays */
+    if (s->hkeys[h] == 0) {
+      if (s->count >= s->cap) return 0;
+      s->hkeys[h] = key + 1;
+      s->ti[s->count] = i;
+      s->tj[s->count] = j;
+      s->tv[s->count] = v;
+      ++s->count;
+      if (i == j) ++s->nlin; else ++s->nquad;
+      break;
+    }
+    h = (h + 1) & s->hmask;
+  }
+  if (!s->have_used || j > s->max_used) s->max_used = j;   /* std::set<int>::rbegin */
+  if (i > s->max_used) s->max_used = i;
+  s->have_used = 1;
+  return 1;
+}
+static int qp_index(QP *s, unsigned *out) {
+  qp_skip(s);
+  const char *q = s->p;
+  if (!qp_uint(s, &q, out)) return 0;
+  if (qp_at(s, q) != ' ') return 0;
+  s->p = q + 1;
+  return 1;
+}
+static int qp_coefficient(QP *s) {
+  const char *save = s->p;
+  unsigned i, j;
+  double v;
+  if (!qp_index(s, &i) || !qp_index(s, &j)) { s->p = save; return 0; }
+  qp_skip(s);
+  if (!qp_real(s, &v)) { s->p = save; return 0; }
+  if (!qp_add(s, (int)i, (int)j, v)) return 0;
+  if (!qp_eol_or_eoi(s)) { s->p = save; return 0; }
+  return 1;
+}
+
+int or_qubo_parse(const char *text, int64_t len, int64_t cap, int32_t *ti, int32_t *tj,
+                  double *tv, int64_t *nt, int32_t *n, int64_t *num_linear,
+                  int64_t *num_quadratic) {
+  QP s;
+  memset(&s, 0, sizeof s);
+  s.p = text;
+  s.end = text + len;
+  s.num_quadratic = s.num_linear = s.max_nodes = -1;
+  s.cap = cap;
+  s.ti = ti;
+  s.tj = tj;
+  s.tv = tv;
+  uint64_t hc = 16;
+  while (hc < 2 * (uint64_t)cap + 16) hc <<= 1;
+  s.hkeys = (uint64_t *)calloc(hc, sizeof(uint64_t));
+  s.hmask = hc - 1;
+  while (qp_comment(&s)) {
+  }
+  qp_header(&s);
+  while (qp_comment(&s) || qp_coefficient(&s)) {
+  }
+  free(s.hkeys);
+  if (s.lower) return OR_QP_LOWER;
+  qp_skip(&s);
+  if (s.p != s.end) return OR_QP_SYNTAX;
+  /* QUBOBuilder::build_qubo, qubo.hpp:353-377 */
+  if (s.nlin == 0 && s.nquad == 0) return OR_QP_EMPTY;
+  if (s.num_quadratic == -1 || s.num_linear == -1 || s.max_nodes == -1) return OR_QP_NO_HEADER;
+  if (s.num_linear > s.max_nodes) return OR_QP_LINEAR_GT_NODES;
+  /* size_t != int: the int converts to size_t (sign-extended) */
+  if ((uint64_t)s.nquad != (uint64_t)(int64_t)s.num_quadratic) return OR_QP_QUAD_COUNT;
+  if ((uint64_t)s.nlin != (uint64_t)(int64_t)s.num_linear) return OR_QP_LINEAR_COUNT;
+  *nt = s.count;
+  *n = s.max_used + 1;
+  *num_linear = s.nlin;
+  *num_quadratic = s.nquad;
+  return OR_QP_OK;
+}

@@ -34,6 +34,13 @@
  *                      visited-state distribution on N=4 at fixed beta
  *                      against the Boltzmann law.
  *   or_mpnn_*          finite differences of the FP64 loss.
+ *   or_qubo_parse      the accept/reject strings and expected models of the
+ *                      reference's tests/io_test.cpp:12-124 and of the product
+ *                      reader's tests; the example files' models (flattened
+ *                      and compared with the reference's
+ *                      tests/qubo_helpers_test.cpp:14-27 expectation and the
+ *                      exhaustive KAT energies); numbers against Python's
+ *                      float(), an independent correctly rounded conversion.
  */
 #ifndef ONESOLVER_ORACLE_H_
 #define ONESOLVER_ORACLE_H_
@@ -112,6 +119,27 @@ void or_warm_init(int n, const float *p, uint32_t num_replicas, uint64_t seed,
 /* Random initial bits of sweep mode (shared by dense and CSR kernels). */
 void or_sweep_init(int n, uint32_t num_replicas, uint64_t seed, uint32_t instance,
                    uint8_t *states);
+
+/* Outcomes of or_qubo_parse: 0 or one of the reference's std::invalid_argument
+ * throws (include/model/qubo.hpp:313-314, :355-373, :413-415). */
+enum {
+  OR_QP_OK = 0,
+  OR_QP_SYNTAX = 1,           /* "Parsing failed. Incorrect file format." */
+  OR_QP_LOWER = 2,            /* "... coefficient from lower triangle ..." */
+  OR_QP_EMPTY = 3,            /* "An empty input, no coefficients defined." */
+  OR_QP_NO_HEADER = 4,        /* "No header line or header line misformatted." */
+  OR_QP_LINEAR_GT_NODES = 5,  /* "Number of linear terms is greater than num nodes." */
+  OR_QP_QUAD_COUNT = 6,       /* "Number of quadratic terms is not equal ..." */
+  OR_QP_LINEAR_COUNT = 7      /* "Number of linear terms is not equal ..." */
+};
+
+/* parse_qubo + QUBOBuilder (include/model/qubo.hpp:282-417) over text[0, len):
+ * the distinct terms in order of first appearance (ti <= tj; the first value
+ * of a repeated key is kept), at most `cap` of them; *n = max index + 1.
+ * Returns an OR_QP_* code; the outputs are valid for OR_QP_OK only. */
+int or_qubo_parse(const char *text, int64_t len, int64_t cap, int32_t *ti, int32_t *tj,
+                  double *tv, int64_t *nt, int32_t *n, int64_t *num_linear,
+                  int64_t *num_quadratic);
 
 #ifdef __cplusplus
 }
