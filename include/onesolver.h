@@ -274,6 +274,10 @@ typedef struct os_qubo os_qubo;
  * ("Parsing failed. Incorrect file format.", "An empty input, no coefficients
  * defined.", ...).  Numbers convert as strtod does (correctly rounded). */
 int os_qubo_parse(os_ctx *ctx, const char *text, size_t len, os_qubo **out);
+/* The same for text already in GPU memory (`text`: a device pointer on the
+ * context's device, 16-byte aligned, len bytes readable).  No reference
+ * counterpart: the reference parses from a host stream only. */
+int os_qubo_parse_device(os_ctx *ctx, const void *text, size_t len, os_qubo **out);
 /* Load the binary form written by os_qubo_write_binary (magic "OSQB",
  * version 1, little-endian: int32 n, int32 max_nodes, int64 num_linear,
  * int64 num_quadratic, then int32 i[T], int32 j[T], float64 v[T] with

@@ -127,6 +127,7 @@ class Context:
     def __init__(self, device: int = 0):
         h = _vp()
         _check(lib().os_ctx_create(device, ctypes.byref(h)))
+        self.device = device
         self._h = h
 
     @property
@@ -364,6 +365,7 @@ def _qubo_lib():
     if not getattr(L, "_qubo_ready", False):
         i64 = ctypes.c_int64
         L.os_qubo_parse.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_vp)]
+        L.os_qubo_parse_device.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.POINTER(_vp)]
         L.os_qubo_read_binary.argtypes = [_vp, ctypes.c_char_p, ctypes.c_size_t,
                                           ctypes.POINTER(_vp)]
         L.os_qubo_destroy.argtypes = [_vp]
@@ -379,17 +381,60 @@ def _qubo_lib():
     return L
 
 
+class DeviceBytes:
+    """Bytes copied into GPU memory through the HIP runtime the library itself
+    loads (the device of `ctx`), e.g. `.qubo` text for Qubo(ctx, device=...)."""
+
+    def __init__(self, ctx: Context, data):
+        lib()
+        self._hip = ctypes.CDLL("libamdhip64.so.7")
+        data = bytes(data)
+        self.size = len(data)
+        self.ptr = _vp()
+        if self._hip.hipSetDevice(ctypes.c_int(ctx.device)) != 0:
+            raise OneSolverError(-2, "hipSetDevice failed")
+        if self._hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.size, 16))) != 0:
+            raise OneSolverError(-2, "hipMalloc failed")
+        if self.size and self._hip.hipMemcpy(self.ptr, data, ctypes.c_size_t(self.size),
+                                             ctypes.c_int(1)) != 0:
+            raise OneSolverError(-2, "hipMemcpy failed")
+
+    def close(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def source(self):
+        return (self.ptr.value, self.size)
+
+
 class Qubo:
     """A `.qubo` model parsed on the GPU (os_qubo): the grammar and checks of
     qubo::QUBOModel::load (include/model/qubo.hpp:282-430).  Rejected input
     raises InvalidArgument carrying the reference's exception message."""
 
-    def __init__(self, ctx: Context, text=None, *, binary=None):
-        if (text is None) == (binary is None):
-            raise ValueError("give text or binary")
+    def __init__(self, ctx: Context, text=None, *, binary=None, device=None):
+        if sum(x is not None for x in (text, binary, device)) != 1:
+            raise ValueError("give one of text, binary, device")
         L = _qubo_lib()
         h = _vp()
-        if text is not None:
+        if device is not None:
+            ptr, length = device   # text already in GPU memory (16-byte aligned)
+            _check(L.os_qubo_parse_device(ctx.handle, ptr, length, ctypes.byref(h)))
+        elif text is not None:
             data = text.encode() if isinstance(text, str) else bytes(text)
             _check(L.os_qubo_parse(ctx.handle, data, len(data), ctypes.byref(h)))
         else:

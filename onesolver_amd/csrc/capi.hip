@@ -59,6 +59,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   for (auto *b : {&b_q, &b_beta, &b_bits, &b_run, &b_energy, &b_scratch, &b_fields, &b_bytes})
     b->release();
+  for (DevBuf &b : ingest) b.release();
   if (ev_start) (void)hipEventDestroy(ev_start);
   if (ev_stop) (void)hipEventDestroy(ev_stop);
   if (stream) (void)hipStreamDestroy(stream);

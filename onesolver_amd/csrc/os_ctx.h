@@ -29,6 +29,8 @@ struct Ctx {
   hipEvent_t ev_stop = nullptr;
   // scratch buffers of the one-shot entry points
   DevBuf b_q, b_beta, b_bits, b_run, b_energy, b_scratch, b_fields, b_bytes;
+  // .qubo ingest work arrays (qubo_text.hip), kept between parses
+  DevBuf ingest[16];
 
   explicit Ctx(int dev);
   ~Ctx();

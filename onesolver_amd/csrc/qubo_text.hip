@@ -131,24 +131,41 @@ __device__ __forceinline__ uint32_t byte_at(const uint4 &w, int k) {
   return (d >> ((k & 3) * 8)) & 0xFFu;
 }
 
+__device__ __forceinline__ uint32_t byte_or_zero(const uint8_t *text, uint64_t len, uint64_t p) {
+  return p < len ? (uint32_t)text[p] : 0u;
+}
+
+// 16 bytes at 16-aligned `p` (text 16-aligned); bytes at or past len read 0
+__device__ __forceinline__ uint4 load16(const uint8_t *text, uint64_t len, uint64_t p) {
+  if (p + 16 <= len) return *reinterpret_cast<const uint4 *>(text + p);
+  uint32_t d[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t x = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) x |= byte_or_zero(text, len, p + 4 * k + t) << (8 * t);
+    d[k] = x;
+  }
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
 // Line ends among this thread's 32 bytes; `ends` receives their bit mask.
 __device__ __forceinline__ uint32_t thread_ends(const uint8_t *text, uint64_t len,
                                                 uint64_t base, uint32_t *ends) {
-  const uint4 *p = reinterpret_cast<const uint4 *>(text + base);
-  const uint4 a = p[0], b = p[1];
+  const uint4 a = load16(text, len, base), b = load16(text, len, base + 16);
+  const uint32_t after = byte_or_zero(text, len, base + 32);
   uint32_t m = 0;
 #pragma unroll
   for (int k = 0; k < 32; ++k) {
     const uint32_t c = k < 16 ? byte_at(a, k) : byte_at(b, k - 16);
     if (c == '\n') m |= 1u << k;
     if (c == '\r') {
-      // a '\r' ends a line unless a '\n' follows; the text is zero-padded
-      const uint32_t nx = k < 15 ? byte_at(a, k + 1) : k < 31 ? byte_at(b, k - 15)
-                                                             : (uint32_t)text[base + 32];
+      // a '\r' ends a line unless a '\n' follows
+      const uint32_t nx = k < 15 ? byte_at(a, k + 1) : k < 31 ? byte_at(b, k - 15) : after;
       if (nx != '\n') m |= 1u << k;
     }
   }
-  // bytes at or past len are padding (zero), never an end
+  // bytes at or past len read as zero, never an end
   *ends = m;
   return __popc(m);
 }
@@ -202,25 +219,34 @@ __global__ void __launch_bounds__(kThreads) line_starts_kernel(const uint8_t *te
 
 // ---------------------------------------------------------------- line parser
 
-// Bytes of one line through an aligned 16-byte window; -1 past the line.
+// The bytes of one line, at offsets p < e from its start s; -1 past the
+// line.  kLds: the line is in LDS at `lds`; otherwise it is read from the
+// global text through an aligned 16-byte window.
+template <bool kLds>
 struct Cursor {
-  const uint4 *text;
-  uint64_t p, e, wpos;
+  const uint8_t *text;  // global text (!kLds)
+  const uint8_t *lds;   // the line in LDS (kLds)
+  uint64_t len, s, wpos;
+  uint32_t p, e;
   uint4 w;
-  __device__ int peek() {
+  __device__ __forceinline__ int peek() {
     if (p >= e) return -1;
-    const uint64_t a = p >> 4;
-    if (a != wpos) {
-      w = text[a];
-      wpos = a;
+    if constexpr (kLds) {
+      return (int)lds[p];
+    } else {
+      const uint64_t at = s + p, a = at >> 4;
+      if (a != wpos) {
+        w = load16(text, len, a << 4);
+        wpos = a;
+      }
+      return (int)byte_at(w, (int)(at & 15));
     }
-    return (int)byte_at(w, (int)(p & 15));
   }
-  __device__ void skip_blanks() {
+  __device__ __forceinline__ void skip_blanks() {
     for (int c = peek(); c == ' ' || c == '\t'; c = peek()) ++p;
   }
   // qi::eol | qi::eoi (eoi only on the text's last line)
-  __device__ bool eol(bool last) {
+  __device__ __forceinline__ bool eol(bool last) {
     skip_blanks();
     const int c = peek();
     if (c < 0) return last;
@@ -229,7 +255,7 @@ struct Cursor {
     return false;
   }
   // qi::uint_: digits, at most 2^32 - 1
-  __device__ bool uint_(uint32_t *out) {
+  __device__ __forceinline__ bool uint_(uint32_t *out) {
     int c = peek();
     if (c < '0' || c > '9') return false;
     uint64_t v = 0;
@@ -242,8 +268,8 @@ struct Cursor {
     return true;
   }
   // case-insensitive word; consumed only when it matches whole
-  __device__ bool word_ci(const char *wd) {
-    const uint64_t save = p;
+  __device__ __forceinline__ bool word_ci(const char *wd) {
+    const uint32_t save = p;
     for (; *wd; ++wd, ++p) {
       int c = peek();
       if (c >= 'A' && c <= 'Z') c += 32;
@@ -263,7 +289,7 @@ struct Cursor {
 // that 192-bit product give 54 bits (mantissa and round bit) and
 // floor(log2 10^q) = f + q fixes the exponent.  Truncating T(q) and the
 // lowest product word leaves the product low by less than one unit of `mid`.
-__device__ bool decimal_to_double(uint64_t w, int q, bool neg, uint64_t *bits) {
+__device__ __forceinline__ bool decimal_to_double(uint64_t w, int q, bool neg, uint64_t *bits) {
   const uint64_t sign = neg ? 0x8000000000000000ull : 0ull;
   if (w == 0) {
     *bits = sign;
@@ -301,8 +327,9 @@ constexpr uint64_t kInf = 0x7FF0000000000000ull;
 // qi::double_ at the cursor.  Returns 0 (no number), 1 (value in *bits),
 // 2 (a number for the host's strtod: [*ts, cursor)), 3 (a "nan(" without
 // ')' on this line: the sequential reader decides).
-__device__ int real_(Cursor &c, uint64_t *bits, uint64_t *ts) {
-  const uint64_t start = c.p;
+template <class Cur>
+__device__ __forceinline__ int real_(Cur &c, uint64_t *bits, uint32_t *ts) {
+  const uint32_t start = c.p;
   bool neg = false;
   int ch = c.peek();
   if (ch == '+' || ch == '-') {
@@ -314,7 +341,7 @@ __device__ int real_(Cursor &c, uint64_t *bits, uint64_t *ts) {
   if (ch == 'n' || ch == 'N') {
     if (!c.word_ci("nan")) return 0;
     if (c.peek() == '(') {
-      const uint64_t save = c.p;
+      const uint32_t save = c.p;
       int x;
       while ((x = c.peek()) >= 0 && x != ')') ++c.p;
       if (x < 0) {
@@ -349,7 +376,7 @@ __device__ int real_(Cursor &c, uint64_t *bits, uint64_t *ts) {
     ++c.p;
   }
   if (c.peek() == '.') {
-    const uint64_t dot = c.p;
+    const uint32_t dot = c.p;
     ++c.p;
     bool frac = false;
     while ((ch = c.peek()) >= '0' && ch <= '9') {
@@ -373,7 +400,7 @@ __device__ int real_(Cursor &c, uint64_t *bits, uint64_t *ts) {
   if (!digits) return 0;
   ch = c.peek();
   if (ch == 'e' || ch == 'E') {
-    const uint64_t save = c.p;
+    const uint32_t save = c.p;
     ++c.p;
     bool eneg = false;
     ch = c.peek();
@@ -401,12 +428,13 @@ __device__ int real_(Cursor &c, uint64_t *bits, uint64_t *ts) {
 struct LineOut {
   uint32_t i, j;
   uint64_t v;
-  uint64_t ts, te;  // the number's text, for the host's strtod
+  uint32_t ts, te;  // the number's text (offsets in the line), for the host's strtod
   uint32_t hdr[4];
   bool hostnum;
 };
 
-__device__ uint8_t parse_line(Cursor &c, bool last, LineOut &o) {
+template <class Cur>
+__device__ __forceinline__ uint8_t parse_line(Cur &c, bool last, LineOut &o) {
   c.skip_blanks();
   const int ch = c.peek();
   if (ch < 0) return last ? K_BLANK_END : K_ERROR;
@@ -426,6 +454,7 @@ __device__ uint8_t parse_line(Cursor &c, bool last, LineOut &o) {
     const char *lit = "p qubo";
     for (; *lit; ++lit, ++c.p)
       if (c.peek() != *lit) return K_ERROR;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
       c.skip_blanks();
       if (!c.uint_(&o.hdr[k])) return K_ERROR;
@@ -450,52 +479,123 @@ __device__ uint8_t parse_line(Cursor &c, bool last, LineOut &o) {
   return c.eol(last) ? K_COEF : K_ERROR;
 }
 
-__global__ void __launch_bounds__(kThreads) parse_lines_kernel(
-    const uint8_t *text, const uint64_t *starts, uint32_t num_lines, uint8_t *kind, uint32_t *li,
-    uint32_t *lj, uint64_t *lv, Stats *st, HostNum *hostnum, uint32_t hostnum_cap) {
-  const uint32_t l = blockIdx.x * kThreads + threadIdx.x;
-  uint32_t coef = 0, mx = 0;
-  uint8_t k = K_COMMENT;
-  if (l < num_lines) {
-    Cursor c;
-    c.text = reinterpret_cast<const uint4 *>(text);
-    c.p = starts[l];
-    c.e = starts[l + 1];
-    c.wpos = ~0ull;
-    LineOut o;
-    o.hostnum = false;
-    k = parse_line(c, l + 1 == num_lines, o);
-    kind[l] = k;
-    if (k == K_COEF) {
-      li[l] = o.i;
-      lj[l] = o.j;
-      lv[l] = o.v;
-      coef = 1;
-      mx = o.j;
-      if (o.hostnum) {
-        const uint32_t s = atomicAdd(&st->n_hostnum, 1u);
-        if (s < hostnum_cap) hostnum[s] = HostNum{o.ts, l, (uint32_t)(o.te - o.ts)};
-      }
-    } else if (k >= K_ERROR) {
-      atomicAdd(&st->n_bad, 1u);
+constexpr int kParseRounds = 8;  // lines per thread; consecutive lanes take consecutive lines
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
+  return v;
+}
+
+constexpr int kSpanBytes = 16384;  // LDS staging of one round's text
+
+template <bool kLds>
+__device__ __forceinline__ void parse_one(const uint8_t *text, uint64_t len, const uint8_t *lds,
+                                          uint64_t s, uint64_t e, uint32_t l, uint32_t num_lines,
+                                          uint8_t *kind, uint32_t *li, uint32_t *lj, uint64_t *lv,
+                                          Stats *st, HostNum *hostnum, uint32_t hostnum_cap,
+                                          uint32_t &coef, uint32_t &mx, uint32_t &first) {
+  Cursor<kLds> c;
+  c.text = text;
+  c.lds = lds;
+  c.len = len;
+  c.s = s;
+  c.wpos = ~0ull;
+  c.p = 0;
+  c.e = (uint32_t)(e - s);
+  LineOut o;
+  o.hostnum = false;
+  const uint8_t k = e - s > 0xFFFFFFFFull ? (uint8_t)K_FALLBACK : parse_line(c, l + 1 == num_lines, o);
+  kind[l] = k;
+  if (k == K_COEF) {
+    li[l] = o.i;
+    lj[l] = o.j;
+    lv[l] = o.v;
+    ++coef;
+    mx = max(mx, o.j);
+    if (o.hostnum) {
+      const uint32_t q = atomicAdd(&st->n_hostnum, 1u);
+      if (q < hostnum_cap) hostnum[q] = HostNum{s + o.ts, l, o.te - o.ts};
     }
-    if (k == K_HEADER) {
-      atomicAdd(&st->n_header, 1u);
-      atomicMin(&st->header_line, l);
-      for (int q = 0; q < 4; ++q) st->hdr[q] = o.hdr[q];
-    }
+  } else if (k >= K_ERROR) {
+    atomicAdd(&st->n_bad, 1u);
   }
-  // the wave's lowest line that is neither a comment nor the blank end
-  const uint64_t other = __ballot(l < num_lines && k != K_COMMENT && k != K_BLANK_END);
+  if (k == K_HEADER) {
+    atomicAdd(&st->n_header, 1u);
+    atomicMin(&st->header_line, l);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) st->hdr[q] = o.hdr[q];
+  }
+  // the lowest line that is neither a comment nor the blank end
+  if (k != K_COMMENT && k != K_BLANK_END) first = min(first, l);
+}
+
+// Round r of a work-group parses lines l0 + [0, kThreads), one per thread,
+// from LDS when their text fits kSpanBytes (staged with 16-byte loads).
+__global__ void __launch_bounds__(kThreads) parse_lines_kernel(
+    const uint8_t *text, uint64_t len, const uint64_t *starts, uint32_t num_lines, uint8_t *kind,
+    uint32_t *li, uint32_t *lj, uint64_t *lv, Stats *st, HostNum *hostnum,
+    uint32_t hostnum_cap) {
+  __shared__ uint4 span[kSpanBytes / 16];
+  uint32_t coef = 0, mx = 0, first = kNone;
+  for (int r = 0; r < kParseRounds; ++r) {
+    const uint32_t l0 = (blockIdx.x * kParseRounds + r) * kThreads;
+    if (l0 >= num_lines) break;
+    const uint32_t l1 = min(l0 + (uint32_t)kThreads, num_lines);
+    const uint64_t a0 = starts[l0] & ~15ull, a1 = starts[l1];
+    const bool staged = a1 - a0 <= (uint64_t)kSpanBytes;
+    if (staged) {
+      for (uint32_t k = threadIdx.x; (uint64_t)k * 16 < a1 - a0; k += kThreads)
+        span[k] = load16(text, len, a0 + 16ull * k);
+      __syncthreads();
+    }
+    const uint32_t l = l0 + threadIdx.x;
+    if (l < num_lines) {
+      const uint64_t s = starts[l], e = starts[l + 1];
+      if (staged)
+        parse_one<true>(text, len, reinterpret_cast<const uint8_t *>(span) + (s - a0), s, e, l,
+                        num_lines, kind, li, lj, lv, st, hostnum, hostnum_cap, coef, mx, first);
+      else
+        parse_one<false>(text, len, nullptr, s, e, l, num_lines, kind, li, lj, lv, st, hostnum,
+                         hostnum_cap, coef, mx, first);
+    }
+    if (staged) __syncthreads();
+  }
+  // one atomic per statistic and work-group
+  __shared__ uint32_t red[3][kThreads / 64];
   coef = wave_sum(coef);
   mx = wave_max(mx);
-  if ((threadIdx.x & 63) == 0) {
+  first = wave_min(first);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = coef;
+    red[1][wave] = mx;
+    red[2][wave] = first;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      coef += red[0][w];
+      mx = max(mx, red[1][w]);
+      first = min(first, red[2][w]);
+    }
     if (coef) {
       atomicAdd(&st->n_coef, coef);
       atomicMax(&st->max_index, mx);
     }
-    if (other) atomicMin(&st->first_other, (l & ~63u) + (uint32_t)(__ffsll((long long)other) - 1));
+    if (first != kNone) atomicMin(&st->first_other, first);
   }
+}
+
+constexpr uint32_t kTokBytes = 64;
+
+// the text of hostnum[k] (at most kTokBytes long) into out[k * kTokBytes ...]
+__global__ void gather_tokens_kernel(const uint8_t *text, const HostNum *hostnum, uint32_t count,
+                                     uint8_t *out) {
+  const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= count) return;
+  const HostNum h = hostnum[k];
+  for (uint32_t t = 0; t < h.len && t < kTokBytes; ++t) out[(uint64_t)k * kTokBytes + t] = text[h.start + t];
 }
 
 // hostnum[k].line receives the strtod value vals[k]
@@ -564,25 +664,21 @@ struct Dedupe {
 
 // ---------------------------------------------------------------- compaction
 
-__device__ __forceinline__ uint32_t tile_winners(const uint8_t *kind, const uint32_t *li,
-                                                 const uint32_t *lj, uint32_t num_lines,
-                                                 const Dedupe &dd, uint32_t l0, uint32_t *mask) {
-  uint32_t m = 0;
-#pragma unroll 4
-  for (int k = 0; k < kLinesPerThread; ++k) {
-    const uint32_t l = l0 + k;
-    if (l < num_lines && kind[l] == K_COEF && dd.winner(l, li[l], lj[l])) m |= 1u << k;
-  }
-  *mask = m;
-  return __popc(m);
+// A compaction tile is kTileLines lines in kLinesPerThread rounds; round r
+// covers lines tile + r * kThreads + [0, kThreads), one per thread.
+__device__ __forceinline__ bool is_winner(const uint8_t *kind, const uint32_t *li,
+                                          const uint32_t *lj, uint32_t num_lines,
+                                          const Dedupe &dd, uint32_t l) {
+  return l < num_lines && kind[l] == K_COEF && dd.winner(l, li[l], lj[l]);
 }
 
 __global__ void __launch_bounds__(kThreads) compact_count_kernel(
     const uint8_t *kind, const uint32_t *li, const uint32_t *lj, uint32_t num_lines, Dedupe dd,
     uint32_t *tile_count) {
-  const uint32_t l0 = blockIdx.x * kTileLines + threadIdx.x * kLinesPerThread;
-  uint32_t m;
-  const uint32_t c = tile_winners(kind, li, lj, num_lines, dd, l0, &m);
+  uint32_t c = 0;
+  for (int r = 0; r < kLinesPerThread; ++r)
+    c += is_winner(kind, li, lj, num_lines, dd,
+                   (blockIdx.x * kLinesPerThread + r) * kThreads + threadIdx.x);
   uint32_t total;
   block_exclusive(c, &total);
   if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
@@ -592,24 +688,33 @@ __global__ void __launch_bounds__(kThreads) compact_scatter_kernel(
     const uint8_t *kind, const uint32_t *li, const uint32_t *lj, const uint64_t *lv,
     uint32_t num_lines, Dedupe dd, const uint32_t *tile_off, int32_t *ti, int32_t *tj,
     uint64_t *tv, Stats *st) {
-  const uint32_t l0 = blockIdx.x * kTileLines + threadIdx.x * kLinesPerThread;
-  uint32_t m;
-  const uint32_t c = tile_winners(kind, li, lj, num_lines, dd, l0, &m);
-  uint32_t total;
-  uint32_t pos = tile_off[blockIdx.x] + block_exclusive(c, &total);
-  uint32_t lin = 0;
-  while (m) {
-    const int k = __ffs(m) - 1;
-    m &= m - 1;
-    const uint32_t l = l0 + k;
-    ti[pos] = (int32_t)li[l];
-    tj[pos] = (int32_t)lj[l];
-    tv[pos] = lv[l];
-    lin += li[l] == lj[l];
-    ++pos;
+  __shared__ uint32_t wcnt[kThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t base = tile_off[blockIdx.x], lin = 0;
+  for (int r = 0; r < kLinesPerThread; ++r) {
+    const uint32_t l = (blockIdx.x * kLinesPerThread + r) * kThreads + threadIdx.x;
+    const bool win = is_winner(kind, li, lj, num_lines, dd, l);
+    const uint64_t b = __ballot(win);
+    if (lane == 0) wcnt[wave] = (uint32_t)__popcll(b);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      before += w < wave ? wcnt[w] : 0u;
+      all += wcnt[w];
+    }
+    __syncthreads();
+    if (win) {
+      const uint32_t pos = base + before + (uint32_t)__popcll(b & ((1ull << lane) - 1ull));
+      const uint32_t i = li[l], j = lj[l];
+      ti[pos] = (int32_t)i;
+      tj[pos] = (int32_t)j;
+      tv[pos] = lv[l];
+      lin += i == j;
+    }
+    base += all;
   }
   lin = wave_sum(lin);
-  if ((threadIdx.x & 63) == 0 && lin) atomicAdd(&st->n_lin, lin);
+  if (lane == 0 && lin) atomicAdd(&st->n_lin, lin);
 }
 
 // ---------------------------------------------------------------- consumers
@@ -770,14 +875,21 @@ void sequential_parse(os::Qubo &q, const char *text, size_t len) {
   q.ctx->sync();
 }
 
+// Work arrays of a parse: the context's, kept for the next parse (the
+// sequential fallback releases them).
 struct Scratch {
-  DevBuf text, tiles, starts, kind, li, lj, lv, stats, hostnum, hvals, owner, keys, vals;
+  DevBuf &text, &tiles, &starts, &kind, &li, &lj, &lv, &stats, &hostnum, &hvals, &owner, &keys,
+      &vals;
+  explicit Scratch(Ctx &c)
+      : text(c.ingest[0]), tiles(c.ingest[1]), starts(c.ingest[2]), kind(c.ingest[3]),
+        li(c.ingest[4]), lj(c.ingest[5]), lv(c.ingest[6]), stats(c.ingest[7]),
+        hostnum(c.ingest[8]), hvals(c.ingest[9]), owner(c.ingest[10]), keys(c.ingest[11]),
+        vals(c.ingest[12]) {}
   void release() {
     for (auto *b : {&text, &tiles, &starts, &kind, &li, &lj, &lv, &stats, &hostnum, &hvals,
                     &owner, &keys, &vals})
       b->release();
   }
-  ~Scratch() { release(); }
 };
 
 // First-wins dedupe and file-order compaction of `num_lines` parsed lines
@@ -839,26 +951,39 @@ uint32_t dedupe_compact(os::Qubo &q, Scratch &s, uint32_t num_lines, uint32_t n,
   return T;
 }
 
-void parse_text(os::Qubo &q, const char *text, size_t len) {
+// Text from the host (`text`) or already on the GPU (`dev`, 16-byte aligned).
+void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   Ctx &c = *q.ctx;
   hipStream_t stream = c.stream;
   OS_REQUIRE(len < (1ull << 40), "text longer than 2^40 bytes");
-  Scratch s;
+  Scratch s(c);
   const uint64_t tiles64 = (len + kTileBytes) / kTileBytes;  // one more byte: a final '\r'
   OS_REQUIRE(tiles64 < (1ull << 31), "text too long");
   const uint32_t tiles = (uint32_t)tiles64;
-  const uint64_t padded = tiles64 * kTileBytes + 64;
-  s.text.ensure(padded);
-  OS_HIP_CHECK(hipMemsetAsync(s.text.as<uint8_t>() + len, 0, padded - len, stream));
-  if (len) OS_HIP_CHECK(hipMemcpyAsync(s.text.ptr, text, len, hipMemcpyHostToDevice, stream));
+  if (!dev) {
+    s.text.ensure(std::max<size_t>(len, 16));
+    if (len) OS_HIP_CHECK(hipMemcpyAsync(s.text.ptr, text, len, hipMemcpyHostToDevice, stream));
+    dev = s.text.as<uint8_t>();
+  }
+  // the host copy of a device text, made only when the host needs it
+  std::vector<char> host_copy;
+  auto host_text = [&]() -> const char * {
+    if (text) return text;
+    if (host_copy.size() != len) {
+      host_copy.resize(len);
+      if (len)
+        OS_HIP_CHECK(hipMemcpyAsync(host_copy.data(), dev, len, hipMemcpyDeviceToHost, stream));
+      c.sync();
+    }
+    return host_copy.data();
+  };
   s.tiles.ensure(4ull * (tiles + 1));
   s.stats.ensure(sizeof(Stats));
   Stats *st = s.stats.as<Stats>();
   init_stats_kernel<<<1, 64, 0, stream>>>(st);
 
   OS_HIP_CHECK(hipEventRecord(c.ev_start, stream));
-  line_count_kernel<<<tiles, kThreads, 0, stream>>>(s.text.as<uint8_t>(), len,
-                                                   s.tiles.as<uint32_t>());
+  line_count_kernel<<<tiles, kThreads, 0, stream>>>(dev, len, s.tiles.as<uint32_t>());
   scan_tiles_kernel<<<1, kScanThreads, 0, stream>>>(s.tiles.as<uint32_t>(), tiles);
   OS_HIP_CHECK(hipGetLastError());
   uint32_t ends = 0;
@@ -868,8 +993,7 @@ void parse_text(os::Qubo &q, const char *text, size_t len) {
   OS_REQUIRE(ends < 0xFFFFFFF0u, "more than 2^32 - 16 lines");
   const uint32_t L = ends + 1;
   s.starts.ensure(8ull * (L + 1));
-  line_starts_kernel<<<tiles, kThreads, 0, stream>>>(s.text.as<uint8_t>(), len,
-                                                    s.tiles.as<uint32_t>(), tiles,
+  line_starts_kernel<<<tiles, kThreads, 0, stream>>>(dev, len, s.tiles.as<uint32_t>(), tiles,
                                                     s.starts.as<uint64_t>());
   OS_HIP_CHECK(hipEventRecord(c.ev_stop, stream));
   OS_HIP_CHECK(hipGetLastError());
@@ -884,8 +1008,8 @@ void parse_text(os::Qubo &q, const char *text, size_t len) {
   OS_HIP_CHECK(hipEventCreate(&e1));
   OS_HIP_CHECK(hipEventCreate(&e2));
   OS_HIP_CHECK(hipEventRecord(e1, stream));
-  parse_lines_kernel<<<blocks_for(L, kThreads), kThreads, 0, stream>>>(
-      s.text.as<uint8_t>(), s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
+  parse_lines_kernel<<<blocks_for(L, (uint64_t)kThreads * kParseRounds), kThreads, 0, stream>>>(
+      dev, len, s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
       s.lj.as<uint32_t>(), s.lv.as<uint64_t>(), st, s.hostnum.as<HostNum>(), hcap);
   OS_HIP_CHECK(hipEventRecord(e2, stream));
   OS_HIP_CHECK(hipGetLastError());
@@ -900,8 +1024,9 @@ void parse_text(os::Qubo &q, const char *text, size_t len) {
   const bool header = h.n_header == 1;
   if (h.n_bad || h.n_header > 1 || (header && h.header_line != h.first_other) ||
       h.n_hostnum > hcap) {
+    const char *t = host_text();
     s.release();
-    sequential_parse(q, text, len);
+    sequential_parse(q, t, len);
     return;
   }
   if (h.n_hostnum) {
@@ -910,10 +1035,27 @@ void parse_text(os::Qubo &q, const char *text, size_t len) {
     OS_HIP_CHECK(hipMemcpyAsync(hn.data(), s.hostnum.ptr, sizeof(HostNum) * hn.size(),
                                 hipMemcpyDeviceToHost, stream));
     c.sync();
+    std::vector<char> toks;
+    bool all_short = true;
+    for (const HostNum &x : hn) all_short = all_short && x.len <= kTokBytes;
+    if (!text && all_short) {
+      // only the tokens come back from a device text
+      s.hvals.ensure(kTokBytes * hn.size());
+      gather_tokens_kernel<<<blocks_for(hn.size(), kThreads), kThreads, 0, stream>>>(
+          dev, s.hostnum.as<HostNum>(), (uint32_t)hn.size(), s.hvals.as<uint8_t>());
+      OS_HIP_CHECK(hipGetLastError());
+      toks.resize(kTokBytes * hn.size());
+      OS_HIP_CHECK(hipMemcpyAsync(toks.data(), s.hvals.ptr, toks.size(), hipMemcpyDeviceToHost,
+                                  stream));
+      c.sync();
+    }
     std::vector<double> vals(hn.size());
     std::string tok;
     for (size_t k = 0; k < hn.size(); ++k) {
-      tok.assign(text + hn[k].start, hn[k].len);
+      if (!toks.empty())
+        tok.assign(toks.data() + kTokBytes * k, hn[k].len);
+      else
+        tok.assign(host_text() + hn[k].start, hn[k].len);
       vals[k] = std::strtod(tok.c_str(), nullptr);
     }
     s.hvals.ensure(8 * vals.size());
@@ -976,7 +1118,7 @@ void read_binary(os::Qubo &q, const uint8_t *buf, size_t len) {
   upload_terms(q, reinterpret_cast<const int32_t *>(buf + kHeaderBytes),
                reinterpret_cast<const int32_t *>(buf + kHeaderBytes + 4 * T),
                buf + kHeaderBytes + 8 * T);
-  Scratch s;
+  Scratch s(c);
   s.stats.ensure(sizeof(Stats));
   Stats *st = s.stats.as<Stats>();
   init_stats_kernel<<<1, 64, 0, stream>>>(st);
@@ -1020,7 +1162,31 @@ int os_qubo_parse(os_ctx *ctx_, const char *text, size_t len, os_qubo **out) {
     c.activate();
     std::unique_ptr<os::Qubo> q(new os::Qubo());
     q->ctx = &c;
-    parse_text(*q, text ? text : "", len);
+    parse_text(*q, text ? text : "", nullptr, len);
+    *out = reinterpret_cast<os_qubo *>(q.release());
+  });
+}
+
+int os_qubo_parse_device(os_ctx *ctx_, const void *text, size_t len, os_qubo **out) {
+  return os::guarded([&] {
+    OS_REQUIRE(ctx_ && out && (text || len == 0), "null argument");
+    OS_REQUIRE(((uintptr_t)text & 15) == 0, "device text must be 16-byte aligned");
+    Ctx &c = *reinterpret_cast<Ctx *>(ctx_);
+    c.activate();
+    std::unique_ptr<os::Qubo> q(new os::Qubo());
+    q->ctx = &c;
+    DevBuf empty;
+    if (!text) {
+      empty.ensure(16);
+      text = empty.ptr;
+    }
+    try {
+      parse_text(*q, nullptr, static_cast<const uint8_t *>(text), len);
+    } catch (...) {
+      empty.release();
+      throw;
+    }
+    empty.release();
     *out = reinterpret_cast<os_qubo *>(q.release());
   });
 }

@@ -186,3 +186,39 @@ def test_large_dense_text(ctx):
     assert res["n"] == n and res["num_linear"] == n
     iu = np.triu_indices(n)
     assert np.array_equal(res["v"], q[iu])
+
+
+def test_device_text_matches_host_text(ctx):
+    """os_qubo_parse_device: text already in GPU memory, including texts the
+    sequential reader decides and numbers left to strtod (short tokens
+    gathered on the GPU, long ones through a host copy)."""
+    rng = np.random.default_rng(21)
+    toks = qc.decimal_tokens(rng, 3000) + ["0." + "3" * 100, "1" * 70 + "e-60"]
+    texts = [t for t, _ in qc.GRAMMAR] + [qc.REF_VALID, qc.linear_text(toks), ""]
+    for k, text in enumerate(texts):
+        data = text.encode("latin-1")
+        code, msg, res = oracle.qubo_parse(data)
+        with onesolver_amd.DeviceBytes(ctx, data) as dev:
+            if code:
+                with pytest.raises(onesolver_amd.InvalidArgument) as ei:
+                    onesolver_amd.Qubo(ctx, device=dev.source)
+                assert ei.value.message == msg, k
+                continue
+            with onesolver_amd.Qubo(ctx, device=dev.source) as q:
+                i, j, v = q.terms()
+                assert q.n == res["n"], k
+                assert np.array_equal(i, res["i"]) and np.array_equal(j, res["j"]), k
+                assert np.array_equal(_bits(v), _bits(res["v"])), k
+
+
+def test_long_lines_read_from_global_memory(ctx):
+    """Rounds of 256 lines longer than the 16 KiB staging span are parsed from
+    global memory; short and long rounds mixed."""
+    rng = np.random.default_rng(31)
+    m = 5000
+    parts = ["p qubo 0 %d %d 0\n" % (m, m)]
+    for k in range(m):
+        pad = " " * (150 if (k // 256) % 2 else int(rng.integers(0, 4)))
+        com = ("c " + "x" * int(rng.integers(0, 300)) + "\n") if rng.random() < 0.1 else ""
+        parts.append("%s%s%d %d %.17g\n" % (com, pad, k, k, rng.normal()))
+    _check_same(ctx, "".join(parts))
