@@ -42,7 +42,7 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
 TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
-TRAFFIC_INGEST_8192 = None                               # parse_lines_kernel, N=8192 text
+TRAFFIC_INGEST_8192 = (2 * 642693.5 + 558660.7) * 1024   # parse_lines_kernel, N=8192 text
 
 
 def log(msg):

@@ -54,7 +54,8 @@ using os::DevBuf;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTileBytes = kThreads * 32;    // line index tiles
+constexpr int kIndexRounds = 8;              // line index: 32 bytes per thread and round
+constexpr int kTileBytes = kThreads * 32 * kIndexRounds;
 constexpr int kLinesPerThread = 16;          // compaction tiles
 constexpr int kTileLines = kThreads * kLinesPerThread;
 constexpr int kScanThreads = 1024;
@@ -170,11 +171,15 @@ __device__ __forceinline__ uint32_t thread_ends(const uint8_t *text, uint64_t le
   return __popc(m);
 }
 
+// A tile is kIndexRounds rounds of kThreads x 32 contiguous bytes.
+__device__ __forceinline__ uint64_t round_base(uint32_t tile, int r) {
+  return (uint64_t)tile * kTileBytes + (uint64_t)r * (kThreads * 32) + threadIdx.x * 32u;
+}
+
 __global__ void __launch_bounds__(kThreads) line_count_kernel(const uint8_t *text, uint64_t len,
                                                               uint32_t *tile_count) {
-  const uint64_t base = (uint64_t)blockIdx.x * kTileBytes + threadIdx.x * 32u;
-  uint32_t m;
-  const uint32_t c = thread_ends(text, len, base, &m);
+  uint32_t c = 0, m;
+  for (int r = 0; r < kIndexRounds; ++r) c += thread_ends(text, len, round_base(blockIdx.x, r), &m);
   uint32_t total;
   block_exclusive(c, &total);
   if (threadIdx.x == 0) tile_count[blockIdx.x] = total;
@@ -201,15 +206,19 @@ __global__ void __launch_bounds__(kThreads) line_starts_kernel(const uint8_t *te
                                                                const uint32_t *tile_off,
                                                                uint32_t num_tiles,
                                                                uint64_t *starts) {
-  const uint64_t base = (uint64_t)blockIdx.x * kTileBytes + threadIdx.x * 32u;
-  uint32_t m;
-  const uint32_t c = thread_ends(text, len, base, &m);
-  uint32_t total;
-  uint32_t idx = tile_off[blockIdx.x] + block_exclusive(c, &total);
-  while (m) {
-    const int k = __ffs(m) - 1;
-    m &= m - 1;
-    starts[1 + idx++] = base + k + 1;
+  uint32_t run = tile_off[blockIdx.x];
+  for (int r = 0; r < kIndexRounds; ++r) {
+    const uint64_t base = round_base(blockIdx.x, r);
+    uint32_t m;
+    const uint32_t c = thread_ends(text, len, base, &m);
+    uint32_t total;
+    uint32_t idx = run + block_exclusive(c, &total);
+    while (m) {
+      const int k = __ffs(m) - 1;
+      m &= m - 1;
+      starts[1 + idx++] = base + k + 1;
+    }
+    run += total;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     starts[0] = 0;
