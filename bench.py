@@ -8,7 +8,7 @@ every 512 sweeps.  One step = one full anneal of that batch with Q already
 resident in HBM.  Prints one JSON line (see the keys below).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload c1|c2|c3|c4|c5|exh|pt|ingest|copy]
+                    [--workload c1|c2|c3|c4|c5|exh|pt|ingest|batch|copy]
 
 Other workloads: c1 dense N=28 in reference semantics (1000 iterations x
 1024 tries); c3 a batch of 512 sparse instances; c4 the learned warm start
@@ -17,7 +17,8 @@ fields in HBM; exh the exhaustive search (all 2^n states, dense U[-1,1] upper
 triangle, n=32 by default); pt replica exchange on the config-2 instance
 (256 ladders x 64 temperatures, plain annealing at the same budget beside it);
 ingest the GPU `.qubo` reader on the text of a dense N=8192 instance already
-in GPU memory (33.6M lines, bytes/s); copy the 1 GiB device copy that
+in GPU memory (33.6M lines, bytes/s); batch one-solver-batch over the config-3
+batch written as 512 `.qubo` files (instances/s, files to CSVs); copy the 1 GiB device copy that
 calibrates the PMC traffic.
 """
 import argparse
@@ -533,6 +534,82 @@ def run_exhaustive(args):
     return res
 
 
+def run_batch_cli(args):
+    """one-solver-batch over the config-3 batch written as 512 `.qubo` files:
+    GPU parse of every file, one batched sparse anneal (256 replicas, 2048
+    sweeps, beta 0.1 -> 3.0 as inverse temperature), one Solution CSV per
+    instance.  One step = one run of the CLI process, files to CSVs."""
+    import subprocess
+    import tempfile
+
+    import onesolver_amd
+    from onesolver_amd import instances
+    insts = instances.batch_c3(512)
+    R, S = 256, args.sweeps_c3
+    tmp = tempfile.mkdtemp(prefix="osbatch_")
+    inp, out = os.path.join(tmp, "in"), os.path.join(tmp, "out")
+    os.makedirs(inp)
+    os.makedirs(out)
+    nbytes = 0
+    for k, (rp, cl, vl, dg) in enumerate(insts):
+        n = len(dg)
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        up = cl > rows
+        lines = ["p qubo 0 %d %d %d\n" % (n, n, int(up.sum()))]
+        lines += ["%d %d %.17g\n" % (i, i, dg[i]) for i in range(n)]
+        lines += ["%d %d %.17g\n" % t for t in zip(rows[up], cl[up], vl[up])]
+        text = "".join(lines)
+        nbytes += len(text)
+        with open(os.path.join(inp, "c3_%03d.qubo" % k), "w") as f:
+            f.write(text)
+    cmd = [onesolver_amd.BATCH_CLI_PATH, "--input", inp, "--output", out, "--num-iter", str(S),
+           "--num-tries", str(R), "--beta-min", "0.1", "--beta-max", "3", "--beta-kind",
+           "inverse", "--path", "sparse"]
+    for _ in range(args.warmup):
+        subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=300)
+        ph = {}
+        for line in r.stdout.splitlines():
+            for key in ("Read and parse", "Anneal", "Write"):
+                if line.startswith(key + ": "):
+                    ph[key] = float(line.split(": ")[1].split()[0])
+        phases.append(ph)
+    elapsed = time.perf_counter() - t0
+    csvs = len([f for f in os.listdir(out) if f.endswith(".csv")])
+    assert csvs == len(insts), csvs
+    ns = np.array([len(d) for (_, _, _, d) in insts], dtype=np.float64)
+    attempts = float(ns.sum()) * R * S * args.steps
+    mean = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    res = {
+        "metric": "instances/s", "value": len(insts) * args.steps / elapsed, "unit": "instances/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 fields, fp64 energies",
+        "data": "synthetic (seeded random d-regular graphs, written as .qubo text)",
+        "config": {"workload": f"one-solver-batch: 512 .qubo files ({nbytes / 1e6:.1f} MB) -> "
+                               f"512 CSVs, sparse path, {R} replicas, {S} sweeps",
+                   "instances": len(insts), "replicas": R, "sweeps": S,
+                   "sum_n": int(ns.sum()), "text_bytes": nbytes},
+        "attempts_per_s": attempts / elapsed,
+        "phase_ms": mean,
+        "roofline": None,
+    }
+    if not args.no_cpu_baseline:
+        imax = int(np.argmax(ns))
+        cb = cpu_baseline_csr(insts[imax], onesolver_amd.tscale_from_beta(
+            np.geomspace(0.1, 3.0, S)))
+        # the same attempt rate over the batch's mean instance: instances per second
+        per_inst = float(ns.mean()) * R * S
+        res["cpu_baseline"] = {"value": cb["value"] / per_inst, "unit": "instances/s",
+                               "cores": cb["cores"], "kind": cb["kind"],
+                               "sample": cb["sample"] + f"; {cb['value']:.3g} attempts/s over "
+                                         f"{per_inst:.3g} attempts per mean instance"}
+    return res
+
+
 def _dense_qubo_text(n, seed=0, chunk=1 << 22):
     """`.qubo` text of a dense n-variable instance: the header, then the
     upper triangle row by row as fixed-width lines "iiii jjjj +0.ddddddddddddddddd"
@@ -655,7 +732,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "pt", "ingest", "copy"])
+                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "pt", "ingest", "batch", "copy"])
     ap.add_argument("--ingest-n", type=int, default=8192)
     ap.add_argument("--pt-interval", type=int, default=16)
     ap.add_argument("--exh-n", type=int, default=32)
@@ -688,6 +765,9 @@ def main():
         return
     if args.workload == "c4":
         print(json.dumps(run_warmstart_c4(args)))
+        return
+    if args.workload == "batch":
+        print(json.dumps(run_batch_cli(args)))
         return
     if args.workload == "ingest":
         print(json.dumps(run_ingest(args)))

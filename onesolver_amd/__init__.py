@@ -14,6 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libonesolver.so")
 CLI_PATH = os.path.join(HERE, "bin", "one-solver-anneal")
+BATCH_CLI_PATH = os.path.join(HERE, "bin", "one-solver-batch")
 EXHAUSTIVE_CLI_PATH = os.path.join(HERE, "bin", "one-solver-exhaustive")
 TEST_BIN_PATH = os.path.join(HERE, "bin", "main-test")
 

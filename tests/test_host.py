@@ -165,3 +165,40 @@ def test_exhaustive_cli_host_devices_solve_test1(tmp_path, device):
     assert lines[1].startswith("Output will be saved to: ")
     assert lines[2].startswith("Using device: ")
     assert out.read_text().splitlines() == ["0,1,2,3,energy", "1,1,0,1,-12"]
+
+
+def _batch(args):
+    return subprocess.run([onesolver_amd.BATCH_CLI_PATH] + args, capture_output=True, text=True,
+                          timeout=60)
+
+
+def test_batch_cli_help_and_validation(tmp_path):
+    r = _batch(["--help"])
+    assert r.returncode == 0 and "--beta-kind" in r.stdout and "--path" in r.stdout
+    d = str(tmp_path)
+    for args, msg in [([], "No input file provided."),
+                      (["--input", d], "No output directory provided."),
+                      (["--input", d, "--output", d, "--schedule-type", "x"],
+                       "Unknown beta schedule: x"),
+                      (["--input", d, "--output", d, "--beta-min", "2"],
+                       "Invalid schedule, initial beta is not lesser than final beta"),
+                      (["--input", d, "--output", d, "--beta-kind", "x"], "Unknown beta kind: x"),
+                      (["--input", d, "--output", d, "--path", "x"], "Unknown path: x"),
+                      (["--input", d, "--output", d + "/missing"],
+                       "can not open output directory: " + d + "/missing")]:
+        r = _batch(args)
+        assert r.returncode == 255 and msg in r.stderr, (args, r.stderr)
+    r = _batch(["--input", d + "/nothing.txt", "--output", d])
+    assert r.returncode == 1 and "can not open input file" in r.stderr
+    r = _batch(["--input", d, "--output", d])
+    assert r.returncode == 1 and "no .qubo or .osqb files" in r.stderr
+
+
+def test_batch_cli_without_gpu_reports_no_device(tmp_path):
+    import shutil
+    if onesolver_amd.device_count() > 0:
+        pytest.skip("a GPU is present")
+    shutil.copy(os.path.join(GOLDEN, "examples", "test1.qubo"), tmp_path / "a.qubo")
+    r = _batch(["--input", str(tmp_path), "--output", str(tmp_path)])
+    assert r.returncode == 1
+    assert "No devices of given type could be initialized." in r.stderr
