@@ -222,3 +222,27 @@ def test_long_lines_read_from_global_memory(ctx):
         com = ("c " + "x" * int(rng.integers(0, 300)) + "\n") if rng.random() < 0.1 else ""
         parts.append("%s%s%d %d %.17g\n" % (com, pad, k, k, rng.normal()))
     _check_same(ctx, "".join(parts))
+
+
+def test_dense_problem_streaming_size_and_file_loading(ctx, tmp_path):
+    """os_qubo_dense for n > 1024 (fields in HBM) equals a DenseProblem of the
+    flattened matrix; Qubo.load reads text and binary files alike."""
+    rng = np.random.default_rng(14)
+    n = 1100
+    q = np.triu(rng.integers(-2, 3, (n, n)).astype(np.float64))
+    text = qc.dense_text(q)
+    (tmp_path / "a.qubo").write_text(text)
+    with onesolver_amd.Qubo.load(ctx, tmp_path / "a.qubo") as m:
+        (tmp_path / "a.osqb").write_bytes(m.to_binary())
+        flat = m.flatten()
+    assert np.array_equal(flat, q + np.triu(q, 1).T)
+    ts = np.geomspace(2.0, 0.5, 3).astype(np.float32)
+    with onesolver_amd.Qubo.load(ctx, tmp_path / "a.osqb") as m:
+        d = m.dense()
+    a = d.anneal(64, ts, seed=2, final_states=True)
+    d.close()
+    p = onesolver_amd.DenseProblem(ctx, flat)
+    b = p.anneal(64, ts, seed=2, final_states=True)
+    p.close()
+    assert (a["final_states"] == b["final_states"]).all()
+    assert (a["best_energies"] == b["best_energies"]).all()
