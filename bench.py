@@ -676,9 +676,13 @@ def run_ingest(args):
     assert info == (n, n, L - n), info
     ms = np.mean(np.asarray(times, np.float64), axis=0)
     # host text: the same parse with the text handed over from host memory
+    # (upload through pinned staging buffers included)
+    host_text = text.tobytes()
+    onesolver_amd.Qubo(ctx, host_text).close()
     th = time.perf_counter()
-    onesolver_amd.Qubo(ctx, text.tobytes()).close()
+    onesolver_amd.Qubo(ctx, host_text).close()
     host_s = time.perf_counter() - th
+    del host_text
     dev.close()
     ctx.close()
     nbytes = len(text)

@@ -60,6 +60,10 @@ Ctx::~Ctx() {
   for (auto *b : {&b_q, &b_beta, &b_bits, &b_run, &b_energy, &b_scratch, &b_fields, &b_bytes})
     b->release();
   for (DevBuf &b : ingest) b.release();
+  for (int k = 0; k < 2; ++k) {
+    if (pinned[k]) (void)hipHostFree(pinned[k]);
+    if (pinned_ev[k]) (void)hipEventDestroy(pinned_ev[k]);
+  }
   if (ev_start) (void)hipEventDestroy(ev_start);
   if (ev_stop) (void)hipEventDestroy(ev_stop);
   if (stream) (void)hipStreamDestroy(stream);

@@ -29,8 +29,11 @@ struct Ctx {
   hipEvent_t ev_stop = nullptr;
   // scratch buffers of the one-shot entry points
   DevBuf b_q, b_beta, b_bits, b_run, b_energy, b_scratch, b_fields, b_bytes;
-  // .qubo ingest work arrays (qubo_text.hip), kept between parses
+  // .qubo ingest work arrays (qubo_text.hip), kept between parses, and two
+  // pinned staging buffers for host text (hipHostMalloc, kPinnedBytes each)
   DevBuf ingest[16];
+  void *pinned[2] = {nullptr, nullptr};
+  hipEvent_t pinned_ev[2] = {nullptr, nullptr};
 
   explicit Ctx(int dev);
   ~Ctx();

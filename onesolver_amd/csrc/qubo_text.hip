@@ -39,6 +39,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -960,6 +961,38 @@ uint32_t dedupe_compact(os::Qubo &q, Scratch &s, uint32_t num_lines, uint32_t n,
   return T;
 }
 
+// Host text to the GPU through the context's two pinned staging buffers:
+// while one buffer's copy to the device runs, the other is filled (by
+// kCopyThreads host threads).
+constexpr size_t kPinnedBytes = 32u << 20;
+constexpr int kCopyThreads = 4;
+
+void upload_text(Ctx &c, uint8_t *dst, const char *text, size_t len) {
+  if (len == 0) return;
+  for (int k = 0; k < 2; ++k) {
+    if (!c.pinned[k]) OS_HIP_CHECK(hipHostMalloc(&c.pinned[k], kPinnedBytes, hipHostMallocDefault));
+    if (!c.pinned_ev[k])
+      OS_HIP_CHECK(hipEventCreateWithFlags(&c.pinned_ev[k], hipEventDisableTiming));
+  }
+  int k = 0;
+  for (size_t off = 0; off < len; off += kPinnedBytes, k ^= 1) {
+    const size_t n = std::min(kPinnedBytes, len - off);
+    // the copy that last read this buffer has finished
+    if (off >= 2 * kPinnedBytes) OS_HIP_CHECK(hipEventSynchronize(c.pinned_ev[k]));
+    char *buf = static_cast<char *>(c.pinned[k]);
+    const size_t part = (n + kCopyThreads - 1) / kCopyThreads;
+    std::vector<std::thread> pool;
+    for (int t = 1; t < kCopyThreads; ++t) {
+      const size_t a = std::min(n, t * part), b = std::min(n, a + part);
+      if (a < b) pool.emplace_back([=] { std::memcpy(buf + a, text + off + a, b - a); });
+    }
+    std::memcpy(buf, text + off, std::min(n, part));
+    for (auto &th : pool) th.join();
+    OS_HIP_CHECK(hipMemcpyAsync(dst + off, buf, n, hipMemcpyHostToDevice, c.stream));
+    OS_HIP_CHECK(hipEventRecord(c.pinned_ev[k], c.stream));
+  }
+}
+
 // Text from the host (`text`) or already on the GPU (`dev`, 16-byte aligned).
 void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   Ctx &c = *q.ctx;
@@ -971,7 +1004,7 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   const uint32_t tiles = (uint32_t)tiles64;
   if (!dev) {
     s.text.ensure(std::max<size_t>(len, 16));
-    if (len) OS_HIP_CHECK(hipMemcpyAsync(s.text.ptr, text, len, hipMemcpyHostToDevice, stream));
+    upload_text(c, s.text.as<uint8_t>(), text, len);
     dev = s.text.as<uint8_t>();
   }
   // the host copy of a device text, made only when the host needs it
