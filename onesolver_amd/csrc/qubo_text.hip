@@ -969,6 +969,10 @@ constexpr int kCopyThreads = 4;
 
 void upload_text(Ctx &c, uint8_t *dst, const char *text, size_t len) {
   if (len == 0) return;
+  if (len <= (4u << 20)) {  // small texts: one direct copy
+    OS_HIP_CHECK(hipMemcpyAsync(dst, text, len, hipMemcpyHostToDevice, c.stream));
+    return;
+  }
   for (int k = 0; k < 2; ++k) {
     if (!c.pinned[k]) OS_HIP_CHECK(hipHostMalloc(&c.pinned[k], kPinnedBytes, hipHostMallocDefault));
     if (!c.pinned_ev[k])
