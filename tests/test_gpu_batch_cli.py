@@ -124,3 +124,12 @@ def test_batch_cli_reports_parse_errors(tmp_path):
     r = _run(["--input", str(tmp_path), "--output", str(tmp_path)])
     assert r.returncode == 1
     assert "b.qubo: Incorrect file, encountered coefficient from lower triangle" in r.stderr
+
+
+def test_batch_cli_forced_dense_path_needs_one_size(tmp_path):
+    (tmp_path / "a.qubo").write_text(_sparse_text(20, 3, 1))
+    (tmp_path / "b.qubo").write_text(_sparse_text(30, 3, 2))
+    r = _run(["--input", str(tmp_path), "--output", str(tmp_path), "--path", "dense",
+              "--num-iter", "5", "--num-tries", "8"])
+    assert r.returncode == 1
+    assert "the dense path needs instances of one size" in r.stderr
