@@ -83,7 +83,8 @@ struct Stats {
   uint32_t n_hostnum;    // values left to strtod
   uint32_t n_lin, n_quad;
   uint32_t hdr[4];
-  uint32_t pad[3];
+  uint32_t n_long;       // rounds longer than the LDS span (the global-memory kernel's)
+  uint32_t pad[2];
 };
 
 struct HostNum {
@@ -540,8 +541,14 @@ __device__ __forceinline__ void parse_one(const uint8_t *text, uint64_t len, con
   if (k != K_COMMENT && k != K_BLANK_END) first = min(first, l);
 }
 
-// Round r of a work-group parses lines l0 + [0, kThreads), one per thread,
-// from LDS when their text fits kSpanBytes (staged with 16-byte loads).
+// Round r of a work-group parses lines l0 + [0, kThreads), one per thread.
+// kStaged: the rounds whose text fits kSpanBytes, from LDS (staged with
+// 16-byte loads); otherwise the other rounds, from global memory.  The
+// staged kernel counts the rounds it leaves; the global one, launched with
+// the same grid only when there are any, takes them.  Without the
+// global-memory cursor the staged kernel, which handles almost all text,
+// keeps its register count low.
+template <bool kStaged>
 __global__ void __launch_bounds__(kThreads) parse_lines_kernel(
     const uint8_t *text, uint64_t len, const uint64_t *starts, uint32_t num_lines, uint8_t *kind,
     uint32_t *li, uint32_t *lj, uint64_t *lv, Stats *st, HostNum *hostnum,
@@ -554,7 +561,9 @@ __global__ void __launch_bounds__(kThreads) parse_lines_kernel(
     const uint32_t l1 = min(l0 + (uint32_t)kThreads, num_lines);
     const uint64_t a0 = starts[l0] & ~15ull, a1 = starts[l1];
     const bool staged = a1 - a0 <= (uint64_t)kSpanBytes;
-    if (staged) {
+    if (kStaged && !staged && threadIdx.x == 0) atomicAdd(&st->n_long, 1u);
+    if (staged != kStaged) continue;  // uniform over the work-group
+    if (kStaged) {
       for (uint32_t k = threadIdx.x; (uint64_t)k * 16 < a1 - a0; k += kThreads)
         span[k] = load16(text, len, a0 + 16ull * k);
       __syncthreads();
@@ -562,14 +571,14 @@ __global__ void __launch_bounds__(kThreads) parse_lines_kernel(
     const uint32_t l = l0 + threadIdx.x;
     if (l < num_lines) {
       const uint64_t s = starts[l], e = starts[l + 1];
-      if (staged)
+      if (kStaged)
         parse_one<true>(text, len, reinterpret_cast<const uint8_t *>(span) + (s - a0), s, e, l,
                         num_lines, kind, li, lj, lv, st, hostnum, hostnum_cap, coef, mx, first);
       else
         parse_one<false>(text, len, nullptr, s, e, l, num_lines, kind, li, lj, lv, st, hostnum,
                          hostnum_cap, coef, mx, first);
     }
-    if (staged) __syncthreads();
+    if (kStaged) __syncthreads();
   }
   // one atomic per statistic and work-group
   __shared__ uint32_t red[3][kThreads / 64];
@@ -1054,7 +1063,8 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   OS_HIP_CHECK(hipEventCreate(&e1));
   OS_HIP_CHECK(hipEventCreate(&e2));
   OS_HIP_CHECK(hipEventRecord(e1, stream));
-  parse_lines_kernel<<<blocks_for(L, (uint64_t)kThreads * kParseRounds), kThreads, 0, stream>>>(
+  const uint32_t pgrid = blocks_for(L, (uint64_t)kThreads * kParseRounds);
+  parse_lines_kernel<true><<<pgrid, kThreads, 0, stream>>>(
       dev, len, s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
       s.lj.as<uint32_t>(), s.lv.as<uint64_t>(), st, s.hostnum.as<HostNum>(), hcap);
   OS_HIP_CHECK(hipEventRecord(e2, stream));
@@ -1062,6 +1072,16 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   Stats h;
   OS_HIP_CHECK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, stream));
   c.sync();
+  if (h.n_long) {
+    // the rounds too long for LDS, from global memory
+    parse_lines_kernel<false><<<pgrid, kThreads, 0, stream>>>(
+        dev, len, s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
+        s.lj.as<uint32_t>(), s.lv.as<uint64_t>(), st, s.hostnum.as<HostNum>(), hcap);
+    OS_HIP_CHECK(hipEventRecord(e2, stream));
+    OS_HIP_CHECK(hipGetLastError());
+    OS_HIP_CHECK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, stream));
+    c.sync();
+  }
   (void)hipEventElapsedTime(&q.ms_index, c.ev_start, c.ev_stop);
   (void)hipEventElapsedTime(&q.ms_parse, e1, e2);
   (void)hipEventDestroy(e1);
