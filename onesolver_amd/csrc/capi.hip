@@ -64,6 +64,8 @@ Ctx::~Ctx() {
     if (pinned[k]) (void)hipHostFree(pinned[k]);
     if (pinned_ev[k]) (void)hipEventDestroy(pinned_ev[k]);
   }
+  for (hipEvent_t e : ingest_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ev_start) (void)hipEventDestroy(ev_start);
   if (ev_stop) (void)hipEventDestroy(ev_stop);
   if (stream) (void)hipStreamDestroy(stream);

@@ -34,6 +34,7 @@ struct Ctx {
   DevBuf ingest[16];
   void *pinned[2] = {nullptr, nullptr};
   hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+  hipEvent_t ingest_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // parse phase timing
 
   explicit Ctx(int dev);
   ~Ctx();

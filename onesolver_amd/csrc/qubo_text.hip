@@ -799,15 +799,23 @@ struct Qubo {
   Ctx *ctx = nullptr;
   int32_t n = 0, max_nodes = 0;
   int64_t num_linear = 0, num_quadratic = 0;
-  DevBuf ti, tj, tv;
+  // the term arrays: views (never released themselves) into one allocation
+  DevBuf mem, ti, tj, tv;
   // timings of the last text parse (ms): line index, parse, dedupe + compaction
   float ms_index = 0, ms_parse = 0, ms_dedupe = 0;
-  ~Qubo() {
-    ti.release();
-    tj.release();
-    tv.release();
-  }
+  ~Qubo() { mem.release(); }
   int64_t terms() const { return num_linear + num_quadratic; }
+  // room for T terms: int32 i[T], int32 j[T], 64-bit values[T], 256-byte aligned
+  void alloc_terms(size_t T) {
+    const size_t a = (4 * T + 4 + 255) & ~(size_t)255, b = (8 * T + 8 + 255) & ~(size_t)255;
+    mem.ensure(2 * a + b);
+    char *base = static_cast<char *>(mem.ptr);
+    ti.ptr = base;
+    tj.ptr = base + a;
+    tv.ptr = base + 2 * a;
+    ti.cap = tj.cap = a;
+    tv.cap = b;
+  }
 };
 
 }  // namespace os
@@ -861,9 +869,7 @@ void builder_checks(int64_t nlin, int64_t nquad, bool header, int max_nodes, int
 void upload_terms(os::Qubo &q, const int32_t *ti, const int32_t *tj, const void *tv) {
   Ctx &c = *q.ctx;
   const size_t T = (size_t)q.terms();
-  q.ti.ensure(4 * T + 4);
-  q.tj.ensure(4 * T + 4);
-  q.tv.ensure(8 * T + 8);
+  q.alloc_terms(T);
   OS_HIP_CHECK(hipMemcpyAsync(q.ti.ptr, ti, 4 * T, hipMemcpyHostToDevice, c.stream));
   OS_HIP_CHECK(hipMemcpyAsync(q.tj.ptr, tj, 4 * T, hipMemcpyHostToDevice, c.stream));
   OS_HIP_CHECK(hipMemcpyAsync(q.tv.ptr, tv, 8 * T, hipMemcpyHostToDevice, c.stream));
@@ -955,9 +961,7 @@ uint32_t dedupe_compact(os::Qubo &q, Scratch &s, uint32_t num_lines, uint32_t n,
   OS_HIP_CHECK(hipMemcpyAsync(&T, s.tiles.as<uint32_t>() + tiles, 4, hipMemcpyDeviceToHost,
                               stream));
   c.sync();
-  q.ti.ensure(4ull * T + 4);
-  q.tj.ensure(4ull * T + 4);
-  q.tv.ensure(8ull * T + 8);
+  q.alloc_terms(T);
   compact_scatter_kernel<<<tiles, kThreads, 0, stream>>>(
       s.kind.as<uint8_t>(), s.li.as<uint32_t>(), s.lj.as<uint32_t>(), s.lv.as<uint64_t>(),
       num_lines, dd, s.tiles.as<uint32_t>(), q.ti.as<int32_t>(), q.tj.as<int32_t>(),
@@ -1059,9 +1063,9 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   s.lv.ensure(8ull * L);
   const uint32_t hcap = std::min<uint32_t>(L, 1u << 22);
   s.hostnum.ensure(sizeof(HostNum) * hcap);
-  hipEvent_t e1, e2;
-  OS_HIP_CHECK(hipEventCreate(&e1));
-  OS_HIP_CHECK(hipEventCreate(&e2));
+  for (hipEvent_t &e : c.ingest_ev)
+    if (!e) OS_HIP_CHECK(hipEventCreate(&e));
+  hipEvent_t e1 = c.ingest_ev[0], e2 = c.ingest_ev[1];
   OS_HIP_CHECK(hipEventRecord(e1, stream));
   const uint32_t pgrid = blocks_for(L, (uint64_t)kThreads * kParseRounds);
   parse_lines_kernel<true><<<pgrid, kThreads, 0, stream>>>(
@@ -1084,8 +1088,6 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   }
   (void)hipEventElapsedTime(&q.ms_index, c.ev_start, c.ev_stop);
   (void)hipEventElapsedTime(&q.ms_parse, e1, e2);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
 
   const bool header = h.n_header == 1;
   if (h.n_bad || h.n_header > 1 || (header && h.header_line != h.first_other) ||
@@ -1136,17 +1138,13 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   if (h.n_coef == 0)
     builder_checks(0, 0, header, max_nodes, num_linear, num_quadratic);
   const uint32_t n = h.max_index + 1;
-  hipEvent_t e3, e4;
-  OS_HIP_CHECK(hipEventCreate(&e3));
-  OS_HIP_CHECK(hipEventCreate(&e4));
+  hipEvent_t e3 = c.ingest_ev[2], e4 = c.ingest_ev[3];
   OS_HIP_CHECK(hipEventRecord(e3, stream));
   uint32_t nlin = 0;
   const uint32_t T = dedupe_compact(q, s, L, n, h.n_coef, st, &nlin);
   OS_HIP_CHECK(hipEventRecord(e4, stream));
   c.sync();
   (void)hipEventElapsedTime(&q.ms_dedupe, e3, e4);
-  (void)hipEventDestroy(e3);
-  (void)hipEventDestroy(e4);
   builder_checks(nlin, (int64_t)T - nlin, header, max_nodes, num_linear, num_quadratic);
   q.n = (int32_t)n;
   q.max_nodes = max_nodes;
