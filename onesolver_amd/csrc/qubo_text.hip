@@ -84,7 +84,8 @@ struct Stats {
   uint32_t n_lin, n_quad;
   uint32_t hdr[4];
   uint32_t n_long;       // rounds longer than the LDS span (the global-memory kernel's)
-  uint32_t pad[2];
+  uint32_t n_runs;       // coefficient lines whose previous line is not one
+  uint32_t n_unsorted;   // coefficient lines whose key is not above the previous line's
 };
 
 struct HostNum {
@@ -635,6 +636,29 @@ __global__ void init_stats_kernel(Stats *st) {
 
 // ---------------------------------------------------------------- dedupe
 
+// Counts runs of consecutive coefficient lines and the lines whose key (i, j)
+// does not rise above the previous line's: one run with rising keys has no
+// repeated key, so the dedupe can keep every coefficient line.
+__global__ void __launch_bounds__(kThreads) order_check_kernel(const uint8_t *kind,
+                                                               const uint32_t *li,
+                                                               const uint32_t *lj,
+                                                               uint32_t num_lines, Stats *st) {
+  const uint32_t l = blockIdx.x * kThreads + threadIdx.x;
+  uint32_t runs = 0, unsorted = 0;
+  if (l < num_lines && kind[l] == K_COEF) {
+    if (l == 0 || kind[l - 1] != K_COEF)
+      runs = 1;
+    else
+      unsorted = li[l - 1] > li[l] || (li[l - 1] == li[l] && lj[l - 1] >= lj[l]);
+  }
+  runs = wave_sum(runs);
+  unsorted = wave_sum(unsorted);
+  if ((threadIdx.x & 63) == 0) {
+    if (runs) atomicAdd(&st->n_runs, runs);
+    if (unsorted) atomicAdd(&st->n_unsorted, unsorted);
+  }
+}
+
 __device__ __forceinline__ uint64_t hash_key(uint64_t k) { return k * 0x9E3779B97F4A7C15ull; }
 
 __global__ void owner_min_kernel(const uint8_t *kind, const uint32_t *li, const uint32_t *lj,
@@ -664,12 +688,14 @@ __global__ void hash_insert_kernel(const uint8_t *kind, const uint32_t *li, cons
 }
 
 struct Dedupe {
+  bool all;               // no repeated key: every coefficient line is kept
   const uint32_t *owner;  // dense: [n x n], else null
   uint32_t n;
   const uint64_t *keys;
   const uint32_t *vals;
   uint64_t mask;
   __device__ bool winner(uint32_t l, uint32_t i, uint32_t j) const {
+    if (all) return true;
     if (owner) return owner[(uint64_t)i * n + j] == l;
     const uint64_t key = ((uint64_t)i << 32) | j;
     uint64_t h = (hash_key(key) >> 20) & mask;
@@ -920,14 +946,16 @@ struct Scratch {
 // First-wins dedupe and file-order compaction of `num_lines` parsed lines
 // into q's term arrays; returns the number of terms, linear ones in *nlin.
 uint32_t dedupe_compact(os::Qubo &q, Scratch &s, uint32_t num_lines, uint32_t n, uint32_t n_coef,
-                        Stats *st, uint32_t *nlin) {
+                        Stats *st, uint32_t *nlin, bool rising = false) {
   Ctx &c = *q.ctx;
   hipStream_t stream = c.stream;
   Dedupe dd{};
   dd.n = n;
   const uint32_t gl = blocks_for(num_lines, kThreads);
   const uint64_t nn = (uint64_t)n * n;
-  if (n <= 16384 && nn <= 16ull * n_coef + (1ull << 22)) {
+  if (rising) {
+    dd.all = true;
+  } else if (n <= 16384 && nn <= 16ull * n_coef + (1ull << 22)) {
     s.owner.ensure(4 * nn);
     OS_HIP_CHECK(hipMemsetAsync(s.owner.ptr, 0xFF, 4 * nn, stream));
     owner_min_kernel<<<gl, kThreads, 0, stream>>>(s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
@@ -1072,11 +1100,16 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
       dev, len, s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
       s.lj.as<uint32_t>(), s.lv.as<uint64_t>(), st, s.hostnum.as<HostNum>(), hcap);
   OS_HIP_CHECK(hipEventRecord(e2, stream));
+  order_check_kernel<<<blocks_for(L, kThreads), kThreads, 0, stream>>>(
+      s.kind.as<uint8_t>(), s.li.as<uint32_t>(), s.lj.as<uint32_t>(), L, st);
   OS_HIP_CHECK(hipGetLastError());
   Stats h;
   OS_HIP_CHECK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, stream));
   c.sync();
+  // the order check saw the lines of the staged rounds only
+  bool rising = h.n_runs == 1 && h.n_unsorted == 0;
   if (h.n_long) {
+    rising = false;
     // the rounds too long for LDS, from global memory
     parse_lines_kernel<false><<<pgrid, kThreads, 0, stream>>>(
         dev, len, s.starts.as<uint64_t>(), L, s.kind.as<uint8_t>(), s.li.as<uint32_t>(),
@@ -1141,7 +1174,7 @@ void parse_text(os::Qubo &q, const char *text, const uint8_t *dev, size_t len) {
   hipEvent_t e3 = c.ingest_ev[2], e4 = c.ingest_ev[3];
   OS_HIP_CHECK(hipEventRecord(e3, stream));
   uint32_t nlin = 0;
-  const uint32_t T = dedupe_compact(q, s, L, n, h.n_coef, st, &nlin);
+  const uint32_t T = dedupe_compact(q, s, L, n, h.n_coef, st, &nlin, rising);
   OS_HIP_CHECK(hipEventRecord(e4, stream));
   c.sync();
   (void)hipEventElapsedTime(&q.ms_dedupe, e3, e4);
