@@ -100,3 +100,39 @@ def test_numbers_match_python_float():
                     dtype=np.uint64)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, [toks[k] for k in bad[:5]]
+
+
+def test_power_of_five_table_is_current_and_exact():
+    """The committed pow5_table.h equals its generator's output, and each
+    entry satisfies its definition: T(q) = floor of 5^q normalised to
+    [2^127, 2^128), and floor(q log2 10), checked with exact integers."""
+    import importlib.util
+    import io
+    from contextlib import redirect_stdout
+
+    from conftest import ROOT
+    csrc = os.path.join(ROOT, "onesolver_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen_pow5", os.path.join(csrc, "gen_pow5.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        gen.main()
+    with open(os.path.join(csrc, "pow5_table.h")) as f:
+        assert f.read() == buf.getvalue()
+    for q in range(gen.QMIN, gen.QMAX + 1):
+        hi, lo, e2 = gen.entry(q)
+        t = (hi << 64) | lo
+        if q >= 0:
+            x, b = 5 ** q, (5 ** q).bit_length()
+            # t * 2^(b - 128) <= 5^q < (t + 1) * 2^(b - 128)
+            assert (t << max(0, b - 128)) <= (x << max(0, 128 - b)) < ((t + 1) << max(0, b - 128))
+        else:
+            y = 5 ** (-q)
+            b = y.bit_length()
+            assert t * y <= (1 << (127 + b)) < (t + 1) * y
+        # 2^e2 <= 10^q < 2^(e2 + 1), as rationals
+        num, den = (10 ** q, 1) if q >= 0 else (1, 10 ** (-q))
+        lo2 = (1 << e2, 1) if e2 >= 0 else (1, 1 << (-e2))
+        hi2 = (1 << (e2 + 1), 1) if e2 + 1 >= 0 else (1, 1 << (-(e2 + 1)))
+        assert lo2[0] * den <= num * lo2[1] and num * hi2[1] < hi2[0] * den
