@@ -43,9 +43,12 @@ def lib():
         L.or_exhaustive_literal.argtypes = [ctypes.c_int, _f64p, _u8p, _f64p]
         L.or_exhaustive.argtypes = [ctypes.c_int, _f64p, _u8p, _f64p, ctypes.c_int]
         anneal_args = [ctypes.c_int, _f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
-                       ctypes.c_uint64, ctypes.c_uint32, _u8p, _f64p, ctypes.c_int]
+                       ctypes.c_uint64, ctypes.c_uint32, _u8p, _f64p, ctypes.c_int, _u8p]
         L.or_anneal_literal.argtypes = anneal_args
         L.or_anneal_fields.argtypes = anneal_args
+        L.or_anneal_stream.argtypes = [ctypes.c_int, _f64p, _f64p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                       ctypes.c_int, _u8p, _f64p, ctypes.c_int, _u8p]
         L.or_sweep_dense.argtypes = [ctypes.c_int, _f64p, _f32p, ctypes.c_int, ctypes.c_uint32,
                                      ctypes.c_uint64, ctypes.c_uint32, _u8p, _u8p, _u8p, _f64p,
                                      ctypes.c_int]
@@ -133,15 +136,34 @@ def exhaustive(flat, n, literal=False, threads=0):
 
 
 def anneal(flat, n, beta, num_tries, sweeps_per_beta=1, seed=1234, first_replica=0,
-           literal=False, threads=0):
+           literal=False, threads=0, final=False):
+    """Reference-mode chains: (best_states, best_energies), plus the final
+    (current) states when final=True."""
     flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
     beta = np.ascontiguousarray(beta, dtype=np.float64)
     bs = np.zeros((num_tries, n), dtype=np.uint8)
     be = np.zeros(num_tries, dtype=np.float64)
+    fs = np.zeros((num_tries, n), dtype=np.uint8) if final else None
     fn = lib().or_anneal_literal if literal else lib().or_anneal_fields
     fn(n, _p(flat, _f64p), _p(beta, _f64p), len(beta), sweeps_per_beta, num_tries, seed,
-       first_replica, _p(bs, _u8p), _p(be, _f64p), threads)
-    return bs, be
+       first_replica, _p(bs, _u8p), _p(be, _f64p), threads, _p(fs, _u8p))
+    return (bs, be, fs) if final else (bs, be)
+
+
+def anneal_shared_stream(flat, n, beta, num_tries, sweeps_per_beta=1, seed=1234,
+                         uniform_words=1, threads=0, final=False, block_tries=0):
+    """Reference-mode chains whose replicas read one stream at offsets 0, 1,
+    2, ... as the reference's engines do (or_anneal_stream); with
+    block_tries, independent runs of that many replicas (run b: seed + b)."""
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    beta = np.ascontiguousarray(beta, dtype=np.float64)
+    bs = np.zeros((num_tries, n), dtype=np.uint8)
+    be = np.zeros(num_tries, dtype=np.float64)
+    fs = np.zeros((num_tries, n), dtype=np.uint8) if final else None
+    lib().or_anneal_stream(n, _p(flat, _f64p), _p(beta, _f64p), len(beta), sweeps_per_beta,
+                           num_tries, block_tries, seed, uniform_words, _p(bs, _u8p),
+                           _p(be, _f64p), threads, _p(fs, _u8p))
+    return (bs, be, fs) if final else (bs, be)
 
 
 def sweep_dense(flat, n, tscale, num_replicas, seed=1, instance=0, init=None, threads=0):

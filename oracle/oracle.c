@@ -61,6 +61,7 @@ void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 enum {
   TAG_REF_INIT = 0x01,
   TAG_REF_STEP = 0x02,
+  TAG_REF_STREAM = 0x03,
   TAG_SWEEP_INIT = 0x10,
   TAG_SWEEP_THR = 0x11,
   TAG_WARM = 0x12,
@@ -355,7 +356,7 @@ static void ref_attempt(uint64_t t, uint32_t replica, const uint32_t key[2], int
 void or_anneal_literal(int n, const double *flat, const double *beta, int num_iter,
                        int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
                        uint32_t first_replica, uint8_t *best_states, double *best_energies,
-                       int threads) {
+                       int threads, uint8_t *final_states) {
   uint32_t key[2];
   seed_key(seed, key);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
@@ -388,6 +389,7 @@ void or_anneal_literal(int n, const double *flat, const double *beta, int num_it
       }
     }
     best_energies[ii] = best_energy;
+    if (final_states) memcpy(final_states + (size_t)ii * n, x, (size_t)n);
     free(x);
   }
 }
@@ -399,7 +401,7 @@ void or_anneal_literal(int n, const double *flat, const double *beta, int num_it
 void or_anneal_fields(int n, const double *flat, const double *beta, int num_iter,
                       int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
                       uint32_t first_replica, uint8_t *best_states, double *best_energies,
-                      int threads) {
+                      int threads, uint8_t *final_states) {
   uint32_t key[2];
   seed_key(seed, key);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
@@ -440,6 +442,85 @@ void or_anneal_fields(int n, const double *flat, const double *beta, int num_ite
       }
     }
     best_energies[ii] = best_energy;
+    if (final_states) memcpy(final_states + (size_t)ii * n, x, (size_t)n);
+    free(h);
+    free(x);
+  }
+}
+
+/* One shared 32-bit stream read at per-replica offsets, as the reference's
+ * engines are built (annealing.hpp:87: philox4x32x10 engine(1234, i), i.e.
+ * replica i starts i outputs into one seed's stream, device_rng.hpp:26-30).
+ * Word p of the stream is word p%4 of Philox counter (p/4 lo, p/4 hi, 0,
+ * TAG_REF_STREAM).  Each scalar draw consumes the next word:
+ *   bit()       = w >> 31                          (device_rng.hpp:46)
+ *   bit_index() = floor(w * N / 2^32)              (:55)
+ *   uniform()   = w * 2^-32, from uniform_words words, the last one used
+ *                 (:62; 1 or 2: how many outputs oneMKL's uniform<double>
+ *                 takes is not known here)
+ * and uniform() is drawn only on the branch that is not downhill (the
+ * short-circuit || of annealing.hpp:106-107).  The word-to-value transforms
+ * are this restatement's choice; the per-replica law equals
+ * or_anneal_fields', and the replicas share words as the reference's do.
+ * The num_tries replicas form independent runs of block_tries replicas each
+ * (0: one run), run b on the stream of seed + b.                            */
+static uint32_t stream_word(uint64_t p, const uint32_t key[2]) {
+  uint32_t ctr[4] = {(uint32_t)(p >> 2), (uint32_t)(p >> 34), 0u, TAG_REF_STREAM};
+  uint32_t out[4];
+  or_philox4x32_10(ctr, key, out);
+  return out[p & 3];
+}
+
+void or_anneal_stream(int n, const double *flat, const double *beta, int num_iter,
+                      int sweeps_per_beta, uint32_t num_tries, uint32_t block_tries,
+                      uint64_t seed, int uniform_words, uint8_t *best_states,
+                      double *best_energies, int threads, uint8_t *final_states) {
+  if (block_tries == 0) block_tries = num_tries;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+  for (int64_t ii = 0; ii < (int64_t)num_tries; ++ii) {
+    /* blocks of block_tries replicas, block b on the stream of seed + b */
+    uint32_t key[2];
+    seed_key(seed + (uint64_t)ii / block_tries, key);
+    uint64_t pos = (uint64_t)ii % block_tries;
+    uint8_t *x = (uint8_t *)malloc((size_t)n);
+    double *h = (double *)malloc(sizeof(double) * (size_t)n);
+    uint8_t *best = best_states + (size_t)ii * n;
+    for (int j = 0; j < n; ++j) x[j] = (uint8_t)(stream_word(pos++, key) >> 31);
+    memcpy(best, x, (size_t)n);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && x[l]) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    double best_energy = or_energy(flat, x, n);
+    double current_energy = best_energy;
+    for (int iter = 0; iter < num_iter; iter++) {
+      double b = beta[iter];
+      for (int sweep = 0; sweep < sweeps_per_beta; sweep++) {
+        const int s = (int)(((uint64_t)stream_word(pos++, key) * (uint64_t)(uint32_t)n) >> 32);
+        double de = x[s] ? -h[s] : h[s];
+        int accept = de < 0.0;
+        if (!accept) {
+          double u = 0.0;
+          for (int k = 0; k < uniform_words; ++k) u = (double)stream_word(pos++, key) * 0x1p-32;
+          accept = exp(-de / b) > u;
+        }
+        if (accept) {
+          double d = x[s] ? -1.0 : 1.0;
+          x[s] = (uint8_t)(1 - x[s]);
+          for (int l = 0; l < n; ++l)
+            if (l != s) h[l] += flat[(size_t)l * n + s] * d;
+          current_energy += de;
+        }
+        if (current_energy < best_energy) {
+          best_energy = current_energy;
+          memcpy(best, x, (size_t)n);
+        }
+      }
+    }
+    best_energies[ii] = best_energy;
+    if (final_states) memcpy(final_states + (size_t)ii * n, x, (size_t)n);
     free(h);
     free(x);
   }

@@ -84,11 +84,21 @@ int or_exhaustive(int n, const double *flat, uint8_t *state, double *energy, int
 void or_anneal_literal(int n, const double *flat, const double *beta, int num_iter,
                        int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
                        uint32_t first_replica, uint8_t *best_states,
-                       double *best_energies, int threads);
+                       double *best_energies, int threads, uint8_t *final_states);
 void or_anneal_fields(int n, const double *flat, const double *beta, int num_iter,
                       int sweeps_per_beta, uint32_t num_tries, uint64_t seed,
                       uint32_t first_replica, uint8_t *best_states,
-                      double *best_energies, int threads);
+                      double *best_energies, int threads, uint8_t *final_states);
+
+/* The reference's replicas read one stream at offsets 0, 1, 2, ...
+ * (annealing.hpp:87); this restates that sharing, with the chain of
+ * or_anneal_fields.  uniform_words: stream words per uniform() draw.
+ * Replicas come in runs of block_tries (0: one run of num_tries); run b
+ * uses the stream of seed + b.                                            */
+void or_anneal_stream(int n, const double *flat, const double *beta, int num_iter,
+                      int sweeps_per_beta, uint32_t num_tries, uint32_t block_tries,
+                      uint64_t seed, int uniform_words, uint8_t *best_states,
+                      double *best_energies, int threads, uint8_t *final_states);
 
 /* ---- sweep-mode annealers (configs with "sweeps": sequential site order) -- */
 /* tscale[t]: threshold scale of sweep t (1/beta_t for inverse-temperature

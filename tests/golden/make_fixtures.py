@@ -1,8 +1,8 @@
 """Copy the oneSolver data fixtures used by the parity tests into tests/golden/data.
 
 Data only (MIT licence, (c) hyperQ - Ewa Hendzel): example .qubo instances,
-the Chimera droplet benchmark instances in QUBO form, a subset of the
-published annealer outputs and the first line of the tensor-network
+the Chimera droplet benchmark instances in QUBO form, the published
+annealer outputs and the first line of the tensor-network
 ground-state files.  Run in a container where the oneSolver tree is mounted:
 
     python tests/golden/make_fixtures.py /root/reference
@@ -32,19 +32,21 @@ def main(ref):
             first = f.readline()
         with open(os.path.join(OUT, "chimera", f"{n}_groundstate_TN.txt"), "w") as f:
             f.write(first)
-    # Published annealer outputs: header + rows (state bits, energy, params).
-    # Keep the first 40 rows and the best-energy row of each file.
-    for n, keep in ((128, 40), (512, 16)):
+    # Published annealer outputs (results/<n>/001.csv): every row of the
+    # grid, as beta_min,num_iter,num_tries,schedule,energy,state with the
+    # state's N bits (variable 0 first) written as one hexadecimal number.
+    for n in (128, 512):
         path = os.path.join(ann, "results", str(n), "001.csv")
         with open(path) as f:
             lines = f.read().splitlines()
-        header, rows = lines[0], lines[1:]
-        best = min(rows, key=lambda r: float(r.split(",")[n]))
-        chosen = rows[:keep] + ([best] if best not in rows[:keep] else [])
-        with open(os.path.join(OUT, "chimera", f"{n}_results_subset.csv"), "w") as f:
-            f.write(header + "\n")
-            for r in chosen:
-                f.write(r + "\n")
+        with open(os.path.join(OUT, "chimera", f"{n}_results.csv"), "w") as f:
+            f.write("beta_min,num_iter,num_tries,schedule,energy,state\n")
+            for r in lines[1:]:
+                t = r.split(",")
+                bits = "".join(t[:n])
+                assert len(t) == n + 5 and set(bits) <= {"0", "1"}
+                state = format(int(bits, 2), f"0{n // 4}x")
+                f.write(",".join([t[n + 1], t[n + 2], t[n + 3], t[n + 4], t[n], state]) + "\n")
 
 
 def kat_instances(ref):

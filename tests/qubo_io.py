@@ -25,3 +25,24 @@ def load_qubo(path):
         q[i, j] += v
         q[j, i] += v
     return q
+
+
+def published_grid(n):
+    """Rows of the reference's published annealer runs on the N=n Chimera
+    instance (benchmarks/annealing/results/<n>/001.csv, committed as
+    golden/data/chimera/<n>_results.csv): dicts with beta_min, num_iter,
+    num_tries, schedule, the printed energy and the returned state."""
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "data", "chimera",
+                        f"{n}_results.csv")
+    rows = []
+    with open(path) as f:
+        next(f)
+        for line in f:
+            bmin, it, k, sch, e, hx = line.strip().split(",")
+            bits = format(int(hx, 16), f"0{n}b")
+            rows.append({"beta_min": float(bmin), "num_iter": int(it), "num_tries": int(k),
+                         "schedule": sch, "energy": float(e),
+                         "state": np.frombuffer(bits.encode(), dtype=np.uint8) - ord("0")})
+    return rows

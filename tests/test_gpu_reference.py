@@ -111,3 +111,29 @@ def test_cli_gpu_device_end_to_end(tmp_path):
     head, row = out.read_text().splitlines()
     assert head.endswith(",energy")
     assert float(row.split(",")[-1]) == -32.0
+
+
+def test_reference_anneal_published_runs_band(ctx):
+    """The GPU's reference-mode chains against the reference's published
+    runs (benchmarks/annealing/results/128/001.csv, beta 0.1 -> 10 as a
+    temperature): every published best-of-10 energy lies inside the 0.5-99.5%
+    band of the GPU's best-of-10 over 2000 blocks of independent replicas."""
+    import published_band as pb
+    from qubo_io import published_grid
+
+    n = 128
+    q = load_qubo(os.path.join(GOLDEN, "chimera", f"{n}_001.qubo"))
+    pts = pb.published_points(published_grid(n))
+    cells = [("linear", 1000), ("geometric", 500), ("geometric", 1000)]
+
+    def sample(kind, it, count):
+        beta = pb.schedule(oracle, kind, it)
+        print(f"[gpu] reference anneal n={n} {kind} {it} x {count}", flush=True)
+        _, _, states, e = ctx.anneal_reference(q, beta, count, seed=2024)
+        assert np.all(np.abs(e - oracle.energy(q, states)) <= 1e-9 * np.abs(e))
+        return e
+
+    pits = pb.point_pits(sample, pts, cells, (10,), 2000)
+    ok, outside, mean = pb.band_verdict(pits)
+    print(f"[gpu] published band: mean PIT {mean:.3f} {pits}", flush=True)
+    assert ok, (outside, mean)

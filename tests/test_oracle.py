@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from conftest import GOLDEN
-from qubo_io import load_qubo
+from qubo_io import load_qubo, published_grid
 
 # --------------------------------------------------------------------------
 # Philox4x32-10
@@ -140,12 +140,9 @@ def test_exhaustive_ties_go_to_lowest_index():
 
 
 def _published_rows(n):
-    path = os.path.join(GOLDEN, "chimera", f"{n}_results_subset.csv")
-    with open(path) as f:
-        lines = f.read().splitlines()[1:]
-    for line in lines:
-        t = line.split(",")
-        yield np.array([int(v) for v in t[:n]], dtype=np.uint8), float(t[n])
+    """(state, printed energy) of every row of results/<n>/001.csv."""
+    for row in published_grid(n):
+        yield row["state"], row["energy"]
 
 
 @pytest.mark.parametrize("n", [128, 512])
@@ -159,7 +156,7 @@ def test_energy_matches_published_annealer_outputs(n):
         tol = 0.5 * 10 ** (np.floor(np.log10(abs(e))) - 5)
         assert abs(e - printed) <= tol * 1.0001
         count += 1
-    assert count >= 17
+    assert count == 1200
 
 
 @pytest.mark.parametrize("n,expected", [(128, -235.866654), (512, -928.746690),
