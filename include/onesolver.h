@@ -72,6 +72,31 @@ int os_anneal_reference(os_ctx *ctx, int n, const double *q_sym, const double *b
                         uint8_t *best_states, double *best_energies, uint8_t *best_state,
                         double *best_energy);
 
+/* Bytes of the fixed per-GPU scratch array of os_anneal_reference_noalloc. */
+#define OS_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* os_anneal_reference for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The same chains, energies
+ * and tie rule as os_anneal_reference, with every array in device memory of
+ * the context's GPU: d_q_sym (n x n FP64) and d_beta (num_iter FP64) are read;
+ * the outputs, each optional, are d_best_states [num_tries x n] (0/1 bytes),
+ * d_best_energies [num_tries] (FP64, recomputed from the states) and
+ * d_best_index [1] (int32: the replica of the lowest energy, first index on
+ * ties).  Everything else lives in a fixed scratch array of
+ * OS_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: the call
+ * makes no hipMalloc or hipFree.  It returns OS_ERR_UNSUPPORTED when the run
+ * needs more scratch than that: 4*ceil(n/32) + 16 + 8*ceil(n/64) bytes per
+ * try (each part rounded up to 256 bytes), plus 8*n per try for n > 4096.
+ * All calls on one GPU share its scratch, so calls that overlap on one GPU,
+ * from any context, race.  Synchronous, like the other entry points. */
+int os_anneal_reference_noalloc(os_ctx *ctx, int n, const double *d_q_sym, const double *d_beta,
+                                int num_iter, int sweeps_per_beta, uint32_t num_tries,
+                                uint64_t seed, uint8_t *d_best_states, double *d_best_energies,
+                                int32_t *d_best_index);
+/* OS_NOALLOC_SCRATCH_BYTES (returns the size, not a status). */
+size_t os_noalloc_scratch_bytes(void);
+
 /* FP64 QUBO energies of `count` states (0/1 bytes, count x n).  Replaces
  * sa::energy (include/simulated_annealing/annealing.hpp:31-40). */
 int os_energy_fp64(os_ctx *ctx, int n, const double *q_sym, uint32_t count,

@@ -52,6 +52,7 @@ Ctx::Ctx(int dev) : device(dev) {
   OS_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   OS_HIP_CHECK(hipEventCreate(&ev_start));
   OS_HIP_CHECK(hipEventCreate(&ev_stop));
+  noalloc_setup(dev);
 }
 
 Ctx::~Ctx() {

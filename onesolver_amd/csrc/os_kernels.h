@@ -19,6 +19,10 @@ void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count
                         const uint32_t *bits, double *out, double *scratch, float *fields, int ldf,
                         unsigned int *drift_max, int num_instances = 1);
 
+// capi_noalloc.hip: looks up this device's address of the fixed scratch
+// array (once per device; the device must be current)
+void noalloc_setup(int device);
+
 // anneal_ref.hip
 int ref_lds_limit();
 size_t ref_lds_bytes(int n);
