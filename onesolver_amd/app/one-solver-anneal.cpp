@@ -12,7 +12,14 @@
  *
  * Extensions (optional, reference-compatible defaults): --seed (the
  * reference's fixed engine seed, 1234) and --sweeps-per-beta (the library
- * parameter of sa::anneal that the reference CLI leaves at 1).
+ * parameter of sa::anneal that the reference CLI leaves at 1).  The fast
+ * path is opt-in: --mode sweep runs sa::anneal_sweeps (sweep.hpp) on the gpu
+ * device -- num-iter beta values, each for sweeps-per-beta sweeps of N
+ * attempts in site order -- with --beta-kind (temperature, the reference's
+ * use of beta, or inverse), --path (auto, dense or sparse),
+ * --refresh-interval (dense: sweeps between FP64 field refreshes) and
+ * --warm-start (a file of trained warm-start network parameters).  In the
+ * default --mode reference the output is the reference's, line for line.
  */
 #include <cstdlib>
 #include <fstream>
@@ -29,6 +36,7 @@
 #include "onesolver/model/solution.hpp"
 #include "onesolver/simulated_annealing/annealing.hpp"
 #include "onesolver/simulated_annealing/schedules.hpp"
+#include "onesolver/simulated_annealing/sweep.hpp"
 
 namespace {
 
@@ -49,12 +57,22 @@ const char *kHelp =
     "\n"
     "Extensions:\n"
     "  --seed arg (=1234)                    seed of the Philox4x32-10 streams\n"
-    "  --sweeps-per-beta arg (=1)            attempts per beta value\n";
+    "  --sweeps-per-beta arg (=1)            attempts (sweep mode: sweeps) per beta\n"
+    "                                        value\n"
+    "  --mode arg (=reference)               reference (random-site attempts, as\n"
+    "                                        the reference) or sweep (gpu only)\n"
+    "  --beta-kind arg (=temperature)        sweep mode: beta as a temperature (as\n"
+    "                                        the reference) or inverse\n"
+    "  --path arg (=auto)                    sweep mode: auto, dense or sparse\n"
+    "  --refresh-interval arg (=0)           sweep mode, dense: sweeps between FP64\n"
+    "                                        field refreshes (0: at the end)\n"
+    "  --warm-start arg                      sweep mode: file of warm-start network\n"
+    "                                        parameters (fp32)\n";
 
-const std::vector<std::string> kNames = {"help",     "input",       "output",
-                                         "num-iter", "num-tries",   "schedule-type",
-                                         "beta-min", "beta-max",    "device-type",
-                                         "seed",     "sweeps-per-beta"};
+const std::vector<std::string> kNames = {
+    "help",        "input",           "output",   "num-iter",  "num-tries", "schedule-type",
+    "beta-min",    "beta-max",        "device-type", "seed",   "sweeps-per-beta", "mode",
+    "beta-kind",   "path",            "refresh-interval", "warm-start"};
 
 }  // namespace
 
@@ -76,6 +94,11 @@ int main(int argc, char *argv[]) {
     device_type = cli::convert<std::string>(opt, "device-type", "host");
     const uint64_t seed = cli::convert<uint64_t>(opt, "seed", 1234u);
     const unsigned sweeps_per_beta = cli::convert<unsigned>(opt, "sweeps-per-beta", 1u);
+    const std::string mode = cli::convert<std::string>(opt, "mode", "reference");
+    const std::string beta_kind = cli::convert<std::string>(opt, "beta-kind", "temperature");
+    const std::string path = cli::convert<std::string>(opt, "path", "auto");
+    const int refresh_interval = cli::convert<int>(opt, "refresh-interval", 0);
+    const std::string warm_file = cli::convert<std::string>(opt, "warm-start", "");
 
     if (!opt.values.count("input")) {
       std::cerr << "No input file provided." << std::endl;
@@ -105,6 +128,33 @@ int main(int argc, char *argv[]) {
     }
     if (sweeps_per_beta < 1) {
       std::cerr << "Invalid sweeps per beta, need at least 1" << std::endl;
+      return -1;
+    }
+    if (mode != "reference" && mode != "sweep") {
+      std::cerr << "Unknown mode: " << mode << std::endl;
+      return -1;
+    }
+    if (beta_kind != "temperature" && beta_kind != "inverse") {
+      std::cerr << "Unknown beta kind: " << beta_kind << std::endl;
+      return -1;
+    }
+    if (path != "auto" && path != "dense" && path != "sparse") {
+      std::cerr << "Unknown path: " << path << std::endl;
+      return -1;
+    }
+    if (refresh_interval < 0) {
+      std::cerr << "Invalid refresh interval, need 0 or more" << std::endl;
+      return -1;
+    }
+    const bool sweep_only = opt.values.count("beta-kind") || opt.values.count("path") ||
+                            opt.values.count("refresh-interval") || !warm_file.empty();
+    if (mode == "reference" && sweep_only) {
+      std::cerr << "--beta-kind, --path, --refresh-interval and --warm-start need --mode sweep"
+                << std::endl;
+      return -1;
+    }
+    if (mode == "sweep" && device_type != "gpu") {
+      std::cerr << "Sweep mode runs on the gpu device" << std::endl;
       return -1;
     }
 
@@ -140,8 +190,27 @@ int main(int argc, char *argv[]) {
         construct_geometric_beta_schedule(beta_schedule, beta_min, beta_max, num_iter);
     }
 
-    auto solution = sa::anneal(instance, *device, beta_schedule, (int)num_iter, num_tries,
-                               (int)sweeps_per_beta, seed);
+    auto run = [&]() {
+      if (mode != "sweep")
+        return sa::anneal(instance, *device, beta_schedule, (int)num_iter, num_tries,
+                          (int)sweeps_per_beta, seed);
+      sa::SweepOptions so;
+      so.beta_kind =
+          beta_kind == "temperature" ? sa::BetaKind::temperature : sa::BetaKind::inverse;
+      so.path = path == "dense"    ? sa::SweepPath::dense
+                : path == "sparse" ? sa::SweepPath::sparse
+                                   : sa::SweepPath::automatic;
+      so.seed = seed;
+      so.refresh_interval = refresh_interval;
+      if (!warm_file.empty()) so.warm_params = sa::load_warm_params(warm_file);
+      sa::SweepReport rep;
+      auto sol = sa::anneal_sweeps(instance, *device, beta_schedule, (int)num_iter, num_tries,
+                                   (int)sweeps_per_beta, so, &rep);
+      std::cout << "Mode: sweep (" << rep.path << " path, " << rep.attempts << " attempts, "
+                << rep.kernel_ms << " ms in the sweep kernels)" << std::endl;
+      return sol;
+    };
+    auto solution = run();
 
     std::ofstream results_file(output_file);
     solution.save(results_file);

@@ -82,6 +82,30 @@ def test_cli_validation_returns_minus_one(args, msg):
     assert msg in r.stderr
 
 
+@pytest.mark.parametrize("args,msg", [
+    (["--path", "dense"], "--beta-kind, --path, --refresh-interval and --warm-start need --mode sweep"),
+    (["--warm-start", "w.bin"], "need --mode sweep"),
+    (["--mode", "sweeps"], "Unknown mode: sweeps"),
+    (["--mode", "sweep"], "Sweep mode runs on the gpu device"),
+    (["--mode", "sweep", "--device-type", "cpu"], "Sweep mode runs on the gpu device"),
+    (["--mode", "sweep", "--device-type", "gpu", "--path", "csr"], "Unknown path: csr"),
+    (["--mode", "sweep", "--device-type", "gpu", "--beta-kind", "x"], "Unknown beta kind: x"),
+    (["--mode", "sweep", "--device-type", "gpu", "--refresh-interval", "-2"],
+     "Invalid refresh interval, need 0 or more"),
+])
+def test_cli_sweep_mode_validation(args, msg):
+    r = _run(["--input", "x", "--output", "y"] + args)
+    assert r.returncode == 255
+    assert msg in r.stderr
+
+
+def test_cli_help_lists_sweep_mode_options():
+    r = _run(["--help"])
+    for opt in ["--mode arg (=reference)", "--beta-kind arg (=temperature)", "--path arg (=auto)",
+                "--refresh-interval arg (=0)", "--warm-start arg"]:
+        assert opt in r.stdout
+
+
 def test_cli_parse_errors_exit_one(tmp_path):
     r = _run(["--bogus", "1"])
     assert r.returncode == 1 and "error: unrecognised option '--bogus'" in r.stderr
