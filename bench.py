@@ -42,6 +42,7 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
+TRAFFIC_CSR_C3_COLOUR = None  # csr_sweep_col_kernel: not measured yet
 TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
 TRAFFIC_INGEST_8192 = (2 * 643639.7 + 558661.2) * 1024   # parse_lines_kernel<true>, N=8192 text
 
@@ -306,7 +307,9 @@ def cpu_baseline_csr(inst, ts_full, budget_s=10.0):
 def run_csr_c3(args):
     """Config 3: 512 sparse d-regular instances, n log-uniform in [64, 2048],
     256 replicas each, 2048 sweeps, one persistent kernel replayed from a
-    hipGraph with the tuned launch shape."""
+    hipGraph with the tuned launch shape.  --c3-order colour (default) visits
+    each instance's sites class by class of a greedy colouring, several
+    independent sites per wave step; sequential visits them 0..n-1."""
     import onesolver_amd
     from onesolver_amd import instances
 
@@ -317,13 +320,17 @@ def run_csr_c3(args):
     ctx = onesolver_amd.Context(0)
     log(f"device {ctx.device_name()}")
     b = onesolver_amd.CsrBatch(ctx, insts)
+    order = 1 if args.c3_order == "colour" else 0
+    shape = b.colour_shape if order else b.launch_shape
     for i in range(args.warmup):
-        out = b.anneal(R, ts, seed=1234, best_states=False, tune=not args.c3_no_tune)
-        log(f"warmup {i}: graph {out['stats']['kernel_ms']:.1f} ms shape {b.launch_shape()}")
+        out = b.anneal(R, ts, seed=1234, best_states=False, tune=not args.c3_no_tune,
+                       site_order=order)
+        log(f"warmup {i}: graph {out['stats']['kernel_ms']:.1f} ms shape {shape()}")
     kernel_ms = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
-        out = b.anneal(R, ts, seed=1234 + i, best_states=False, tune=not args.c3_no_tune)
+        out = b.anneal(R, ts, seed=1234 + i, best_states=False, tune=not args.c3_no_tune,
+                       site_order=order)
         kernel_ms += out["stats"]["kernel_ms"]
         log(f"step {i}: graph {out['stats']['kernel_ms']:.1f} ms")
     elapsed = time.perf_counter() - t0
@@ -341,14 +348,17 @@ def run_csr_c3(args):
         "data": "synthetic (seeded random d-regular graphs)",
         "config": {"workload": f"batched sparse: 512 instances n in [64,2048] log-uniform, "
                                f"d-regular d in 3..8, weights +-1, {R} replicas, {S} sweeps, "
-                               f"persistent kernel + hipGraph", "instances": 512,
-                   "replicas": R, "sweeps": S, "sum_n": int(ns.sum()),
-                   "launch_shape": list(b.launch_shape()[:2]),
-                   "row_prefetch": bool(b.launch_shape()[2])},
+                               f"persistent kernel + hipGraph, site order {args.c3_order}",
+                   "instances": 512, "replicas": R, "sweeps": S, "sum_n": int(ns.sum()),
+                   "site_order": args.c3_order, "launch_shape": list(shape()),
+                   "launch_shape_fields": ["waves per group", "groups",
+                                           "sites per step" if order else "row prefetch"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": TRAFFIC_CSR_C3 if S == 2048 and b.launch_shape()[2] else None,
-                     "kernel": "csr_sweep_kernel", "avg_launch_ms": avg_launch_s * 1e3,
+                     "traffic": (TRAFFIC_CSR_C3_COLOUR if order else TRAFFIC_CSR_C3)
+                     if S == 2048 else None,
+                     "kernel": "csr_sweep_col_kernel" if order else "csr_sweep_pf_kernel",
+                     "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_alg},
     }
     imax = int(np.argmax(ns))
@@ -389,6 +399,9 @@ def run_warmstart_c4(args):
         "plain_time_to_99_s": plain["wall_s"] if plain else None,
         "speedup_vs_plain": (plain["wall_s"] / learned["wall_s"]) if plain and learned else None,
         "random_init_warm_schedule_s": (res["random_warm_schedule"][1] or {}).get("wall_s"),
+        "speedup_vs_random_same_schedule": (
+            res["random_warm_schedule"][1]["wall_s"] / learned["wall_s"]
+            if learned and res["random_warm_schedule"][1] else None),
         "untrained_net_s": (res["untrained_net"][1] or {}).get("wall_s"),
         "heldout_bce": heldout, "uninformed_bce": float(np.log(2.0)),
         "train_s": t_train, "label_s": t_labels, "infer_ms": res["t_infer_s"] * 1e3,
@@ -753,6 +766,7 @@ def main():
     ap.add_argument("--c4-epochs", type=int, default=2)
     ap.add_argument("--c4-eval", type=int, default=32)
     ap.add_argument("--sweeps-c3", type=int, default=2048)
+    ap.add_argument("--c3-order", choices=["colour", "sequential"], default="colour")
     ap.add_argument("--c3-no-tune", action="store_true",
                     help="skip the launch-shape tuner (library default shape)")
     ap.add_argument("--n", type=int, default=1024)

@@ -264,6 +264,10 @@ typedef struct os_csr_params {
   const uint8_t *init_states;  /* optional start states, per instance [num_replicas x n[i]] */
   int32_t use_graph;           /* replay the run from a captured hipGraph */
   int32_t tune;                /* pick the launch shape by timing candidates on first use */
+  int32_t site_order;          /* 0: every sweep visits sites 0..n-1 in order; 1: class by
+                                  class of a greedy colouring (os_csr_batch_site_order), so a
+                                  wave updates several independent sites of its replicas at
+                                  once; site j keeps its threshold stream either way */
 } os_csr_params;
 
 int os_csr_batch_create(os_ctx *ctx, int32_t num_instances, const int32_t *n,
@@ -284,6 +288,18 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
  * rows have at most 8 couplings), 0 for the general one. */
 int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
                               int32_t *prefetch);
+/* The colour-class order's launch shape in use (0 before its first run):
+ * a wave updates sites_per_step sites of each of its 64/sites_per_step
+ * replicas per step. */
+int os_csr_batch_colour_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
+                              int32_t *sites_per_step);
+/* The site order of site_order 1 for one instance (outputs optional):
+ * order [n] lists the sites class by class, each class ascending;
+ * class_start [num_classes + 1] bounds the classes.  Greedy colouring in
+ * index order: site j takes the smallest colour held by none of its
+ * neighbours l < j, so no two sites of a class are coupled. */
+int os_csr_batch_site_order(os_csr_batch *batch, int32_t instance, int32_t *order,
+                            int32_t *class_start, int32_t *num_classes);
 
 /* ---- .qubo instances: text and binary ingest on the GPU ----
  *

@@ -517,7 +517,8 @@ class CsrParams(ctypes.Structure):
     """os_csr_params of include/onesolver.h."""
     _fields_ = [("num_replicas", ctypes.c_uint32), ("num_sweeps", ctypes.c_int32),
                 ("tscale", _vp), ("seed", ctypes.c_uint64), ("instance_base", ctypes.c_uint32),
-                ("init_states", _vp), ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32)]
+                ("init_states", _vp), ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32),
+                ("site_order", ctypes.c_int32)]
 
 
 def _csr_lib():
@@ -529,6 +530,8 @@ def _csr_lib():
         L.os_csr_batch_anneal.argtypes = [_vp, ctypes.POINTER(CsrParams), _f64p, _f64p, _u8p,
                                           _u8p, _f64p, _i32p, ctypes.POINTER(SweepStats)]
         L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
+        L.os_csr_batch_colour_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
+        L.os_csr_batch_site_order.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]
         L._csr_ready = True
     return L
 
@@ -572,10 +575,30 @@ class CsrBatch:
                                                     ctypes.byref(r)))
         return w.value, g.value, r.value
 
+    def colour_shape(self):
+        """(waves per work-group, resident work-groups, sites per step) of the
+        colour-class order; zeros before its first run."""
+        w, g, s = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(_csr_lib().os_csr_batch_colour_shape(self._h, ctypes.byref(w), ctypes.byref(g),
+                                                    ctypes.byref(s)))
+        return w.value, g.value, s.value
+
+    def site_order(self, instance):
+        """(order [n], class starts) of the colour-class order of an instance."""
+        n = int(self.ns[instance])
+        k = ctypes.c_int32(0)
+        _check(_csr_lib().os_csr_batch_site_order(self._h, instance, None, None, ctypes.byref(k)))
+        order = np.zeros(n, dtype=np.int32)
+        cs = np.zeros(k.value + 1, dtype=np.int32)
+        _check(_csr_lib().os_csr_batch_site_order(self._h, instance, _p(order, _i32p),
+                                                  _p(cs, _i32p), None))
+        return order, cs
+
     def anneal(self, num_replicas, tscale, seed=1, instance_base=0, init_states=None,
-               best_states=True, final_states=False, use_graph=True, tune=True):
+               best_states=True, final_states=False, use_graph=True, tune=True, site_order=0):
         """Returns per-instance lists of best/final states ([R, n_i] uint8) and
-        [I, R] arrays of FP64 best energies and running-best energies."""
+        [I, R] arrays of FP64 best energies and running-best energies.
+        site_order 1: sweeps visit the sites class by class (site_order())."""
         ts = np.ascontiguousarray(tscale, dtype=np.float32)
         I, R = len(self.ns), int(num_replicas)
         ini = None
@@ -584,7 +607,8 @@ class CsrBatch:
                 [np.asarray(s, dtype=np.uint8).reshape(R, n).ravel()
                  for s, n in zip(init_states, self.ns)]))
         prm = CsrParams(R, len(ts), ts.ctypes.data, seed, instance_base,
-                        None if ini is None else ini.ctypes.data, int(use_graph), int(tune))
+                        None if ini is None else ini.ctypes.data, int(use_graph), int(tune),
+                        int(site_order))
         tot = int(self.ns.sum()) * R
         be = np.zeros((I, R), dtype=np.float64)
         rb = np.zeros((I, R), dtype=np.float64)

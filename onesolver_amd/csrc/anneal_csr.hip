@@ -23,6 +23,14 @@ namespace os {
 
 namespace {
 
+// This wave's LDS writes are done before its later LDS reads (other lanes
+// read what a lane wrote).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Bit j of replica lane in a wave's transposed planes.
 __device__ __forceinline__ uint32_t plane_bit(const uint32_t *xs, int j, int lane) {
   return (xs[(j >> 5) * 64 + lane] >> (j & 31)) & 1u;
@@ -299,6 +307,134 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
   }
 }
 
+// Colour-class variant.  The sites of an instance are visited class by class
+// (greedy colouring: no two sites of a class are coupled), every class in
+// ascending index order; within a class the updates are independent, so a
+// wave updates G sites of each of its 64/G replicas per step: lane =
+// replica slot * G + site slot.  Site j keeps its threshold (Philox counter
+// (j>>2, r, t)), its field order and its dE, so the chain is the sequential
+// sweep's with the order of the class list; the FP64 running energy adds the
+// dE of a sweep per lane and then over the G lanes of a replica.  Flips are
+// atomic XORs on the LDS planes, since sites of one class can share a word.
+template <int kW, int G>
+__global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
+    const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
+    const float4 *__restrict__ ell_val, const float *__restrict__ diag,
+    const int32_t *__restrict__ corder, const int32_t *__restrict__ cstart,
+    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
+    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
+    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
+    int max_words) {
+  static_assert(G == 4 || G == 8 || G == 16, "sites per step");
+  constexpr int RW = 64 / G;  // replicas per wave
+  extern __shared__ uint32_t lds[];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int rr = lane / G, s = lane % G;
+  uint32_t *xs = lds + (size_t)w * max_words * RW;
+  const int num_items = run->num_items, S = run->num_sweeps, Rp = run->Rp;
+  const uint32_t k0 = run->k0, k1 = run->k1;
+  const uint32_t base = run->instance_base;
+  const int init_mode = run->init_mode;
+  auto bit = [&](int j) { return (xs[(j >> 5) * RW + rr] >> (j & 31)) & 1u; };
+  for (;;) {
+    int it = 0;
+    if (lane == 0) it = atomicAdd(queue, 1);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= num_items) break;
+    const int2 wi = items[it];
+    const CsrInst d = inst[wi.x];
+    const int n = d.n, words = d.words, chunks = d.width >> 2, ncol = d.ncol;
+    const uint32_t r = (uint32_t)(wi.y * RW + rr);
+    const uint32_t id = (base + (uint32_t)wi.x) << 8;
+    const int4 *ecol = ell_col + (d.ell_off >> 2);
+    const float4 *eval = ell_val + (d.ell_off >> 2);
+    const float *dg = diag + d.spin_off;
+    const int32_t *ord = corder + d.spin_off;
+    const int32_t *cs = cstart + d.class_off;
+    uint32_t *bb = best_bits + d.state_off + r;
+    // start state, as in csr_sweep_kernel; the G lanes of a replica share
+    // its words
+    for (int wd = s; wd < words; wd += G) {
+      uint32_t v;
+      if (init_mode) {
+        v = init_bits[d.state_off + (size_t)wd * Rp + r];
+      } else {
+        const uint4 o = philox4x32_10(make_uint4((uint32_t)(wd >> 2), r, 0u, id | TAG_SWEEP_INIT),
+                                      k0, k1);
+        v = pick(o, (uint32_t)wd & 3u);
+      }
+      const int valid = n - 32 * wd;
+      if (valid < 32) v = valid <= 0 ? 0u : (v & ((1u << valid) - 1u));
+      xs[wd * RW + rr] = v;
+    }
+    wave_sync();
+    // FP64 energy in the oracle's order (every lane of a replica computes it)
+    double e = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double off = 0.0;
+      for (int c = 0; c < chunks; ++c) {
+        const int4 cc = ecol[(size_t)j * chunks + c];
+        const float4 vv = eval[(size_t)j * chunks + c];
+        off = bit(cc.x) ? off + (double)vv.x : off;
+        off = bit(cc.y) ? off + (double)vv.y : off;
+        off = bit(cc.z) ? off + (double)vv.z : off;
+        off = bit(cc.w) ? off + (double)vv.w : off;
+      }
+      if (bit(j)) e += (double)dg[j] + 0.5 * off;
+    }
+    double eb = e;
+    for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
+    for (int t = 0; t < S; ++t) {
+      const float ts = tscale[t];
+      double de_sum = 0.0;
+      for (int c = 0; c < ncol; ++c) {
+        const int c1 = cs[c + 1];
+        for (int p0 = cs[c]; p0 < c1; p0 += G) {
+          const int pos = p0 + s;
+          const bool active = pos < c1;
+          const int j = active ? ord[pos] : 0;
+          const uint4 o = philox4x32_10(
+              make_uint4((uint32_t)(j >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+          const float thr = neg_ln_u(pick(o, (uint32_t)j & 3u)) * ts;
+          float h = dg[j];
+          for (int q = 0; q < chunks; ++q) {
+            const int4 cc = ecol[(size_t)j * chunks + q];
+            const float4 vv = eval[(size_t)j * chunks + q];
+            h = bit(cc.x) ? h + vv.x : h;
+            h = bit(cc.y) ? h + vv.y : h;
+            h = bit(cc.z) ? h + vv.z : h;
+            h = bit(cc.w) ? h + vv.w : h;
+          }
+          const float de = bit(j) ? -h : h;
+          if (active && de < thr) {
+            atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
+            de_sum += (double)de;
+          }
+        }
+        // the next class reads this class's flips
+        wave_sync();
+      }
+#pragma unroll
+      for (int off = 1; off < G; off <<= 1) de_sum += __shfl_xor(de_sum, off, 64);
+      e += de_sum;
+      if (e < eb) {
+        eb = e;
+        for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
+      }
+    }
+    uint32_t *fb = final_bits + d.state_off + r;
+    for (int wd = s; wd < words; wd += G) fb[(size_t)wd * Rp] = xs[wd * RW + rr];
+    if (s == 0) {
+      e_best[d.rep_off + r] = eb;
+      e_final[d.rep_off + r] = e;
+    }
+    // the next item's start state overwrites these planes
+    wave_sync();
+  }
+}
+
 // FP64 energy of states given as planes: E = sum_j x_j (Q_jj + sum_{l>j} Q_jl x_l),
 // the reference's energy function (upper triangle incl. diagonal), with the
 // double-precision couplings of the CSR input.  Thread = (item, lane).
@@ -413,6 +549,72 @@ int csr_max_groups_per_cu(bool pf, int waves, int max_words) {
   }
   OS_HIP_CHECK(e);
   return blocks;
+}
+
+int csr_col_lds_bytes(int waves, int sites, int max_words) {
+  return waves * max_words * (64 / sites) * 4;
+}
+
+namespace {
+
+template <int kW, int G>
+const void *col_kernel() {
+  return (const void *)csr_sweep_col_kernel<kW, G>;
+}
+
+const void *col_kernel_for(int waves, int sites) {
+#define OS_COL_PICK(W, GG) \
+  if (waves == W && sites == GG) return col_kernel<W, GG>();
+  OS_COL_PICK(1, 4)
+  OS_COL_PICK(1, 8)
+  OS_COL_PICK(1, 16)
+  OS_COL_PICK(2, 4)
+  OS_COL_PICK(2, 8)
+  OS_COL_PICK(2, 16)
+  OS_COL_PICK(4, 4)
+  OS_COL_PICK(4, 8)
+  OS_COL_PICK(4, 16)
+#undef OS_COL_PICK
+  throw Error(-3, "csr colour sweep kernel: unsupported shape");
+}
+
+}  // namespace
+
+int csr_col_max_groups_per_cu(int waves, int sites, int max_words) {
+  const size_t lds = (size_t)csr_col_lds_bytes(waves, sites, max_words);
+  int blocks = 0;
+  OS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &blocks, col_kernel_for(waves, sites), 64 * waves, lds));
+  return blocks;
+}
+
+void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
+                           const CsrInst *inst, const int2 *items, const CsrRun *run,
+                           const int4 *ell_col, const float4 *ell_val, const float *diag,
+                           const int32_t *corder, const int32_t *cstart, const float *tscale,
+                           const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
+                           double *e_best, double *e_final, int *queue) {
+  const size_t lds = (size_t)csr_col_lds_bytes(waves, sites, max_words);
+  const void *k = col_kernel_for(waves, sites);
+  if (lds > 65536)
+    OS_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+#define OS_COL_CASE(W, GG)                                                                      \
+  if (waves == W && sites == GG) {                                                              \
+    csr_sweep_col_kernel<W, GG><<<groups, 64 * W, lds, st>>>(                                   \
+        inst, items, run, ell_col, ell_val, diag, corder, cstart, tscale, init_bits, best_bits, \
+        final_bits, e_best, e_final, queue, max_words);                                         \
+  }
+  OS_COL_CASE(1, 4)
+  OS_COL_CASE(1, 8)
+  OS_COL_CASE(1, 16)
+  OS_COL_CASE(2, 4)
+  OS_COL_CASE(2, 8)
+  OS_COL_CASE(2, 16)
+  OS_COL_CASE(4, 4)
+  OS_COL_CASE(4, 8)
+  OS_COL_CASE(4, 16)
+#undef OS_COL_CASE
+  OS_HIP_CHECK(hipGetLastError());
 }
 
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,

@@ -7,9 +7,13 @@
 // kernel, then recomputes the FP64 energy of every best state; the three
 // launches (counter reset, sweeps, energies) are captured once into a
 // hipGraph and replayed, the run parameters living in device memory.  The
-// launch shape (waves per work-group, resident work-groups) is picked by
-// timing short runs of the candidates on the batch itself.
+// launch shape (waves per work-group, resident work-groups and, for the
+// colour-class site order, sites per wave step) is picked by timing short
+// runs of the candidates on the batch itself.  Every instance is coloured
+// greedily when the batch is created (index order, smallest colour free of
+// lower-index neighbours) for the colour-class site order.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -29,6 +33,10 @@ struct CsrBatch {
   int64_t total_spins = 0;
   std::vector<CsrInst> inst;
   DevBuf d_inst, ell_col, ell_val, diag, diag64, row_ptr, col, val64;
+  // colour classes: sites class by class (offset spin_off), class starts
+  // (offset class_off)
+  DevBuf corder, cstart;
+  std::vector<int32_t> h_order, h_cstart;
   // per run shape
   int Rp = 0;
   DevBuf items, run, tscale, init, best, fin, eb, ef, e64, queue;
@@ -37,10 +45,18 @@ struct CsrBatch {
   int waves = 0, groups = 0;
   bool reg = false;
   int max_width = 0;
+  // colour-class site order: its launch shape (0: not tuned) and work items
+  // ((instance, group of 64/sites replicas), for sites = items_sites)
+  int c_waves = 0, c_groups = 0, c_sites = 0;
+  DevBuf items_c;
+  int num_items_c = 0, items_sites = 0, items_c_Rp = 0;
+  std::vector<int> order_by_work;  // instances, largest first
+  // the run that enqueue() launches: 0 sequential, 1 colour classes
+  int mode = 0;
   // captured graph and what it was captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int g_waves = 0, g_groups = 0, g_Rp = 0;
+  int g_waves = 0, g_groups = 0, g_Rp = 0, g_mode = 0, g_sites = 0;
   bool g_reg = false;
   const void *g_tscale = nullptr;
 
@@ -53,17 +69,30 @@ struct CsrBatch {
   ~CsrBatch() {
     drop_graph();
     for (auto *b : {&d_inst, &ell_col, &ell_val, &diag, &diag64, &row_ptr, &col, &val64, &items,
-                    &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue})
+                    &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &corder,
+                    &cstart, &items_c})
       b->release();
   }
 
-  void enqueue(bool rg, int w, int g, bool energy) {
+  // mode 0: sequential sweeps with shape (rg, w, g); mode 1: colour classes
+  // with w waves, g work-groups and `sites` sites per step (items_c built
+  // for that many sites)
+  void enqueue(bool rg, int w, int g, bool energy, int sites = 0) {
     hipStream_t st = ctx->stream;
     launch_csr_queue_reset(st, queue.as<int>());
-    launch_csr_sweeps(st, rg, w, g, max_words, d_inst.as<CsrInst>(), items.as<int2>(),
-                      run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(), diag.as<float>(),
-                      tscale.as<float>(), init.as<uint32_t>(), best.as<uint32_t>(),
-                      fin.as<uint32_t>(), eb.as<double>(), ef.as<double>(), queue.as<int>());
+    if (mode == 1)
+      launch_csr_col_sweeps(st, w, sites, g, max_words, d_inst.as<CsrInst>(), items_c.as<int2>(),
+                            run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(),
+                            diag.as<float>(), corder.as<int32_t>(), cstart.as<int32_t>(),
+                            tscale.as<float>(), init.as<uint32_t>(), best.as<uint32_t>(),
+                            fin.as<uint32_t>(), eb.as<double>(), ef.as<double>(),
+                            queue.as<int>());
+    else
+      launch_csr_sweeps(st, rg, w, g, max_words, d_inst.as<CsrInst>(), items.as<int2>(),
+                        run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(),
+                        diag.as<float>(), tscale.as<float>(), init.as<uint32_t>(),
+                        best.as<uint32_t>(), fin.as<uint32_t>(), eb.as<double>(),
+                        ef.as<double>(), queue.as<int>());
     if (energy)
       launch_csr_energy(st, d_inst.as<CsrInst>(), items.as<int2>(), num_items, Rp,
                         row_ptr.as<int32_t>(), col.as<int32_t>(), val64.as<double>(),
@@ -91,6 +120,8 @@ void shape_for(CsrBatch &b, int Rp) {
     const int64_t wc = (int64_t)b.inst[c].n * (b.inst[c].width + 8);
     return wa > wc;
   });
+  b.order_by_work = order;
+  b.items_sites = 0;
   const int groups = Rp / 64;
   std::vector<int2> items;
   items.reserve((size_t)b.I * groups);
@@ -119,11 +150,36 @@ void shape_for(CsrBatch &b, int Rp) {
   b.drop_graph();
 }
 
-float time_launch(CsrBatch &b, bool rg, int w, int g) {
+// Work items of the colour kernel for `sites` sites per step: (instance,
+// group of 64/sites replicas), largest instances first.
+void col_items(CsrBatch &b, int sites) {
+  if (b.items_sites == sites && b.items_c_Rp == b.Rp) return;
+  const int per = 64 / sites, groups = b.Rp / per;
+  std::vector<int2> items;
+  items.reserve((size_t)b.I * groups);
+  for (int i : b.order_by_work)
+    for (int gidx = 0; gidx < groups; ++gidx) items.push_back(make_int2(i, gidx));
+  b.items_c.ensure(sizeof(int2) * std::max<size_t>(items.size(), 1));
+  OS_HIP_CHECK(hipMemcpyAsync(b.items_c.ptr, items.data(), sizeof(int2) * items.size(),
+                              hipMemcpyHostToDevice, b.ctx->stream));
+  OS_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+  b.num_items_c = (int)items.size();
+  b.items_sites = sites;
+  b.items_c_Rp = b.Rp;
+}
+
+// The run parameters in device memory for the launch enqueue() makes.
+void put_run(CsrBatch &b, CsrRun r) {
+  r.num_items = b.mode == 1 ? b.num_items_c : b.num_items;
+  OS_HIP_CHECK(hipMemcpyAsync(b.run.ptr, &r, sizeof(CsrRun), hipMemcpyHostToDevice,
+                              b.ctx->stream));
+}
+
+float time_launch(CsrBatch &b, bool rg, int w, int g, int sites = 0) {
   Ctx &c = *b.ctx;
-  b.enqueue(rg, w, g, false);  // warm-up
+  b.enqueue(rg, w, g, false, sites);  // warm-up
   OS_HIP_CHECK(hipEventRecord(c.ev_start, c.stream));
-  b.enqueue(rg, w, g, false);
+  b.enqueue(rg, w, g, false, sites);
   OS_HIP_CHECK(hipEventRecord(c.ev_stop, c.stream));
   OS_HIP_CHECK(hipEventSynchronize(c.ev_stop));
   float ms = 0.f;
@@ -131,12 +187,42 @@ float time_launch(CsrBatch &b, bool rg, int w, int g) {
   return ms;
 }
 
+// Times the colour kernel's candidate shapes (sites per step, waves per
+// work-group, resident work-groups) on a short run of the batch itself.
+void tune_col(CsrBatch &b, const CsrRun &full) {
+  Ctx &c = *b.ctx;
+  CsrRun probe = full;
+  probe.num_sweeps = std::min(full.num_sweeps, 2);
+  float best_ms = 0.f;
+  for (int sites : {4, 8, 16}) {
+    col_items(b, sites);
+    put_run(b, probe);
+    for (int w : {1, 2, 4}) {
+      const int maxb = csr_col_max_groups_per_cu(w, sites, b.max_words);
+      if (maxb < 1) continue;
+      std::vector<int> gpcs{maxb};
+      if (maxb > 1) gpcs.push_back(std::max(1, maxb / 2));
+      for (int gpc : gpcs) {
+        const int g = std::max(1, std::min(c.num_cus * gpc, (b.num_items_c + w - 1) / w));
+        const float ms = time_launch(b, false, w, g, sites);
+        if (b.c_waves == 0 || ms < best_ms) {
+          best_ms = ms;
+          b.c_waves = w;
+          b.c_groups = g;
+          b.c_sites = sites;
+        }
+      }
+    }
+  }
+  OS_REQUIRE(b.c_waves > 0, "csr batch: no colour-kernel shape fits");
+}
+
 // Times the candidate launch shapes on a short run of the batch itself.
 void tune(CsrBatch &b, const CsrRun &full) {
   Ctx &c = *b.ctx;
   CsrRun probe = full;
   probe.num_sweeps = std::min(full.num_sweeps, 2);
-  OS_HIP_CHECK(hipMemcpyAsync(b.run.ptr, &probe, sizeof(CsrRun), hipMemcpyHostToDevice, c.stream));
+  put_run(b, probe);
   float best_ms = 0.f;
   for (const bool rg : {false, true}) {
     if (rg && !csr_pf_eligible(b.max_width)) continue;
@@ -219,6 +305,31 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
           ell_v[at] = (float)val[nnz + k];
         }
       for (int j = 0; j < ni; ++j) dg.push_back((float)diag[spins + j]);
+      // colour classes (greedy, index order)
+      {
+        std::vector<int32_t> colour(ni);
+        std::vector<uint8_t> used((size_t)maxdeg + 2, 0);
+        int ncol = 0;
+        for (int j = 0; j < ni; ++j) {
+          for (int k = rp[j]; k < rp[j + 1]; ++k)
+            if (col[nnz + k] < j) used[colour[col[nnz + k]]] = 1;
+          int cc = 0;
+          while (used[cc]) ++cc;
+          colour[j] = cc;
+          ncol = std::max(ncol, cc + 1);
+          for (int k = rp[j]; k < rp[j + 1]; ++k)
+            if (col[nnz + k] < j) used[colour[col[nnz + k]]] = 0;
+        }
+        d.ncol = ncol;
+        d.class_off = (int64_t)b->h_cstart.size();
+        std::vector<int32_t> start(ncol + 1, 0);
+        for (int j = 0; j < ni; ++j) ++start[colour[j] + 1];
+        for (int cc = 0; cc < ncol; ++cc) start[cc + 1] += start[cc];
+        b->h_cstart.insert(b->h_cstart.end(), start.begin(), start.end());
+        std::vector<int32_t> fill(start.begin(), start.end() - 1), ord(ni);
+        for (int j = 0; j < ni; ++j) ord[fill[colour[j]]++] = j;
+        b->h_order.insert(b->h_order.end(), ord.begin(), ord.end());
+      }
       b->max_words = std::max(b->max_words, d.words);
       b->max_width = std::max(b->max_width, d.width);
       b->total_words += d.words;
@@ -253,6 +364,12 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
       OS_HIP_CHECK(hipMemcpyAsync(b->col.ptr, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
       OS_HIP_CHECK(hipMemcpyAsync(b->val64.ptr, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
     }
+    b->corder.ensure(sizeof(int32_t) * b->h_order.size());
+    b->cstart.ensure(sizeof(int32_t) * b->h_cstart.size());
+    OS_HIP_CHECK(hipMemcpyAsync(b->corder.ptr, b->h_order.data(),
+                                sizeof(int32_t) * b->h_order.size(), hipMemcpyHostToDevice, s));
+    OS_HIP_CHECK(hipMemcpyAsync(b->cstart.ptr, b->h_cstart.data(),
+                                sizeof(int32_t) * b->h_cstart.size(), hipMemcpyHostToDevice, s));
     OS_HIP_CHECK(hipStreamSynchronize(s));
     *out = reinterpret_cast<os_csr_batch *>(b.release());
   });
@@ -277,6 +394,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     OS_REQUIRE(p->num_replicas >= 1, "num_replicas must be positive");
     OS_REQUIRE(p->num_sweeps >= 0, "num_sweeps must be non-negative");
     OS_REQUIRE(p->num_sweeps == 0 || p->tscale, "tscale is null");
+    OS_REQUIRE(p->site_order == 0 || p->site_order == 1, "site_order must be 0 or 1");
     CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
     OS_REQUIRE((uint64_t)p->instance_base + (uint64_t)b.I <= (1u << 24),
                "instance ids must stay below 2^24");
@@ -312,7 +430,8 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
                                   hipMemcpyHostToDevice, s));
       OS_HIP_CHECK(hipStreamSynchronize(s));
     }
-    if (b.waves == 0) {
+    b.mode = p->site_order;
+    if (b.mode == 0 && b.waves == 0) {
       if (p->tune) {
         os::tune(b, run);
       } else {
@@ -322,14 +441,45 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
                                         b.num_items));
       }
     }
-    OS_HIP_CHECK(hipMemcpyAsync(b.run.ptr, &run, sizeof(os::CsrRun), hipMemcpyHostToDevice, s));
-    if (p->use_graph && (!b.gexec || b.g_reg != b.reg || b.g_waves != b.waves ||
-                         b.g_groups != b.groups ||
-                         b.g_Rp != Rp)) {
+    // OS_CSR_COLOUR_SITES (4, 8 or 16) fixes the colour kernel's sites per step
+    if (b.mode == 1) {
+      const char *senv = std::getenv("OS_CSR_COLOUR_SITES");
+      const int forced = senv ? std::atoi(senv) : 0;
+      OS_REQUIRE(!senv || forced == 4 || forced == 8 || forced == 16,
+                 "OS_CSR_COLOUR_SITES must be 4, 8 or 16");
+      if (forced && forced != b.c_sites) {
+        b.c_sites = forced;
+        b.c_waves = 1;
+        os::col_items(b, b.c_sites);
+        b.c_groups = std::max(
+            1, std::min(c.num_cus * os::csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
+                        b.num_items_c));
+      }
+    }
+    if (b.mode == 1 && b.c_waves == 0) {
+      if (p->tune) {
+        os::tune_col(b, run);
+      } else {
+        b.c_sites = 8;
+        b.c_waves = 1;
+        os::col_items(b, b.c_sites);
+        b.c_groups = std::max(
+            1, std::min(c.num_cus * os::csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
+                        b.num_items_c));
+      }
+    }
+    if (b.mode == 1) os::col_items(b, b.c_sites);
+    const bool rg = b.mode == 0 && b.reg;
+    const int wv = b.mode == 1 ? b.c_waves : b.waves;
+    const int gr = b.mode == 1 ? b.c_groups : b.groups;
+    const int st_sites = b.mode == 1 ? b.c_sites : 0;
+    os::put_run(b, run);
+    if (p->use_graph && (!b.gexec || b.g_mode != b.mode || b.g_reg != rg || b.g_waves != wv ||
+                         b.g_groups != gr || b.g_sites != st_sites || b.g_Rp != Rp)) {
       b.drop_graph();
       OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
-        b.enqueue(b.reg, b.waves, b.groups, true);
+        b.enqueue(rg, wv, gr, true, st_sites);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s, &g);
@@ -338,9 +488,11 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
       }
       OS_HIP_CHECK(hipStreamEndCapture(s, &b.graph));
       OS_HIP_CHECK(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
-      b.g_reg = b.reg;
-      b.g_waves = b.waves;
-      b.g_groups = b.groups;
+      b.g_mode = b.mode;
+      b.g_reg = rg;
+      b.g_waves = wv;
+      b.g_groups = gr;
+      b.g_sites = st_sites;
       b.g_Rp = Rp;
       b.g_tscale = b.tscale.ptr;
     }
@@ -348,7 +500,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     if (p->use_graph)
       OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
     else
-      b.enqueue(b.reg, b.waves, b.groups, true);
+      b.enqueue(rg, wv, gr, true, st_sites);
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
     // results
     std::vector<double> e64((size_t)b.I * Rp);
@@ -416,6 +568,32 @@ int os_csr_batch_launch_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_
     if (waves_per_group) *waves_per_group = b.waves;
     if (groups) *groups = b.groups;
     if (prefetch) *prefetch = b.reg ? 1 : 0;
+  });
+}
+
+int os_csr_batch_colour_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_t *groups,
+                              int32_t *sites_per_step) {
+  return os::guarded([&] {
+    OS_REQUIRE(b_, "null argument");
+    const CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+    if (waves_per_group) *waves_per_group = b.c_waves;
+    if (groups) *groups = b.c_groups;
+    if (sites_per_step) *sites_per_step = b.c_sites;
+  });
+}
+
+int os_csr_batch_site_order(os_csr_batch *b_, int32_t instance, int32_t *order,
+                            int32_t *class_start, int32_t *num_classes) {
+  return os::guarded([&] {
+    OS_REQUIRE(b_, "null argument");
+    const CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+    OS_REQUIRE(instance >= 0 && instance < b.I, "instance out of range");
+    const os::CsrInst &d = b.inst[instance];
+    if (order)
+      std::memcpy(order, b.h_order.data() + d.spin_off, sizeof(int32_t) * d.n);
+    if (class_start)
+      std::memcpy(class_start, b.h_cstart.data() + d.class_off, sizeof(int32_t) * (d.ncol + 1));
+    if (num_classes) *num_classes = d.ncol;
   });
 }
 

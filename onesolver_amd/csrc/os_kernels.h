@@ -100,13 +100,14 @@ struct CsrInst {
   int32_t n;          // spins
   int32_t words;      // ceil((n + 1) / 32) state words (bit n: the ELL padding target)
   int32_t width;      // ELL row width (multiple of 4)
-  int32_t pad_;
+  int32_t ncol;       // colour classes (site order 1)
   int64_t ell_off;    // first ELL entry (multiple of 4)
   int64_t spin_off;   // first spin in diag / diag64
   int64_t row_off;    // first entry in row_ptr (n + 1 entries per instance)
   int64_t csr_off;    // first entry in col / val
   int64_t state_off;  // first word of the instance's [words][Rp] state planes
   int64_t rep_off;    // first replica in per-replica arrays (Rp per instance)
+  int64_t class_off;  // first entry in the class starts (ncol + 1 per instance)
 };
 // Parameters of one batched run, read by the kernel from device memory so a
 // captured graph replays with new values.
@@ -129,6 +130,18 @@ void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_w
                        const int4 *ell_col, const float4 *ell_val, const float *diag,
                        const float *tscale, const uint32_t *init_bits, uint32_t *best_bits,
                        uint32_t *final_bits, double *e_best, double *e_final, int *queue);
+// colour-class sweeps: G (4, 8 or 16) sites of 64/G replicas per wave step;
+// items are (instance, group of 64/G replicas); corder lists the sites of
+// each instance class by class (offset spin_off), cstart the class starts
+// (offset class_off)
+int csr_col_lds_bytes(int waves, int sites, int max_words);
+int csr_col_max_groups_per_cu(int waves, int sites, int max_words);
+void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
+                           const CsrInst *inst, const int2 *items, const CsrRun *run,
+                           const int4 *ell_col, const float4 *ell_val, const float *diag,
+                           const int32_t *corder, const int32_t *cstart, const float *tscale,
+                           const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
+                           double *e_best, double *e_final, int *queue);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
                        const double *diag64, const uint32_t *bits, double *out);

@@ -55,6 +55,11 @@ def lib():
         L.or_sweep_csr.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p, _f32p, _f32p, ctypes.c_int,
                                    ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, _u8p, _u8p,
                                    _u8p, _f64p, ctypes.c_int]
+        L.or_sweep_csr_order.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p, _f32p, _f32p,
+                                         ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                         ctypes.c_uint32, _i32p, _u8p, _u8p, _u8p, _f64p,
+                                         ctypes.c_int]
+        L.or_csr_colour_order.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p]
         L.or_warm_init.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, ctypes.c_uint64,
                                    ctypes.c_uint32, _u8p]
         L.or_sweep_init.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
@@ -200,8 +205,23 @@ def tempering_dense(flat, n, tscale, ladders, num_sweeps, interval, seed=1, inst
     return fin, bs, be, sl
 
 
+def csr_colour_order(row_ptr, col):
+    """Greedy colour classes (or_csr_colour_order): (order, class starts)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    cl = np.ascontiguousarray(col, dtype=np.int32)
+    n = len(rp) - 1
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    maxdeg = int(np.diff(rp).max()) if n else 0
+    cs = np.zeros(maxdeg + 2, dtype=np.int32)
+    k = lib().or_csr_colour_order(n, _p(rp, _i32p), _p(cl, _i32p), _p(order, _i32p),
+                                  _p(cs, _i32p))
+    return order[:n], cs[:k + 1]
+
+
 def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0, init=None,
-              threads=0):
+              threads=0, order=None):
+    """Sparse sweep-mode chains; `order`: the sites' visiting order per sweep
+    (default 0..n-1)."""
     rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
     cl = np.ascontiguousarray(col, dtype=np.int32)
     vl = np.ascontiguousarray(val, dtype=np.float32)
@@ -212,9 +232,10 @@ def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0,
     bs = np.zeros((num_replicas, n), dtype=np.uint8)
     be = np.zeros(num_replicas, dtype=np.float64)
     ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint8)
-    lib().or_sweep_csr(n, _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f32p), _p(dg, _f32p),
-                       _p(ts, _f32p), len(ts), num_replicas, seed, instance, _p(ini, _u8p),
-                       _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), threads)
+    od = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+    lib().or_sweep_csr_order(n, _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f32p), _p(dg, _f32p),
+                             _p(ts, _f32p), len(ts), num_replicas, seed, instance, _p(od, _i32p),
+                             _p(ini, _u8p), _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), threads)
     return fin, bs, be
 
 

@@ -739,6 +739,52 @@ void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float
                   uint32_t num_replicas, uint64_t seed, uint32_t instance,
                   const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
                   double *best_energies, int threads) {
+  or_sweep_csr_order(n, row_ptr, col, val, diag, tscale, num_sweeps, num_replicas, seed,
+                     instance, NULL, init, final_states, best_states, best_energies, threads);
+}
+
+/* Colour classes of a symmetric CSR graph: greedy in index order, site j
+ * taking the smallest colour no neighbour l < j holds.  order[] lists the
+ * sites by (colour, index); class c is order[cstart[c] .. cstart[c+1]).
+ * Returns the number of colours (cstart needs max degree + 2 entries).      */
+int or_csr_colour_order(int n, const int32_t *row_ptr, const int32_t *col, int32_t *order,
+                        int32_t *cstart) {
+  int32_t *colour = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int maxdeg = 0;
+  for (int j = 0; j < n; ++j)
+    if (row_ptr[j + 1] - row_ptr[j] > maxdeg) maxdeg = row_ptr[j + 1] - row_ptr[j];
+  uint8_t *used = (uint8_t *)calloc((size_t)maxdeg + 2, 1);
+  int ncol = 0;
+  for (int j = 0; j < n; ++j) {
+    for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
+      if (col[k] < j) used[colour[col[k]]] = 1;
+    int c = 0;
+    while (used[c]) ++c;
+    colour[j] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+    for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
+      if (col[k] < j) used[colour[col[k]]] = 0;
+  }
+  for (int c = 0; c <= ncol; ++c) cstart[c] = 0;
+  for (int j = 0; j < n; ++j) ++cstart[colour[j] + 1];
+  for (int c = 0; c < ncol; ++c) cstart[c + 1] += cstart[c];
+  int32_t *fill = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ncol + 1));
+  memcpy(fill, cstart, sizeof(int32_t) * (size_t)(ncol + 1));
+  for (int j = 0; j < n; ++j) order[fill[colour[j]]++] = j;
+  free(fill);
+  free(used);
+  free(colour);
+  return ncol;
+}
+
+/* The sparse sweep with the sites of each sweep visited in order[0..n-1]
+ * (NULL: 0..n-1).  Site j keeps its threshold (counter (j>>2, r, t)), so
+ * only the order of the updates differs from or_sweep_csr.                 */
+void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                        const float *diag, const float *tscale, int num_sweeps,
+                        uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                        const int32_t *order, const uint8_t *init, uint8_t *final_states,
+                        uint8_t *best_states, double *best_energies, int threads) {
   uint32_t key[2];
   seed_key(seed, key);
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
@@ -756,8 +802,10 @@ void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float
     uint32_t w[4];
     for (int t = 0; t < num_sweeps; ++t) {
       float ts = tscale[t];
-      for (int j = 0; j < n; ++j) {
-        if ((j & 3) == 0) sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
+      for (int pos = 0; pos < n; ++pos) {
+        const int j = order ? order[pos] : pos;
+        if (order || (j & 3) == 0)
+          sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
         float thr = or_neg_ln_u(w[j & 3]) * ts;
         float h = diag[j];
         for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)

@@ -123,6 +123,18 @@ void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float
                   uint32_t num_replicas, uint64_t seed, uint32_t instance,
                   const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
                   double *best_energies, int threads);
+/* or_sweep_csr with the sites of every sweep visited in order[0..n-1]
+ * (NULL: 0..n-1); each site keeps its threshold stream.                    */
+void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                        const float *diag, const float *tscale, int num_sweeps,
+                        uint32_t num_replicas, uint64_t seed, uint32_t instance,
+                        const int32_t *order, const uint8_t *init, uint8_t *final_states,
+                        uint8_t *best_states, double *best_energies, int threads);
+/* Greedy colour classes of a symmetric CSR graph (index order, smallest
+ * colour free of lower-index neighbours); order[] by (colour, index),
+ * class c = order[cstart[c] .. cstart[c+1]); returns the colour count.     */
+int or_csr_colour_order(int n, const int32_t *row_ptr, const int32_t *col, int32_t *order,
+                        int32_t *cstart);
 /* Warm-start initial state from marginals p[j]: x_j = (u < p_j). */
 void or_warm_init(int n, const float *p, uint32_t num_replicas, uint64_t seed,
                   uint32_t instance, uint8_t *states);

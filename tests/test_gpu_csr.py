@@ -192,3 +192,63 @@ def test_csr_rejects_bad_input(ctx):
         onesolver_amd.CsrBatch(ctx, [(np.array([1, 1, 2], dtype=np.int32),
                                       np.array([1, 0], dtype=np.int32), np.array([1.0, 1.0]),
                                       np.zeros(2))])
+
+
+def _check_colour_against_oracle(b, out, insts, ts, R, seed, base):
+    """site_order 1: the library's class order is the oracle's greedy
+    colouring; final and best states equal or_sweep_csr with that visiting
+    order bit for bit; running best energies agree to FP64 rounding (the
+    kernel sums a sweep's dE per lane, then over lanes) and exactly where
+    every dE is exact."""
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        order, cs = b.site_order(i)
+        o2, cs2 = oracle.csr_colour_order(rp, cl)
+        assert (order == o2).all() and (cs == cs2).all(), f"colouring differs, instance {i}"
+        fin, bs, be = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=seed, instance=base + i,
+                                       order=order)
+        assert (out["final_states"][i] == fin).all(), f"final states differ, instance {i}"
+        assert (out["best_states"][i] == bs).all(), f"best states differ, instance {i}"
+        np.testing.assert_allclose(out["run_best_energies"][i], be, rtol=1e-9, atol=1e-12)
+        e64 = oracle.csr_energy(rp, cl, vl, dg, bs)
+        np.testing.assert_allclose(out["best_energies"][i], e64, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("sites", [4, 8, 16])
+def test_csr_colour_classes_match_oracle(ctx, monkeypatch, sites):
+    """The colour-class kernel with 4, 8 and 16 sites per step on a mixed
+    batch: regular graphs, hubs, isolated spins, n != 0 mod 32, R != 0 mod 64."""
+    insts = _batch() + [instances.regular_csr(2048, 5, seed=7, instance=9)]
+    R, S = 100, 7
+    ts = _tscale(S)
+    monkeypatch.setenv("OS_CSR_COLOUR_SITES", str(sites))
+    print(f"[gpu] csr colour classes, {sites} sites per step", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=13, instance_base=3, final_states=True, site_order=1)
+    _check_colour_against_oracle(b, out, insts, ts, R, 13, 3)
+    assert b.colour_shape()[2] == sites
+    b.close()
+
+
+def test_csr_colour_classes_exact_energies_and_tuning(ctx):
+    """Integer couplings and quarter-step diagonals: the running energies are
+    exact, so they equal the oracle's; the tuner picks a shape, and the
+    sequential order on the same batch still gives the sequential chains."""
+    insts = []
+    for i, (n, d) in enumerate([(64, 3), (333, 6), (1024, 8), (47, 4)]):
+        rp, cl, vl, dg = instances.regular_csr(n, d, seed=2, instance=i)
+        insts.append((rp, cl, vl, np.round(np.asarray(dg, np.float64) * 4) / 4))
+    R, S = 64, 9
+    ts = _tscale(S)
+    print("[gpu] csr colour classes, exact energies, tuned", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=5, final_states=True, site_order=1, tune=True)
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        order, _ = b.site_order(i)
+        fin, bs, be = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=5, instance=i, order=order)
+        assert (out["final_states"][i] == fin).all() and (out["best_states"][i] == bs).all()
+        assert (out["run_best_energies"][i] == be).all()
+    w, g, sites = b.colour_shape()
+    assert w in (1, 2, 4) and g >= 1 and sites in (4, 8, 16)
+    seq = b.anneal(R, ts, seed=5, final_states=True)
+    _check_against_oracle(seq, insts, ts, R, 5, 0)
+    b.close()

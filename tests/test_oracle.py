@@ -374,3 +374,80 @@ def test_tempering_keeps_boltzmann_marginals():
         p /= p.sum()
         chi2 = ((counts - L * p) ** 2 / (L * p)).sum()
         assert chi2 < 45.0, (k, chi2)  # 15 dof: p ~ 1e-4
+
+
+# --------------------------------------------------------------------------
+# sparse sweeps in colour-class order
+# --------------------------------------------------------------------------
+
+
+def _greedy_colours(rp, col):
+    """Greedy colouring in index order, restated: smallest colour not held by
+    a lower-index neighbour."""
+    n = len(rp) - 1
+    colour = np.zeros(n, dtype=np.int64)
+    for j in range(n):
+        taken = {int(colour[l]) for l in col[rp[j]:rp[j + 1]] if l < j}
+        c = 0
+        while c in taken:
+            c += 1
+        colour[j] = c
+    return colour
+
+
+@pytest.mark.parametrize("n,d,seed", [(60, 4, 1), (257, 6, 2), (1000, 3, 3), (9, 8, 4)])
+def test_csr_colour_order_is_the_greedy_colouring(n, d, seed):
+    rp, col, val, diag = _random_csr(n, d, seed)
+    order, cs = oracle.csr_colour_order(rp, col)
+    colour = _greedy_colours(rp, col)
+    assert sorted(order.tolist()) == list(range(n))
+    assert cs[0] == 0 and cs[-1] == n and len(cs) == colour.max() + 2
+    for c in range(len(cs) - 1):
+        cls = order[cs[c]:cs[c + 1]]
+        assert (colour[cls] == c).all() and (np.diff(cls) > 0).all()
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    assert (colour[rows] != colour[col]).all()  # no coupling inside a class
+
+
+def test_sweep_csr_in_identity_order_is_the_sequential_sweep():
+    rp, col, val, diag = _random_csr(80, 5, 6)
+    ts = (1.0 / np.geomspace(0.1, 3.0, 12)).astype(np.float32)
+    a = oracle.sweep_csr(rp, col, val, diag, ts, 16, seed=3)
+    b = oracle.sweep_csr(rp, col, val, diag, ts, 16, seed=3, order=np.arange(80, dtype=np.int32))
+    for x, y in zip(a, b):
+        assert (x == y).all()
+    # a different order is a different chain with the same thresholds per site
+    o, _ = oracle.csr_colour_order(rp, col)
+    c = oracle.sweep_csr(rp, col, val, diag, ts, 16, seed=3, order=o)
+    assert not (a[0] == c[0]).all()
+
+
+def test_sweep_csr_colour_order_greedy_limit_and_boltzmann_law():
+    # tscale 0: only downhill flips, so the final states are single-flip minima
+    from onesolver_amd import instances
+
+    rp, col, val, diag = _random_csr(120, 4, 8)
+    q = instances.csr_to_dense(rp, col, val, diag)
+    o, _ = oracle.csr_colour_order(rp, col)
+    fin, _, _ = oracle.sweep_csr(rp, col, val, diag, np.zeros(30, np.float32), 24, seed=2, order=o)
+    for x in fin:
+        h = np.diag(q) + (q - np.diag(np.diag(q))) @ x
+        assert (np.where(x == 1, -h, h) >= -1e-6).all()
+    # at fixed beta the class-ordered sweep keeps exp(-beta E) invariant
+    q4 = np.array([[0.3, -1.0, 0.5, 0.0], [-1.0, -0.2, 0.7, 0.4], [0.5, 0.7, 0.1, -0.6],
+                   [0.0, 0.4, -0.6, 0.2]])
+    rows, cols = np.nonzero(q4 - np.diag(np.diag(q4)))
+    rp4 = np.searchsorted(rows, np.arange(5)).astype(np.int32)
+    o4, cs4 = oracle.csr_colour_order(rp4, cols.astype(np.int32))
+    assert not (o4 == np.arange(4)).all()  # the visiting order really changes
+    beta, R = 1.3, 100000
+    fin, _, _ = oracle.sweep_csr(rp4, cols.astype(np.int32), q4[rows, cols].astype(np.float32),
+                                 np.diag(q4).astype(np.float32),
+                                 np.full(30, 1.0 / beta, np.float32), R, seed=21, order=o4)
+    counts = np.bincount(fin @ (1 << np.arange(4)), minlength=16)
+    states = ((np.arange(16)[:, None] >> np.arange(4)) & 1).astype(np.uint8)
+    e = oracle.energy(q4.astype(np.float32).astype(np.float64), states)
+    p = np.exp(-beta * (e - e.min()))
+    p /= p.sum()
+    chi2 = ((counts - R * p) ** 2 / (R * p)).sum()
+    assert chi2 < 45.0  # 15 dof: p ~ 1e-4
