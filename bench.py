@@ -490,6 +490,77 @@ def run_tempering(args):
     }
 
 
+def run_population(args):
+    """Population annealing at the annealing configs' budgets, beside plain
+    annealing on the same schedule (best energies and time).  Dense: the
+    config-2 instance (SK N=1024, seed 0), 16384 replicas, 64 temperature
+    steps (beta 0.1 .. 3.0 geometric) x 64 sweeps = 4096 sweeps, FP64 refresh
+    every 512.  Sparse: the config-3 batch (512 instances), 256 replicas,
+    32 steps x 64 sweeps = 2048 sweeps, colour-class site order."""
+    import onesolver_amd
+    from onesolver_amd import instances
+
+    K, theta = (64, 64) if args.pa_path == "dense" else (32, 64)
+    S = K * theta
+    betas = np.geomspace(0.1, 3.0, K)
+    ts_steps = onesolver_amd.tscale_from_beta(betas)
+    ts_sa = np.repeat(ts_steps, theta)
+    ctx = onesolver_amd.Context(0)
+    if args.pa_path == "dense":
+        n, R = 1024, 16384
+        q32, const = instances.sk_qubo(n, seed=0, instance=0)
+        prob = onesolver_amd.DenseProblem(ctx, q32.astype(np.float64))
+        kw = dict(seed=1234, refresh_interval=512, best_states=False)
+        run = lambda: prob.population(R, ts_steps, theta, **kw)  # noqa: E731
+        plain = lambda: prob.anneal(R, ts_sa, **kw)  # noqa: E731
+        best_of = lambda out: float(out["best_energy"])  # noqa: E731
+        sum_n = n
+        work = f"dense SK QUBO N=1024, {R} replicas"
+    else:
+        insts = instances.batch_c3(512)
+        R = 256
+        prob = onesolver_amd.CsrBatch(ctx, insts)
+        kw = dict(seed=1234, best_states=False, site_order=1)
+        run = lambda: prob.population(R, ts_steps, theta, **kw)  # noqa: E731
+        plain = lambda: prob.anneal(R, ts_sa, **kw)  # noqa: E731
+        best_of = lambda out: float(np.sum(out["inst_best_energy"]))  # noqa: E731
+        sum_n = int(sum(len(d) for (_, _, _, d) in insts))
+        work = f"config-3 sparse batch (512 instances), {R} replicas, colour-class order"
+        const = 0.0
+    for _ in range(args.warmup):
+        run()
+    t0 = time.perf_counter()
+    best = np.inf
+    for i in range(args.steps):
+        out = run()
+        best = min(best, best_of(out))
+        log(f"step {i}: population best {best_of(out):.4f}")
+    elapsed = time.perf_counter() - t0
+    plain()
+    t1 = time.perf_counter()
+    sa = plain()
+    sa_s = time.perf_counter() - t1
+    prob.close()
+    ctx.close()
+    attempts = float(R) * S * sum_n * args.steps
+    return {
+        "metric": "spin-flip attempts/s", "value": attempts / elapsed, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 fields, fp64 energies",
+        "data": "synthetic (Philox-generated instances)",
+        "config": {"workload": f"population annealing, {work}, {K} steps (beta 0.1..3.0) x "
+                               f"{theta} sweeps, resampling between steps",
+                   "path": args.pa_path, "replicas": R, "steps_K": K, "sweeps_per_step": theta,
+                   "sweeps": S},
+        "roofline": None,
+        "best_energy": best, "best_energy_ising": best + const,
+        "annealing_same_budget": {"best_energy": best_of(sa),
+                                  "best_energy_ising": best_of(sa) + const,
+                                  "ms": sa_s * 1e3},
+    }
+
+
 def run_exhaustive(args):
     """All 2^n states of a dense U[-1,1] instance (os_exhaustive), n=32 by
     default.  One step = one search, Q uploaded by the call (0.01 ms)."""
@@ -750,7 +821,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2",
-                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "pt", "ingest", "batch", "copy"])
+                    choices=["c2", "c1", "c3", "c4", "c5", "exh", "pt", "pa", "ingest", "batch",
+                             "copy"])
+    ap.add_argument("--pa-path", choices=["dense", "sparse"], default="dense")
     ap.add_argument("--ingest-n", type=int, default=8192)
     ap.add_argument("--pt-interval", type=int, default=16)
     ap.add_argument("--exh-n", type=int, default=32)
@@ -796,6 +869,9 @@ def main():
         return
     if args.workload == "pt":
         print(json.dumps(run_tempering(args)))
+        return
+    if args.workload == "pa":
+        print(json.dumps(run_population(args)))
         return
     if args.workload == "c5":
         res, _ = run_dense_c5(args)

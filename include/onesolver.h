@@ -46,7 +46,9 @@ const char *os_last_error(void);
 int os_device_count(int *count);
 
 /* Create / destroy a context on GPU `device`.  Replaces the sycl::queue of
- * app/one-solver-anneal.cpp:139-147. */
+ * app/one-solver-anneal.cpp:139-147.  Objects created on a context (dense
+ * problems and batches, sparse batches, .qubo models, networks) use its
+ * stream until they are destroyed: destroy them first. */
 int os_ctx_create(int device, os_ctx **out);
 int os_ctx_destroy(os_ctx *ctx);
 
@@ -171,6 +173,29 @@ typedef struct os_tempering_params {
   int32_t refresh_interval;  /* a multiple of exchange_interval, or <= 0 (end only) */
 } os_tempering_params;
 
+/* Population annealing on a dense batch.  Each instance anneals a population
+ * of num_replicas sweep-mode replicas through num_steps temperature scales:
+ * step k runs sweeps_per_step sweeps at tscale[k] (sweep counters
+ * k * sweeps_per_step, ...).  After every step but the last, the population
+ * is resampled systematically: with E the running energies and
+ * dbeta = 1/tscale[k+1] - 1/tscale[k], replica j continues from the state,
+ * fields and energy of replica anc[j], where W_i is the running sum of
+ * exp(-dbeta (E_i - min E)) and anc[j] is the first i with
+ * W_i > (j + u) / R * W_{R-1}; u is the 53-bit uniform of words 0, 1 of
+ * Philox counter (k, 0, 0, ((instance + i) << 8) | 0x14).  The best state of
+ * each replica index stays with the index.  No reference counterpart: the
+ * reference recommends population annealing
+ * (benchmarks/annealing/performance.md:52-59) but anneals only. */
+typedef struct os_population_params {
+  uint32_t num_replicas;    /* population size R per instance */
+  int32_t num_steps;        /* K >= 1 temperature steps */
+  int32_t sweeps_per_step;  /* >= 1 */
+  const float *tscale;      /* [num_steps] host array, > 0 */
+  uint64_t seed;            /* Philox key */
+  uint32_t instance;        /* stream id of this instance (< 2^24) */
+  int32_t refresh_interval; /* a multiple of sweeps_per_step, or <= 0 (end only) */
+} os_population_params;
+
 typedef struct os_dense os_dense;
 
 /* Upload a dense QUBO (q_sym: n x n FP64 symmetric, the reference's flat
@@ -215,6 +240,13 @@ int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best
 int os_dense_tempering(os_dense *dense, const os_tempering_params *params, double *best_energies,
                        uint8_t *best_states, uint8_t *final_states, uint32_t *final_slots,
                        uint8_t *best_state, double *best_energy, os_sweep_stats *stats);
+/* Population annealing on `dense` (see os_population_params).  Outputs as in
+ * os_dense_anneal, indexed by replica; ancestors (optional,
+ * [I][num_steps - 1][R]) holds anc of every resampling round. */
+int os_dense_population(os_dense *dense, const os_population_params *params,
+                        double *best_energies, uint8_t *best_states, uint8_t *final_states,
+                        uint32_t *ancestors, uint8_t *best_state, double *best_energy,
+                        os_sweep_stats *stats);
 
 /* ---- learned warm start --------------------------------------------------
  * A 3-layer message-passing network (hidden 64, bf16 weights, fp32
@@ -267,8 +299,33 @@ typedef struct os_csr_params {
   int32_t site_order;          /* 0: every sweep visits sites 0..n-1 in order; 1: class by
                                   class of a greedy colouring (os_csr_batch_site_order), so a
                                   wave updates several independent sites of its replicas at
-                                  once; site j keeps its threshold stream either way */
+                                  once.  The update at position p of the visiting order draws
+                                  word p&3 of counter (p>>2, replica, sweep, tag) either way */
 } os_csr_params;
+
+/* Population annealing on a sparse batch (os_csr_batch_population): every
+ * instance anneals num_replicas replicas through num_steps temperature
+ * scales, sweeps_per_step sweeps each (sweep counters k * sweeps_per_step,
+ * ...), with the thresholds and site order of os_csr_batch_anneal.  Each step
+ * starts from the FP64 energy of its start states, as a run does.  After
+ * every step but the last, the population is resampled systematically as in
+ * os_population_params, on the running energies at the step's end and with
+ * the uniform of Philox counter (k, 0, 0, ((instance_base + i) << 8) | 0x14).
+ * The best state of replica index r is its lowest-energy state over the
+ * steps (sweep ends and step starts; the earliest on ties).  No reference
+ * counterpart: the reference recommends population annealing
+ * (benchmarks/annealing/performance.md:52-59) but anneals only. */
+typedef struct os_csr_population_params {
+  uint32_t num_replicas;    /* population size R per instance */
+  int32_t num_steps;        /* K >= 1 temperature steps */
+  int32_t sweeps_per_step;  /* >= 1 */
+  const float *tscale;      /* [num_steps] host array, > 0 */
+  uint64_t seed;            /* Philox key */
+  uint32_t instance_base;   /* stream id of instance 0 (< 2^24) */
+  int32_t site_order;       /* as in os_csr_params */
+  int32_t use_graph;        /* replay every step from the captured hipGraph */
+  int32_t tune;             /* pick the launch shape by timing candidates on first use */
+} os_csr_population_params;
 
 int os_csr_batch_create(os_ctx *ctx, int32_t num_instances, const int32_t *n,
                         const int32_t *row_ptr, const int32_t *col, const double *val,
@@ -283,6 +340,13 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
                         double *best_energies, double *run_best_energies, uint8_t *best_states,
                         uint8_t *final_states, double *inst_best_energy,
                         int32_t *inst_best_replica, os_sweep_stats *stats);
+/* Population annealing (see os_csr_population_params).  Outputs optional, as
+ * in os_csr_batch_anneal; best_energies are the FP64 energies of the best
+ * states; ancestors: per instance [num_steps - 1][num_replicas]. */
+int os_csr_batch_population(os_csr_batch *batch, const os_csr_population_params *params,
+                            double *best_energies, uint8_t *best_states, uint8_t *final_states,
+                            uint32_t *ancestors, double *inst_best_energy,
+                            int32_t *inst_best_replica, os_sweep_stats *stats);
 /* The launch shape in use (0 before the first run); prefetch: 1 if the
  * kernel variant that requests the next site's row ahead runs (batches whose
  * rows have at most 8 couplings), 0 for the general one. */

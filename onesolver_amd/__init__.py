@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -86,6 +87,13 @@ class TemperingParams(ctypes.Structure):
                 ("instance", ctypes.c_uint32), ("refresh_interval", ctypes.c_int32)]
 
 
+class PopulationParams(ctypes.Structure):
+    """os_population_params of include/onesolver.h."""
+    _fields_ = [("num_replicas", ctypes.c_uint32), ("num_steps", ctypes.c_int32),
+                ("sweeps_per_step", ctypes.c_int32), ("tscale", _vp), ("seed", ctypes.c_uint64),
+                ("instance", ctypes.c_uint32), ("refresh_interval", ctypes.c_int32)]
+
+
 class SweepStats(ctypes.Structure):
     """os_sweep_stats of include/onesolver.h."""
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
@@ -130,13 +138,23 @@ class Context:
         _check(lib().os_ctx_create(device, ctypes.byref(h)))
         self.device = device
         self._h = h
+        # objects created on this context: closed before it is destroyed, so
+        # none of them outlives the os_ctx it refers to
+        self._children = weakref.WeakSet()
 
     @property
     def handle(self):
+        if not self._h:
+            raise OneSolverError(-1, "context is closed")
         return self._h
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self._h:
+            for child in list(self._children):
+                child.close()
             lib().os_ctx_destroy(self._h)
             self._h = None
 
@@ -203,6 +221,7 @@ class DenseProblem:
         h = _vp()
         _check(lib().os_dense_create(ctx.handle, self.n, _p(q, _f64p), ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if self._h:
@@ -247,6 +266,44 @@ class DenseProblem:
                 out[k] = out[k][0]
         out["best_energy"] = float(out["best_energy"][0])
         return out
+
+    def population(self, num_replicas, tscale, sweeps_per_step, seed=1, instance=0,
+                   refresh_interval=0, best_states=True, final_states=False):
+        """Population annealing (os_dense_population): num_replicas replicas
+        through the len(tscale) temperature steps; arrays indexed by replica,
+        ancestors [steps - 1, R]."""
+        out = _population(self._h, 1, self.n, num_replicas, tscale, sweeps_per_step, seed,
+                          instance, refresh_interval, best_states, final_states)
+        for k in ("best_energies", "best_states", "final_states", "ancestors", "best_state"):
+            if out[k] is not None:
+                out[k] = out[k][0]
+        out["best_energy"] = float(out["best_energy"][0])
+        return out
+
+
+def _population(h, I, n, num_replicas, tscale, sweeps_per_step, seed, instance, refresh_interval,
+                best_states, final_states):
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    R, K = int(num_replicas), len(ts)
+    prm = PopulationParams(R, K, sweeps_per_step, ts.ctypes.data, seed, instance,
+                           refresh_interval)
+    be = np.zeros((I, R), dtype=np.float64)
+    bs = np.zeros((I, R, n), dtype=np.uint8) if best_states else None
+    fs = np.zeros((I, R, n), dtype=np.uint8) if final_states else None
+    anc = np.zeros((I, max(K - 1, 0), R), dtype=np.uint32)
+    best = np.zeros((I, n), dtype=np.uint8)
+    e = np.zeros(I, dtype=np.float64)
+    st = SweepStats()
+    L = lib()
+    L.os_dense_population.argtypes = [_vp, ctypes.POINTER(PopulationParams), _f64p, _u8p, _u8p,
+                                      ctypes.POINTER(ctypes.c_uint32), _u8p, _f64p,
+                                      ctypes.POINTER(SweepStats)]
+    _check(L.os_dense_population(h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p),
+                                 _p(fs, _u8p),
+                                 anc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                 _p(best, _u8p), _p(e, _f64p), ctypes.byref(st)))
+    return {"best_energies": be, "best_states": bs, "final_states": fs, "ancestors": anc,
+            "best_state": best, "best_energy": e, "stats": st.as_dict()}
 
 
 def _tempering(h, I, n, ladders, tscale, num_sweeps, exchange_interval, seed, instance,
@@ -307,6 +364,7 @@ class DenseBatch:
             _check(L.os_dense_batch_create(ctx.handle, self.I, self.n, _p(q, _f64p),
                                            ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def q(self):
         """The batch's FP64 matrices [I, n, n] (copied from the GPU)."""
@@ -331,6 +389,14 @@ class DenseBatch:
         instance-major, [I, R] and [I, R, n]."""
         return _tempering(self._h, self.I, self.n, ladders, tscale, num_sweeps, exchange_interval,
                           seed, instance, refresh_interval, best_states, final_states)
+
+    def population(self, num_replicas, tscale, sweeps_per_step, seed=1, instance=0,
+                   refresh_interval=0, best_states=True, final_states=False):
+        """Population annealing on every instance (os_dense_population);
+        arrays are instance-major, [I, R], [I, R, n] and ancestors
+        [I, steps - 1, R]."""
+        return _population(self._h, self.I, self.n, num_replicas, tscale, sweeps_per_step, seed,
+                           instance, refresh_interval, best_states, final_states)
 
     def anneal(self, num_replicas, tscale, seed=1, instance=0, refresh_interval=0,
                init_states=None, init_marginals=None, best_states=True, final_states=False):
@@ -399,6 +465,7 @@ class DeviceBytes:
         if self.size and self._hip.hipMemcpy(self.ptr, data, ctypes.c_size_t(self.size),
                                              ctypes.c_int(1)) != 0:
             raise OneSolverError(-2, "hipMemcpy failed")
+        ctx._adopt(self)
 
     def close(self):
         if self.ptr:
@@ -443,6 +510,7 @@ class Qubo:
             _check(L.os_qubo_read_binary(ctx.handle, data, len(data), ctypes.byref(h)))
         self._h = h
         self.ctx = ctx
+        ctx._adopt(self)
         n, mx = ctypes.c_int32(), ctypes.c_int32()
         nl, nq = ctypes.c_int64(), ctypes.c_int64()
         _check(L.os_qubo_info(h, ctypes.byref(n), ctypes.byref(mx), ctypes.byref(nl),
@@ -521,6 +589,14 @@ class CsrParams(ctypes.Structure):
                 ("site_order", ctypes.c_int32)]
 
 
+class CsrPopulationParams(ctypes.Structure):
+    """os_csr_population_params of include/onesolver.h."""
+    _fields_ = [("num_replicas", ctypes.c_uint32), ("num_steps", ctypes.c_int32),
+                ("sweeps_per_step", ctypes.c_int32), ("tscale", _vp), ("seed", ctypes.c_uint64),
+                ("instance_base", ctypes.c_uint32), ("site_order", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32)]
+
+
 def _csr_lib():
     L = lib()
     if not getattr(L, "_csr_ready", False):
@@ -529,6 +605,9 @@ def _csr_lib():
         L.os_csr_batch_destroy.argtypes = [_vp]
         L.os_csr_batch_anneal.argtypes = [_vp, ctypes.POINTER(CsrParams), _f64p, _f64p, _u8p,
                                           _u8p, _f64p, _i32p, ctypes.POINTER(SweepStats)]
+        L.os_csr_batch_population.argtypes = [_vp, ctypes.POINTER(CsrPopulationParams), _f64p,
+                                              _u8p, _u8p, ctypes.POINTER(ctypes.c_uint32),
+                                              _f64p, _i32p, ctypes.POINTER(SweepStats)]
         L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
         L.os_csr_batch_colour_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
         L.os_csr_batch_site_order.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]
@@ -555,6 +634,7 @@ class CsrBatch:
                                               _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f64p),
                                               _p(dg, _f64p), ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if self._h:
@@ -621,18 +701,44 @@ class CsrBatch:
                                               _p(rb, _f64p), _p(bs, _u8p), _p(fs, _u8p),
                                               _p(ib, _f64p), _p(ir, _i32p), ctypes.byref(st)))
 
-        def split(flat):
-            if flat is None:
-                return None
-            out, at = [], 0
-            for n in self.ns:
-                out.append(flat[at:at + R * n].reshape(R, n))
-                at += R * n
-            return out
+        return {"best_energies": be, "run_best_energies": rb, "best_states": self._split(bs, R),
+                "final_states": self._split(fs, R), "inst_best_energy": ib,
+                "inst_best_replica": ir, "stats": st.as_dict()}
 
-        return {"best_energies": be, "run_best_energies": rb, "best_states": split(bs),
-                "final_states": split(fs), "inst_best_energy": ib, "inst_best_replica": ir,
-                "stats": st.as_dict()}
+    def _split(self, flat, R):
+        """Per-instance [R, n_i] views of an instance-major state array."""
+        if flat is None:
+            return None
+        out, at = [], 0
+        for n in self.ns:
+            out.append(flat[at:at + R * n].reshape(R, n))
+            at += R * n
+        return out
+
+    def population(self, num_replicas, tscale, sweeps_per_step, seed=1, instance_base=0,
+                   best_states=True, final_states=False, use_graph=True, tune=True,
+                   site_order=0):
+        """Population annealing (os_csr_batch_population) through the
+        len(tscale) temperature steps; ancestors [I, steps - 1, R]."""
+        ts = np.ascontiguousarray(tscale, dtype=np.float32)
+        I, R, K = len(self.ns), int(num_replicas), len(ts)
+        prm = CsrPopulationParams(R, K, sweeps_per_step, ts.ctypes.data, seed, instance_base,
+                                  int(site_order), int(use_graph), int(tune))
+        tot = int(self.ns.sum()) * R
+        be = np.zeros((I, R), dtype=np.float64)
+        bs = np.zeros(tot, dtype=np.uint8) if best_states else None
+        fs = np.zeros(tot, dtype=np.uint8) if final_states else None
+        anc = np.zeros((I, max(K - 1, 0), R), dtype=np.uint32)
+        ib = np.zeros(I, dtype=np.float64)
+        ir = np.zeros(I, dtype=np.int32)
+        st = SweepStats()
+        _check(_csr_lib().os_csr_batch_population(
+            self._h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p), _p(fs, _u8p),
+            anc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(ib, _f64p), _p(ir, _i32p),
+            ctypes.byref(st)))
+        return {"best_energies": be, "best_states": self._split(bs, R),
+                "final_states": self._split(fs, R), "ancestors": anc, "inst_best_energy": ib,
+                "inst_best_replica": ir, "stats": st.as_dict()}
 
 
 def _mpnn_lib():
@@ -662,6 +768,7 @@ class WarmStartNet:
         h = _vp()
         _check(L.os_mpnn_create(ctx.handle, seed, ctypes.byref(h)))
         self._h = h
+        ctx._adopt(self)
 
     def close(self):
         if self._h:

@@ -14,7 +14,8 @@
 // Per site (reference semantics of a Metropolis sweep, in the oracle's
 // order): h = diag_j, then h = h + w_jk for each set neighbour in row order
 // (fp32, no contraction); dE = x_j ? -h : h; flip iff dE < neg_ln_u(w) *
-// tscale[t], w = Philox word of counter (j>>2, replica, sweep, tag); the
+// tscale[t], w = Philox word of counter (j>>2, replica, t, tag), t the
+// sweep's number (run->sweep_base + the launch's own sweep index); the
 // FP64 running energy adds dE.  At the end of every sweep a replica whose
 // running energy is below its best copies its state to the best-state planes.
 #include "os_kernels.h"
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
   const uint32_t k0 = run->k0, k1 = run->k1;
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
+  const int tb = run->sweep_base;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(queue, 1);
@@ -118,10 +120,10 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
     double eb = e;
     for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
     for (int t = 0; t < S; ++t) {
-      const float ts = tscale[t];
+      const float ts = tscale[tb + t];
       for (int j4 = 0; j4 < n; j4 += 4) {
         const uint4 o = philox4x32_10(
-            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)(tb + t), id | TAG_SWEEP_THR), k0, k1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int j = j4 + q;
@@ -223,6 +225,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
   const uint32_t k0 = run->k0, k1 = run->k1;
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
+  const int tb = run->sweep_base;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(queue, 1);
@@ -267,14 +270,14 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
     double eb = e;
     for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
     for (int t = 0; t < S; ++t) {
-      const float ts = tscale[t];
+      const float ts = tscale[tb + t];
       SiteRow row = load_row(ecol, eval, dg, 0, chunks, n);
       // the state word of the current sites (j >> 5), kept in a register:
       // flips update it and are written through to LDS for the gathers
       uint32_t word = 0u;
       for (int j4 = 0; j4 < n; j4 += 4) {
         const uint4 o = philox4x32_10(
-            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
+            make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)(tb + t), id | TAG_SWEEP_THR), k0, k1);
         uint32_t *xw = &xs[(j4 >> 5) * 64 + lane];
         if ((j4 & 31) == 0) word = *xw;
 #pragma unroll
@@ -311,32 +314,39 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
 // (greedy colouring: no two sites of a class are coupled), every class in
 // ascending index order; within a class the updates are independent, so a
 // wave updates G sites of each of its 64/G replicas per step: lane =
-// replica slot * G + site slot.  Site j keeps its threshold (Philox counter
-// (j>>2, r, t)), its field order and its dE, so the chain is the sequential
-// sweep's with the order of the class list; the FP64 running energy adds the
-// dE of a sweep per lane and then over the G lanes of a replica.  Flips are
-// atomic XORs on the LDS planes, since sites of one class can share a word.
+// replica slot * G + site slot.  The update at position pos of the visiting
+// order draws word pos&3 of Philox counter (pos>>2, r, t) -- for the order
+// 0..n-1 the stream of the sequential kernels -- and each site keeps its
+// field order and dE; the FP64 running energy adds the dE of a sweep per
+// lane and then over the G lanes of a replica.  Thresholds are computed for
+// windows of 4G positions (one Philox call per lane, four logarithms) into
+// LDS.  Flips are atomic XORs on the LDS planes, since sites of one class
+// can share a word.
 template <int kW, int G>
 __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
     const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
     const float4 *__restrict__ ell_val, const float *__restrict__ diag,
     const int32_t *__restrict__ corder, const int32_t *__restrict__ cstart,
-    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
-    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
-    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
-    int max_words) {
+    const int4 *__restrict__ pell_col, const float4 *__restrict__ pell_val,
+    const float *__restrict__ pdiag, const float *__restrict__ tscale,
+    const uint32_t *__restrict__ init_bits, uint32_t *__restrict__ best_bits,
+    uint32_t *__restrict__ final_bits, double *__restrict__ e_best, double *__restrict__ e_final,
+    int *__restrict__ queue, int max_words) {
   static_assert(G == 4 || G == 8 || G == 16, "sites per step");
   constexpr int RW = 64 / G;  // replicas per wave
   extern __shared__ uint32_t lds[];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int rr = lane / G, s = lane % G;
-  uint32_t *xs = lds + (size_t)w * max_words * RW;
+  uint32_t *xs = lds + (size_t)w * (max_words * RW + 4 * 64);
+  // thresholds of the positions [win, win + 4G) for the RW replicas
+  float *thw = reinterpret_cast<float *>(xs + max_words * RW);
   const int num_items = run->num_items, S = run->num_sweeps, Rp = run->Rp;
   const uint32_t k0 = run->k0, k1 = run->k1;
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
+  const int tb = run->sweep_base;
   auto bit = [&](int j) { return (xs[(j >> 5) * RW + rr] >> (j & 31)) & 1u; };
   for (;;) {
     int it = 0;
@@ -353,6 +363,10 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     const float *dg = diag + d.spin_off;
     const int32_t *ord = corder + d.spin_off;
     const int32_t *cs = cstart + d.class_off;
+    // the rows and diagonal in visiting order (position pos: site ord[pos])
+    const int4 *pcol = pell_col + (d.ell_off >> 2);
+    const float4 *pval = pell_val + (d.ell_off >> 2);
+    const float *pdg = pdiag + d.spin_off;
     uint32_t *bb = best_bits + d.state_off + r;
     // start state, as in csr_sweep_kernel; the G lanes of a replica share
     // its words
@@ -387,35 +401,96 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     double eb = e;
     for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
     for (int t = 0; t < S; ++t) {
-      const float ts = tscale[t];
+      const float ts = tscale[tb + t];
       double de_sum = 0.0;
-      for (int c = 0; c < ncol; ++c) {
-        const int c1 = cs[c + 1];
-        for (int p0 = cs[c]; p0 < c1; p0 += G) {
-          const int pos = p0 + s;
-          const bool active = pos < c1;
-          const int j = active ? ord[pos] : 0;
-          const uint4 o = philox4x32_10(
-              make_uint4((uint32_t)(j >> 2), r, (uint32_t)t, id | TAG_SWEEP_THR), k0, k1);
-          const float thr = neg_ln_u(pick(o, (uint32_t)j & 3u)) * ts;
-          float h = dg[j];
-          for (int q = 0; q < chunks; ++q) {
-            const int4 cc = ecol[(size_t)j * chunks + q];
-            const float4 vv = eval[(size_t)j * chunks + q];
-            h = bit(cc.x) ? h + vv.x : h;
-            h = bit(cc.y) ? h + vv.y : h;
-            h = bit(cc.z) ? h + vv.z : h;
-            h = bit(cc.w) ? h + vv.w : h;
-          }
-          const float de = bit(j) ? -h : h;
-          if (active && de < thr) {
-            atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
-            de_sum += (double)de;
+      int win = -4 * G;  // no window yet
+      // steps of the sweep: positions [p0, p0 + G) of class c (ending at
+      // c1); the rows of the next step are requested before this one runs
+      int c = 0, p0 = cs[0], c1 = cs[1];
+      // the first two chunks of a row travel with the step; rows of more
+      // than 8 couplings read the rest when the step runs
+      struct Step {
+        int j, pos;
+        float dj;
+        int4 c0, c1;
+        float4 v0, v1;
+      };
+      auto fetch = [&](int q0) {
+        Step st;
+        const int pos = min(q0 + s, n - 1);
+        st.pos = pos;
+        st.j = ord[pos];
+        st.dj = pdg[pos];
+        st.c0 = pcol[(size_t)pos * chunks];
+        st.v0 = pval[(size_t)pos * chunks];
+        st.c1 = make_int4(n, n, n, n);
+        st.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (chunks > 1) {
+          st.c1 = pcol[(size_t)pos * chunks + 1];
+          st.v1 = pval[(size_t)pos * chunks + 1];
+        }
+        return st;
+      };
+      Step cur = fetch(p0);
+      for (;;) {
+        int np0 = p0 + G, nc = c, nc1 = c1;
+        if (np0 >= c1) {
+          nc = c + 1;
+          if (nc < ncol) {
+            np0 = cs[nc];
+            nc1 = cs[nc + 1];
           }
         }
-        // the next class reads this class's flips
-        wave_sync();
+        const bool more = nc < ncol;
+        Step nxt = cur;
+        if (more) nxt = fetch(np0);
+        if (p0 + G > win + 4 * G) {
+          // next window: lane (rr, s) draws positions win + 4s .. +3
+          win = p0 & ~3;
+          const uint4 o = philox4x32_10(make_uint4((uint32_t)((win >> 2) + s), r,
+                                                   (uint32_t)(tb + t), id | TAG_SWEEP_THR),
+                                        k0, k1);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) thw[(4 * s + q) * RW + rr] = neg_ln_u(pick(o, (uint32_t)q)) * ts;
+          wave_sync();
+        }
+        const int pos = p0 + s;
+        const bool active = pos < c1;
+        const float thr = thw[(pos - win) * RW + rr];
+        float h = cur.dj;
+        h = bit(cur.c0.x) ? h + cur.v0.x : h;
+        h = bit(cur.c0.y) ? h + cur.v0.y : h;
+        h = bit(cur.c0.z) ? h + cur.v0.z : h;
+        h = bit(cur.c0.w) ? h + cur.v0.w : h;
+        if (chunks > 1) {
+          h = bit(cur.c1.x) ? h + cur.v1.x : h;
+          h = bit(cur.c1.y) ? h + cur.v1.y : h;
+          h = bit(cur.c1.z) ? h + cur.v1.z : h;
+          h = bit(cur.c1.w) ? h + cur.v1.w : h;
+        }
+        for (int cc = 2; cc < chunks; ++cc) {
+          const int4 ci = pcol[(size_t)cur.pos * chunks + cc];
+          const float4 vi = pval[(size_t)cur.pos * chunks + cc];
+          h = bit(ci.x) ? h + vi.x : h;
+          h = bit(ci.y) ? h + vi.y : h;
+          h = bit(ci.z) ? h + vi.z : h;
+          h = bit(ci.w) ? h + vi.w : h;
+        }
+        const int j = cur.j;
+        const float de = bit(j) ? -h : h;
+        if (active && de < thr) {
+          atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
+          de_sum += (double)de;
+        }
+        if (!more) break;
+        // a new class reads this class's flips
+        if (nc != c) wave_sync();
+        cur = nxt;
+        p0 = np0;
+        c1 = nc1;
+        c = nc;
       }
+      wave_sync();
 #pragma unroll
       for (int off = 1; off < G; off <<= 1) de_sum += __shfl_xor(de_sum, off, 64);
       e += de_sum;
@@ -465,6 +540,54 @@ __global__ __launch_bounds__(64) void csr_energy_kernel(
     e += s;
   }
   out[d.rep_off + r] = e;
+}
+
+// Population annealing: thread = (item, lane) = replica r of the item's
+// instance; only replicas below R exist for the caller, the rest keep their
+// planes.
+__global__ __launch_bounds__(64) void csr_pop_best_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items, int num_items, int Rp,
+    int first, const double *__restrict__ eb, const uint32_t *__restrict__ best,
+    double *__restrict__ pop_eb, uint32_t *__restrict__ pop_best) {
+  const int it = blockIdx.x;
+  if (it >= num_items) return;
+  const int2 wi = items[it];
+  const CsrInst d = inst[wi.x];
+  const uint32_t r = (uint32_t)(wi.y * 64 + threadIdx.x);
+  const double e = eb[d.rep_off + r];
+  if (!first && !(e < pop_eb[d.rep_off + r])) return;
+  pop_eb[d.rep_off + r] = e;
+  for (int w = 0; w < d.words; ++w)
+    pop_best[d.state_off + (size_t)w * Rp + r] = best[d.state_off + (size_t)w * Rp + r];
+}
+
+__global__ __launch_bounds__(64) void csr_pop_gather_kernel(
+    const CsrInst *__restrict__ inst, const int2 *__restrict__ items, int num_items, int Rp,
+    int R, const uint32_t *__restrict__ anc, const uint32_t *__restrict__ fin,
+    uint32_t *__restrict__ init) {
+  const int it = blockIdx.x;
+  if (it >= num_items) return;
+  const int2 wi = items[it];
+  const CsrInst d = inst[wi.x];
+  const int r = wi.y * 64 + (int)threadIdx.x;
+  const int src = r < R ? (int)anc[(size_t)wi.x * R + r] : r;
+  for (int w = 0; w < d.words; ++w)
+    init[d.state_off + (size_t)w * Rp + r] = fin[d.state_off + (size_t)w * Rp + src];
+}
+
+void launch_csr_pop_best(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                         int Rp, bool first, const double *eb, const uint32_t *best,
+                         double *pop_eb, uint32_t *pop_best) {
+  csr_pop_best_kernel<<<num_items, 64, 0, st>>>(inst, items, num_items, Rp, first ? 1 : 0, eb,
+                                                best, pop_eb, pop_best);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+void launch_csr_pop_gather(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                           int Rp, int R, const uint32_t *anc, const uint32_t *fin,
+                           uint32_t *init) {
+  csr_pop_gather_kernel<<<num_items, 64, 0, st>>>(inst, items, num_items, Rp, R, anc, fin, init);
+  OS_HIP_CHECK(hipGetLastError());
 }
 
 int csr_lds_bytes(int waves, int max_words) { return waves * max_words * 64 * 4; }
@@ -552,7 +675,7 @@ int csr_max_groups_per_cu(bool pf, int waves, int max_words) {
 }
 
 int csr_col_lds_bytes(int waves, int sites, int max_words) {
-  return waves * max_words * (64 / sites) * 4;
+  return waves * (max_words * (64 / sites) + 4 * 64) * 4;
 }
 
 namespace {
@@ -591,7 +714,8 @@ int csr_col_max_groups_per_cu(int waves, int sites, int max_words) {
 void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
                            const CsrInst *inst, const int2 *items, const CsrRun *run,
                            const int4 *ell_col, const float4 *ell_val, const float *diag,
-                           const int32_t *corder, const int32_t *cstart, const float *tscale,
+                           const int32_t *corder, const int32_t *cstart, const int4 *pell_col,
+                           const float4 *pell_val, const float *pdiag, const float *tscale,
                            const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
                            double *e_best, double *e_final, int *queue) {
   const size_t lds = (size_t)csr_col_lds_bytes(waves, sites, max_words);
@@ -601,8 +725,8 @@ void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int
 #define OS_COL_CASE(W, GG)                                                                      \
   if (waves == W && sites == GG) {                                                              \
     csr_sweep_col_kernel<W, GG><<<groups, 64 * W, lds, st>>>(                                   \
-        inst, items, run, ell_col, ell_val, diag, corder, cstart, tscale, init_bits, best_bits, \
-        final_bits, e_best, e_final, queue, max_words);                                         \
+        inst, items, run, ell_col, ell_val, diag, corder, cstart, pell_col, pell_val, pdiag,    \
+        tscale, init_bits, best_bits, final_bits, e_best, e_final, queue, max_words);          \
   }
   OS_COL_CASE(1, 4)
   OS_COL_CASE(1, 8)

@@ -597,6 +597,33 @@ __global__ __launch_bounds__(256) void warm_init_kernel(int n, int npw, int R, i
   xbits[idx] = v;
 }
 
+// Population annealing: row (instance i, r) of the new population is row
+// (i, anc[i][r]) of the old one for r < R; thread = (row, column).
+__global__ __launch_bounds__(256) void dense_gather_kernel(int Rp, int R, int npw, int np,
+                                                           const uint32_t *__restrict__ anc,
+                                                           const uint32_t *__restrict__ x,
+                                                           const float *__restrict__ f,
+                                                           const double *__restrict__ e,
+                                                           uint32_t *__restrict__ x2,
+                                                           float *__restrict__ f2,
+                                                           double *__restrict__ e2) {
+  const int row = blockIdx.y;
+  const int col = blockIdx.x * 256 + threadIdx.x;
+  const int inst = row / Rp, r = row % Rp;
+  const int src = r < R ? inst * Rp + (int)anc[(size_t)inst * R + r] : row;
+  if (col < np) f2[(size_t)row * np + col] = f[(size_t)src * np + col];
+  if (col < npw) x2[(size_t)row * npw + col] = x[(size_t)src * npw + col];
+  if (col == 0) e2[row] = e[src];
+}
+
+void launch_dense_gather(hipStream_t st, int num_instances, int Rp, int R, int npw, int np,
+                         const uint32_t *anc, const uint32_t *x, const float *f, const double *e,
+                         uint32_t *x2, float *f2, double *e2) {
+  const dim3 grid((unsigned)((np + 255) / 256), (unsigned)(num_instances * Rp));
+  dense_gather_kernel<<<grid, 256, 0, st>>>(Rp, R, npw, np, anc, x, f, e, x2, f2, e2);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
 int dense_max_n() { return kWaves * 32 * 8; }
 
 // Column tiles (32 columns) per wave are instantiated for these counts; n is

@@ -38,7 +38,6 @@ __device__ __forceinline__ sf32x16 smfma(uint4 a, uint4 b, sf32x16 c) {
                                                  __builtin_bit_cast(sbf16x8, b), c, 0, 0, 0);
 }
 
-constexpr int kChunk = 2;     // decide launch: blocks whose operands load in one round trip
 constexpr int kPRows = 256;   // pass launch: replicas per tile (4 waves x 64)
 constexpr int kBStride = 272; // pass launch: bytes per staged operand column (256 + pad)
 constexpr int kQ4 = 20;       // decide launch: floats per (spin, lane-in-quad) row of qd4
@@ -240,6 +239,10 @@ __global__ __launch_bounds__(256) void stream_pass_kernel(int np, int Rp, int I,
 // corrects its own later spins with the fp32 diagonal sub-block of Q.  The
 // waves run at raised priority: the launch shares the CUs with the pass
 // launch and is on the critical path.
+//
+// KC: pending blocks whose operands load in one round trip; PIPE: the next
+// chunk's operands are requested before the current chunk is applied.
+template <int KC, bool PIPE>
 __global__ __launch_bounds__(256) void stream_decide_kernel(
     int n, int np, int Rp, int g, int gb, int ring, size_t T, int sweep, float ts, uint32_t k0,
     uint32_t k1, uint32_t inst_base, int last_of_sweep, const uint4 *__restrict__ qpack,
@@ -267,7 +270,7 @@ __global__ __launch_bounds__(256) void stream_decide_kernel(
   const size_t off = (row0 + 32 * mt + 4 * kh) * np + b * kSB + 32 * nt + lc;
   const uint4 *qcol = qpack + ((size_t)s * NB * np + b * kSB + 32 * nt + lc) * 16;
   sf32x16 C = tile_load(Fin + off, np);
-  TileOps<kChunk> o;
+  TileOps<KC> o;
   ops_load(o, gb, cnt, np, NB, ring, T, drow, qcol, dring, kh);
   const int rep = 16 * w + (lane >> 2), r4 = lane & 3;
   const size_t rr = row0 + rep;
@@ -287,10 +290,20 @@ __global__ __launch_bounds__(256) void stream_decide_kernel(
       thr[r][4 * q4 + jj] = j0 + jj < n ? x : -INFINITY;
     }
   }
-  ops_apply(C, o, cnt, kh);
-  for (int c0 = kChunk; c0 < cnt; c0 += kChunk) {
-    ops_load(o, gb + c0, cnt - c0, np, NB, ring, T, drow, qcol, dring, kh);
-    ops_apply(C, o, cnt - c0, kh);
+  if (PIPE) {
+    TileOps<KC> o2;
+    for (int c0 = 0; c0 < cnt; c0 += KC) {
+      if (c0 + KC < cnt) ops_load(o2, gb + c0 + KC, cnt - c0 - KC, np, NB, ring, T, drow, qcol,
+                                  dring, kh);
+      ops_apply(C, o, cnt - c0, kh);
+      o = o2;
+    }
+  } else {
+    ops_apply(C, o, cnt, kh);
+    for (int c0 = KC; c0 < cnt; c0 += KC) {
+      ops_load(o, gb + c0, cnt - c0, np, NB, ring, T, drow, qcol, dring, kh);
+      ops_apply(C, o, cnt - c0, kh);
+    }
   }
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -450,10 +463,24 @@ void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, i
                           int sweep, float ts, uint64_t seed, uint32_t inst_base,
                           bool last_of_sweep, const uint4 *qpack, const float *qdb,
                           const float *Fin, uint32_t *xbits, uint4 *dring, double *energy,
-                          double *ebest, uint32_t *bestbits) {
-  stream_decide_kernel<<<dim3(Rp / 64, I), 256, 0, st>>>(
-      n, np, Rp, g, gb, ring, (size_t)I * Rp, sweep, ts, (uint32_t)seed, (uint32_t)(seed >> 32),
-      inst_base, last_of_sweep ? 1 : 0, qpack, qdb, Fin, xbits, dring, energy, ebest, bestbits);
+                          double *ebest, uint32_t *bestbits, int variant) {
+  const dim3 grid(Rp / 64, I);
+  const size_t T = (size_t)I * Rp;
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  const int lo = last_of_sweep ? 1 : 0;
+#define OS_DECIDE(KC, PIPE)                                                                   \
+  stream_decide_kernel<KC, PIPE><<<grid, 256, 0, st>>>(n, np, Rp, g, gb, ring, T, sweep, ts, k0, \
+                                                       k1, inst_base, lo, qpack, qdb, Fin, xbits, \
+                                                       dring, energy, ebest, bestbits)
+  switch (variant) {
+    case 21: OS_DECIDE(2, true); break;
+    case 30: OS_DECIDE(3, false); break;
+    case 31: OS_DECIDE(3, true); break;
+    case 40: OS_DECIDE(4, false); break;
+    case 50: OS_DECIDE(5, false); break;
+    default: OS_DECIDE(2, false); break;
+  }
+#undef OS_DECIDE
   OS_HIP_CHECK(hipGetLastError());
 }
 

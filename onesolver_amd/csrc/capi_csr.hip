@@ -13,6 +13,7 @@
 // greedily when the batch is created (index order, smallest colour free of
 // lower-index neighbours) for the colour-class site order.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -37,9 +38,14 @@ struct CsrBatch {
   // (offset class_off)
   DevBuf corder, cstart;
   std::vector<int32_t> h_order, h_cstart;
+  // ELL rows and diagonal in that visiting order
+  DevBuf pell_col, pell_val, pdiag;
   // per run shape
   int Rp = 0;
   DevBuf items, run, tscale, init, best, fin, eb, ef, e64, queue;
+  // population annealing: best planes and energies of every replica index
+  // over the steps, ancestors of a resampling round
+  DevBuf pop_best, pop_eb, pop_anc;
   int num_items = 0;
   // launch shape chosen by the tuner (0: not tuned); reg: row-prefetching kernel
   int waves = 0, groups = 0;
@@ -70,7 +76,8 @@ struct CsrBatch {
     drop_graph();
     for (auto *b : {&d_inst, &ell_col, &ell_val, &diag, &diag64, &row_ptr, &col, &val64, &items,
                     &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &corder,
-                    &cstart, &items_c})
+                    &cstart, &items_c, &pell_col, &pell_val, &pdiag, &pop_best, &pop_eb,
+                    &pop_anc})
       b->release();
   }
 
@@ -84,6 +91,7 @@ struct CsrBatch {
       launch_csr_col_sweeps(st, w, sites, g, max_words, d_inst.as<CsrInst>(), items_c.as<int2>(),
                             run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(),
                             diag.as<float>(), corder.as<int32_t>(), cstart.as<int32_t>(),
+                            pell_col.as<int4>(), pell_val.as<float4>(), pdiag.as<float>(),
                             tscale.as<float>(), init.as<uint32_t>(), best.as<uint32_t>(),
                             fin.as<uint32_t>(), eb.as<double>(), ef.as<double>(),
                             queue.as<int>());
@@ -246,6 +254,152 @@ void tune(CsrBatch &b, const CsrRun &full) {
   OS_REQUIRE(b.waves > 0, "csr batch: no launch shape fits (instance too large for LDS)");
 }
 
+// The per-sweep scales of a run in device memory (a new buffer drops the
+// captured graph).
+void put_tscale(CsrBatch &b, const float *ts, int S) {
+  b.tscale.ensure(sizeof(float) * std::max(S, 1));
+  if (S)
+    OS_HIP_CHECK(hipMemcpyAsync(b.tscale.ptr, ts, sizeof(float) * S, hipMemcpyHostToDevice,
+                                b.ctx->stream));
+  if (b.g_tscale != b.tscale.ptr) b.drop_graph();
+}
+
+struct Shape {
+  bool rg;
+  int waves, groups, sites;
+};
+
+// The launch shape of a run in `site_order` (tuned on first use when `tune`).
+Shape select_shape(CsrBatch &b, const CsrRun &run, int site_order, bool tune_it) {
+  Ctx &c = *b.ctx;
+  b.mode = site_order;
+  if (b.mode == 0 && b.waves == 0) {
+    if (tune_it) {
+      tune(b, run);
+    } else {
+      b.reg = csr_pf_eligible(b.max_width);
+      b.waves = 1;
+      b.groups = std::max(1, std::min(c.num_cus * csr_max_groups_per_cu(b.reg, 1, b.max_words),
+                                      b.num_items));
+    }
+  }
+  // OS_CSR_COLOUR_SITES (4, 8 or 16) fixes the colour kernel's sites per step
+  if (b.mode == 1) {
+    const char *senv = std::getenv("OS_CSR_COLOUR_SITES");
+    const int forced = senv ? std::atoi(senv) : 0;
+    OS_REQUIRE(!senv || forced == 4 || forced == 8 || forced == 16,
+               "OS_CSR_COLOUR_SITES must be 4, 8 or 16");
+    if (forced && forced != b.c_sites) {
+      b.c_sites = forced;
+      b.c_waves = 1;
+      col_items(b, b.c_sites);
+      b.c_groups = std::max(
+          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
+                      b.num_items_c));
+    }
+  }
+  if (b.mode == 1 && b.c_waves == 0) {
+    if (tune_it) {
+      tune_col(b, run);
+    } else {
+      b.c_sites = 8;
+      b.c_waves = 1;
+      col_items(b, b.c_sites);
+      b.c_groups = std::max(
+          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
+                      b.num_items_c));
+    }
+  }
+  if (b.mode == 1) col_items(b, b.c_sites);
+  Shape sh;
+  sh.rg = b.mode == 0 && b.reg;
+  sh.waves = b.mode == 1 ? b.c_waves : b.waves;
+  sh.groups = b.mode == 1 ? b.c_groups : b.groups;
+  sh.sites = b.mode == 1 ? b.c_sites : 0;
+  return sh;
+}
+
+// Captures (reset, sweeps, FP64 energies of the best states) for `sh` unless
+// the graph already holds that shape.
+void ensure_graph(CsrBatch &b, const Shape &sh) {
+  hipStream_t s = b.ctx->stream;
+  if (b.gexec && b.g_mode == b.mode && b.g_reg == sh.rg && b.g_waves == sh.waves &&
+      b.g_groups == sh.groups && b.g_sites == sh.sites && b.g_Rp == b.Rp)
+    return;
+  b.drop_graph();
+  OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    b.enqueue(sh.rg, sh.waves, sh.groups, true, sh.sites);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  OS_HIP_CHECK(hipStreamEndCapture(s, &b.graph));
+  OS_HIP_CHECK(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
+  b.g_mode = b.mode;
+  b.g_reg = sh.rg;
+  b.g_waves = sh.waves;
+  b.g_groups = sh.groups;
+  b.g_sites = sh.sites;
+  b.g_Rp = b.Rp;
+  b.g_tscale = b.tscale.ptr;
+}
+
+// Copies a run's results to the caller: FP64 energies of the best states
+// (b.e64), the states of the `best` and `fin` planes and, when `eb` is
+// given, the running energies it holds; synchronises the stream.
+void collect(CsrBatch &b, int R, DevBuf &best, DevBuf &fin, DevBuf *eb, double *best_energies,
+             double *run_best_energies, uint8_t *best_states, uint8_t *final_states,
+             double *inst_best_energy, int32_t *inst_best_replica) {
+  hipStream_t s = b.ctx->stream;
+  const int Rp = b.Rp;
+  std::vector<double> e64((size_t)b.I * Rp);
+  OS_HIP_CHECK(hipMemcpyAsync(e64.data(), b.e64.ptr, sizeof(double) * e64.size(),
+                              hipMemcpyDeviceToHost, s));
+  std::vector<double> ebh;
+  if (run_best_energies && eb) {
+    ebh.resize((size_t)b.I * Rp);
+    OS_HIP_CHECK(hipMemcpyAsync(ebh.data(), eb->ptr, sizeof(double) * ebh.size(),
+                                hipMemcpyDeviceToHost, s));
+  }
+  std::vector<uint32_t> bp, fp;
+  if (best_states) {
+    bp.resize((size_t)b.total_words * Rp);
+    OS_HIP_CHECK(hipMemcpyAsync(bp.data(), best.ptr, sizeof(uint32_t) * bp.size(),
+                                hipMemcpyDeviceToHost, s));
+  }
+  if (final_states) {
+    fp.resize((size_t)b.total_words * Rp);
+    OS_HIP_CHECK(hipMemcpyAsync(fp.data(), fin.ptr, sizeof(uint32_t) * fp.size(),
+                                hipMemcpyDeviceToHost, s));
+  }
+  OS_HIP_CHECK(hipStreamSynchronize(s));
+  uint8_t *bs = best_states, *fs = final_states;
+  for (int i = 0; i < b.I; ++i) {
+    const auto &d = b.inst[i];
+    int arg = 0;
+    for (int r = 0; r < R; ++r) {
+      const double e = e64[(size_t)d.rep_off + r];
+      if (best_energies) best_energies[(size_t)i * R + r] = e;
+      if (!ebh.empty()) run_best_energies[(size_t)i * R + r] = ebh[(size_t)d.rep_off + r];
+      if (e < e64[(size_t)d.rep_off + arg]) arg = r;
+    }
+    if (inst_best_energy) inst_best_energy[i] = e64[(size_t)d.rep_off + arg];
+    if (inst_best_replica) inst_best_replica[i] = arg;
+    if (bs || fs)
+      for (int r = 0; r < R; ++r)
+        for (int j = 0; j < d.n; ++j) {
+          const size_t at = (size_t)d.state_off + (size_t)(j >> 5) * Rp + r;
+          if (bs) bs[(size_t)r * d.n + j] = (uint8_t)((bp[at] >> (j & 31)) & 1u);
+          if (fs) fs[(size_t)r * d.n + j] = (uint8_t)((fp[at] >> (j & 31)) & 1u);
+        }
+    if (bs) bs += (size_t)R * d.n;
+    if (fs) fs += (size_t)R * d.n;
+  }
+}
+
 }  // namespace
 
 }  // namespace os
@@ -364,6 +518,32 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
       OS_HIP_CHECK(hipMemcpyAsync(b->col.ptr, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, s));
       OS_HIP_CHECK(hipMemcpyAsync(b->val64.ptr, val, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
     }
+    // the ELL rows and diagonal in each instance's colour-class order
+    {
+      std::vector<int32_t> pc(ell_c.size());
+      std::vector<float> pv(ell_v.size()), pd(dg.size());
+      for (const auto &d : b->inst) {
+        const int32_t *ord = b->h_order.data() + d.spin_off;
+        for (int pos = 0; pos < d.n; ++pos) {
+          const int j = ord[pos];
+          std::memcpy(&pc[(size_t)(d.ell_off + (int64_t)pos * d.width)],
+                      &ell_c[(size_t)(d.ell_off + (int64_t)j * d.width)], sizeof(int32_t) * d.width);
+          std::memcpy(&pv[(size_t)(d.ell_off + (int64_t)pos * d.width)],
+                      &ell_v[(size_t)(d.ell_off + (int64_t)j * d.width)], sizeof(float) * d.width);
+          pd[(size_t)(d.spin_off + pos)] = dg[(size_t)(d.spin_off + j)];
+        }
+      }
+      b->pell_col.ensure(sizeof(int32_t) * std::max<size_t>(pc.size(), 4));
+      b->pell_val.ensure(sizeof(float) * std::max<size_t>(pv.size(), 4));
+      b->pdiag.ensure(sizeof(float) * std::max<size_t>(pd.size(), 1));
+      OS_HIP_CHECK(hipMemcpyAsync(b->pell_col.ptr, pc.data(), sizeof(int32_t) * pc.size(),
+                                  hipMemcpyHostToDevice, s));
+      OS_HIP_CHECK(hipMemcpyAsync(b->pell_val.ptr, pv.data(), sizeof(float) * pv.size(),
+                                  hipMemcpyHostToDevice, s));
+      OS_HIP_CHECK(hipMemcpyAsync(b->pdiag.ptr, pd.data(), sizeof(float) * pd.size(),
+                                  hipMemcpyHostToDevice, s));
+      OS_HIP_CHECK(hipStreamSynchronize(s));
+    }
     b->corder.ensure(sizeof(int32_t) * b->h_order.size());
     b->cstart.ensure(sizeof(int32_t) * b->h_cstart.size());
     OS_HIP_CHECK(hipMemcpyAsync(b->corder.ptr, b->h_order.data(),
@@ -405,10 +585,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     const int Rp = (R + 63) / 64 * 64;
     const int S = p->num_sweeps;
     os::shape_for(b, Rp);
-    b.tscale.ensure(sizeof(float) * std::max(S, 1));
-    if (S) OS_HIP_CHECK(hipMemcpyAsync(b.tscale.ptr, p->tscale, sizeof(float) * S,
-                                       hipMemcpyHostToDevice, s));
-    if (b.g_tscale != b.tscale.ptr) b.drop_graph();
+    os::put_tscale(b, p->tscale, S);
     os::CsrRun run{};
     run.num_items = b.num_items;
     run.num_sweeps = S;
@@ -430,130 +607,127 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
                                   hipMemcpyHostToDevice, s));
       OS_HIP_CHECK(hipStreamSynchronize(s));
     }
-    b.mode = p->site_order;
-    if (b.mode == 0 && b.waves == 0) {
-      if (p->tune) {
-        os::tune(b, run);
-      } else {
-        b.reg = os::csr_pf_eligible(b.max_width);
-        b.waves = 1;
-        b.groups = std::max(1, std::min(c.num_cus * os::csr_max_groups_per_cu(b.reg, 1, b.max_words),
-                                        b.num_items));
-      }
-    }
-    // OS_CSR_COLOUR_SITES (4, 8 or 16) fixes the colour kernel's sites per step
-    if (b.mode == 1) {
-      const char *senv = std::getenv("OS_CSR_COLOUR_SITES");
-      const int forced = senv ? std::atoi(senv) : 0;
-      OS_REQUIRE(!senv || forced == 4 || forced == 8 || forced == 16,
-                 "OS_CSR_COLOUR_SITES must be 4, 8 or 16");
-      if (forced && forced != b.c_sites) {
-        b.c_sites = forced;
-        b.c_waves = 1;
-        os::col_items(b, b.c_sites);
-        b.c_groups = std::max(
-            1, std::min(c.num_cus * os::csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
-                        b.num_items_c));
-      }
-    }
-    if (b.mode == 1 && b.c_waves == 0) {
-      if (p->tune) {
-        os::tune_col(b, run);
-      } else {
-        b.c_sites = 8;
-        b.c_waves = 1;
-        os::col_items(b, b.c_sites);
-        b.c_groups = std::max(
-            1, std::min(c.num_cus * os::csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
-                        b.num_items_c));
-      }
-    }
-    if (b.mode == 1) os::col_items(b, b.c_sites);
-    const bool rg = b.mode == 0 && b.reg;
-    const int wv = b.mode == 1 ? b.c_waves : b.waves;
-    const int gr = b.mode == 1 ? b.c_groups : b.groups;
-    const int st_sites = b.mode == 1 ? b.c_sites : 0;
+    const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
     os::put_run(b, run);
-    if (p->use_graph && (!b.gexec || b.g_mode != b.mode || b.g_reg != rg || b.g_waves != wv ||
-                         b.g_groups != gr || b.g_sites != st_sites || b.g_Rp != Rp)) {
-      b.drop_graph();
-      OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      try {
-        b.enqueue(rg, wv, gr, true, st_sites);
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(s, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      OS_HIP_CHECK(hipStreamEndCapture(s, &b.graph));
-      OS_HIP_CHECK(hipGraphInstantiate(&b.gexec, b.graph, nullptr, nullptr, 0));
-      b.g_mode = b.mode;
-      b.g_reg = rg;
-      b.g_waves = wv;
-      b.g_groups = gr;
-      b.g_sites = st_sites;
-      b.g_Rp = Rp;
-      b.g_tscale = b.tscale.ptr;
-    }
+    if (p->use_graph) os::ensure_graph(b, sh);
     OS_HIP_CHECK(hipEventRecord(c.ev_start, s));
     if (p->use_graph)
       OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
     else
-      b.enqueue(rg, wv, gr, true, st_sites);
+      b.enqueue(sh.rg, sh.waves, sh.groups, true, sh.sites);
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
-    // results
-    std::vector<double> e64((size_t)b.I * Rp);
-    OS_HIP_CHECK(hipMemcpyAsync(e64.data(), b.e64.ptr, sizeof(double) * e64.size(),
-                                hipMemcpyDeviceToHost, s));
-    std::vector<double> ebh;
-    if (run_best_energies) {
-      ebh.resize((size_t)b.I * Rp);
-      OS_HIP_CHECK(hipMemcpyAsync(ebh.data(), b.eb.ptr, sizeof(double) * ebh.size(),
-                                  hipMemcpyDeviceToHost, s));
-    }
-    std::vector<uint32_t> bp, fp;
-    if (best_states) {
-      bp.resize((size_t)b.total_words * Rp);
-      OS_HIP_CHECK(hipMemcpyAsync(bp.data(), b.best.ptr, sizeof(uint32_t) * bp.size(),
-                                  hipMemcpyDeviceToHost, s));
-    }
-    if (final_states) {
-      fp.resize((size_t)b.total_words * Rp);
-      OS_HIP_CHECK(hipMemcpyAsync(fp.data(), b.fin.ptr, sizeof(uint32_t) * fp.size(),
-                                  hipMemcpyDeviceToHost, s));
-    }
-    OS_HIP_CHECK(hipStreamSynchronize(s));
+    os::collect(b, R, b.best, b.fin, run_best_energies ? &b.eb : nullptr, best_energies,
+                run_best_energies, best_states, final_states, inst_best_energy,
+                inst_best_replica);
     float ms = 0.f;
     OS_HIP_CHECK(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
-    uint8_t *bs = best_states, *fs = final_states;
-    for (int i = 0; i < b.I; ++i) {
-      const auto &d = b.inst[i];
-      int arg = 0;
-      for (int r = 0; r < R; ++r) {
-        const double e = e64[(size_t)d.rep_off + r];
-        if (best_energies) best_energies[(size_t)i * R + r] = e;
-        if (run_best_energies) run_best_energies[(size_t)i * R + r] = ebh[(size_t)d.rep_off + r];
-        if (e < e64[(size_t)d.rep_off + arg]) arg = r;
-      }
-      if (inst_best_energy) inst_best_energy[i] = e64[(size_t)d.rep_off + arg];
-      if (inst_best_replica) inst_best_replica[i] = arg;
-      if (bs || fs)
-        for (int r = 0; r < R; ++r)
-          for (int j = 0; j < d.n; ++j) {
-            const size_t at = (size_t)d.state_off + (size_t)(j >> 5) * Rp + r;
-            if (bs) bs[(size_t)r * d.n + j] = (uint8_t)((bp[at] >> (j & 31)) & 1u);
-            if (fs) fs[(size_t)r * d.n + j] = (uint8_t)((fp[at] >> (j & 31)) & 1u);
-          }
-      if (bs) bs += (size_t)R * d.n;
-      if (fs) fs += (size_t)R * d.n;
-    }
     if (stats) {
       stats->kernel_ms = ms;
       stats->total_ms = ms;
       stats->kernel_launches = 1;
       stats->max_field_drift = 0.0;
       stats->attempts = (uint64_t)b.total_spins * (uint64_t)R * (uint64_t)S;
+      stats->pass_avg_ms = 0.0;
+      stats->pass_launches = 0;
+    }
+  });
+}
+
+int os_csr_batch_population(os_csr_batch *b_, const os_csr_population_params *p,
+                            double *best_energies, uint8_t *best_states, uint8_t *final_states,
+                            uint32_t *ancestors, double *inst_best_energy,
+                            int32_t *inst_best_replica, os_sweep_stats *stats) {
+  return os::guarded([&] {
+    OS_REQUIRE(b_ && p, "null argument");
+    OS_REQUIRE(p->num_replicas >= 1, "num_replicas must be positive");
+    OS_REQUIRE(p->num_steps >= 1 && p->sweeps_per_step >= 1 && p->tscale,
+               "invalid population schedule");
+    OS_REQUIRE((int64_t)p->num_steps * p->sweeps_per_step <= (int64_t)INT32_MAX,
+               "population schedule too long");
+    OS_REQUIRE(p->site_order == 0 || p->site_order == 1, "site_order must be 0 or 1");
+    for (int k = 0; k < p->num_steps; ++k)
+      OS_REQUIRE(p->tscale[k] > 0.0f && std::isfinite(p->tscale[k]),
+                 "temperature scales must be positive");
+    CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+    OS_REQUIRE((uint64_t)p->instance_base + (uint64_t)b.I <= (1u << 24),
+               "instance ids must stay below 2^24");
+    Ctx &c = *b.ctx;
+    c.activate();
+    hipStream_t s = c.stream;
+    const int R = (int)p->num_replicas;
+    const int Rp = (R + 63) / 64 * 64;
+    const int K = p->num_steps, theta = p->sweeps_per_step;
+    os::shape_for(b, Rp);
+    // the per-sweep scales: tscale[k] for the sweeps of step k
+    std::vector<float> ts((size_t)K * theta);
+    for (int k = 0; k < K; ++k)
+      std::fill(ts.begin() + (size_t)k * theta, ts.begin() + (size_t)(k + 1) * theta,
+                p->tscale[k]);
+    os::put_tscale(b, ts.data(), K * theta);
+    os::CsrRun run{};
+    run.num_items = b.num_items;
+    run.num_sweeps = theta;
+    run.Rp = Rp;
+    run.k0 = (uint32_t)p->seed;
+    run.k1 = (uint32_t)(p->seed >> 32);
+    run.instance_base = p->instance_base;
+    const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
+    if (p->use_graph) os::ensure_graph(b, sh);
+    const size_t planes = sizeof(uint32_t) * (size_t)std::max<int64_t>(b.total_words * Rp, 1);
+    b.pop_best.ensure(planes);
+    b.pop_eb.ensure(sizeof(double) * (size_t)b.I * Rp);
+    b.pop_anc.ensure(sizeof(uint32_t) * (size_t)b.I * R);
+    std::vector<double> ef((size_t)b.I * Rp);
+    std::vector<uint32_t> anc((size_t)b.I * R);
+    std::vector<double> wsum(R);
+    OS_HIP_CHECK(hipEventRecord(c.ev_start, s));
+    for (int k = 0; k < K; ++k) {
+      run.sweep_base = k * theta;
+      run.init_mode = k > 0 ? 1 : 0;
+      os::put_run(b, run);
+      if (p->use_graph)
+        OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
+      else
+        b.enqueue(sh.rg, sh.waves, sh.groups, false, sh.sites);
+      // best of every replica index: the step's best where it is lower
+      os::launch_csr_pop_best(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
+                              k == 0, b.eb.as<double>(), b.best.as<uint32_t>(),
+                              b.pop_eb.as<double>(), b.pop_best.as<uint32_t>());
+      if (k + 1 >= K) break;
+      // resampling round k on the running energies at the step's end
+      OS_HIP_CHECK(hipMemcpyAsync(ef.data(), b.ef.ptr, sizeof(double) * ef.size(),
+                                  hipMemcpyDeviceToHost, s));
+      OS_HIP_CHECK(hipStreamSynchronize(s));
+      const double dbeta = 1.0 / (double)p->tscale[k + 1] - 1.0 / (double)p->tscale[k];
+      for (int i = 0; i < b.I; ++i) {
+        const uint32_t tag = ((p->instance_base + (uint32_t)i) << 8) | os::TAG_RESAMPLE;
+        const double u = os::resample_uniform(run.k0, run.k1, tag, (uint32_t)k);
+        uint32_t *a = anc.data() + (size_t)i * R;
+        os::resample(ef.data() + b.inst[i].rep_off, R, dbeta, u, a, wsum.data());
+        if (ancestors)
+          std::memcpy(ancestors + ((size_t)i * (K - 1) + k) * R, a, sizeof(uint32_t) * R);
+      }
+      OS_HIP_CHECK(hipMemcpyAsync(b.pop_anc.ptr, anc.data(), sizeof(uint32_t) * anc.size(),
+                                  hipMemcpyHostToDevice, s));
+      os::launch_csr_pop_gather(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items,
+                                Rp, R, b.pop_anc.as<uint32_t>(), b.fin.as<uint32_t>(),
+                                b.init.as<uint32_t>());
+    }
+    OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
+    // FP64 energies of the best states of every replica index
+    os::launch_csr_energy(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
+                          b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
+                          b.diag64.as<double>(), b.pop_best.as<uint32_t>(), b.e64.as<double>());
+    os::collect(b, R, b.pop_best, b.fin, nullptr, best_energies, nullptr, best_states,
+                final_states, inst_best_energy, inst_best_replica);
+    float ms = 0.f;
+    OS_HIP_CHECK(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+    if (stats) {
+      stats->kernel_ms = ms;
+      stats->total_ms = ms;
+      stats->kernel_launches = K;
+      stats->max_field_drift = 0.0;
+      stats->attempts = (uint64_t)b.total_spins * (uint64_t)R * (uint64_t)K * (uint64_t)theta;
       stats->pass_avg_ms = 0.0;
       stats->pass_launches = 0;
     }

@@ -165,7 +165,18 @@ struct Exchange {
   uint32_t *final_slots = nullptr;  // optional [I][R] slot of each replica at the end
 };
 
-int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double *best_energies,
+// Population annealing: segments of `theta` sweeps; after each but the last,
+// systematic resampling with dbeta = 1/ts[k+1] - 1/ts[k] on the running
+// energies; rows take the state words, fields and energy of their ancestor.
+struct Population {
+  int K = 0;
+  int theta = 0;
+  const float *tscale = nullptr;  // [K]
+  uint32_t *ancestors = nullptr;  // optional [I][K-1][R]
+};
+
+int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const Population *pa,
+              double *best_energies,
               uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
               double *best_energy, os_sweep_stats *stats) {
   return dguard([&] {
@@ -186,7 +197,7 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
     const size_t nn = (size_t)n * n;
     const int S = p->num_sweeps;
     const int refresh = p->refresh_interval > 0 ? p->refresh_interval : std::max(S, 1);
-    const int seg = xc ? xc->interval : refresh;
+    const int seg = xc ? xc->interval : pa ? pa->theta : refresh;
     if (xc) {
       OS_REQUIRE(!d.stream, "replica exchange supports n <= 1024");
       OS_REQUIRE(xc->K >= 2 && xc->tscale, "replica exchange needs >= 2 temperature slots");
@@ -196,6 +207,14 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
                  "refresh_interval must be a multiple of exchange_interval");
       for (int k = 0; k < xc->K; ++k)
         OS_REQUIRE(xc->tscale[k] > 0.0f && std::isfinite(xc->tscale[k]),
+                   "temperature scales must be positive");
+    }
+    if (pa) {
+      OS_REQUIRE(pa->K >= 1 && pa->theta >= 1 && pa->tscale, "invalid population schedule");
+      OS_REQUIRE(p->refresh_interval <= 0 || p->refresh_interval % pa->theta == 0,
+                 "refresh_interval must be a multiple of sweeps_per_step");
+      for (int k = 0; k < pa->K; ++k)
+        OS_REQUIRE(pa->tscale[k] > 0.0f && std::isfinite(pa->tscale[k]),
                    "temperature scales must be positive");
     }
     // replica exchange: the slot of every replica and the replica at every
@@ -257,7 +276,15 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
       OS_HIP_CHECK(hipMemcpyAsync(d.tscale.ptr, p->tscale, sizeof(float) * S,
                                   hipMemcpyHostToDevice, st));
     if (xc) upload_tsr();
-    std::vector<double> e_now(xc ? T : 0);
+    std::vector<double> e_now(xc || pa ? T : 0);
+    std::vector<uint32_t> anc_h(pa ? (size_t)I * R : 0);
+    std::vector<double> wsum(pa ? R : 0);
+    if (pa) {
+      d.pa_x.ensure(sizeof(uint32_t) * T * npw);
+      d.pa_f.ensure(sizeof(float) * T * np);
+      d.pa_e.ensure(sizeof(double) * T);
+      d.pa_anc.ensure(sizeof(uint32_t) * I * R);
+    }
     int round = 0;
     // initial states
     if (p->init_states) {
@@ -294,6 +321,10 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
     const char *genv = std::getenv("OS_STREAM_GROUP");
     d.group = genv ? std::min(16, std::max(1, std::atoi(genv))) : os::Dense::kGroup;
     const int m = d.group, ring = 3 * d.group;
+    // decide launches: operand chunking (OS_STREAM_DECIDE_VARIANT overrides
+    // the default, for tuning)
+    const char *venv = std::getenv("OS_STREAM_DECIDE_VARIANT");
+    const int dvar = venv ? std::atoi(venv) : os::Dense::kDecideVariant;
     // pass launches: blocks whose operands load in one round trip
     // (OS_STREAM_PASS_CHUNK overrides the default, for tuning)
     const char *cenv = std::getenv("OS_STREAM_PASS_CHUNK");
@@ -357,7 +388,7 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
                                    p->instance, blk == nb - 1, d.qpack.as<uint4>(),
                                    d.qdb.as<float>(), fbuf(cur + std::max(0, H - 1)),
                                    d.xbits.as<uint32_t>(), ring_d, d.energy.as<double>(),
-                                   d.ebest.as<double>(), d.bestbits.as<uint32_t>());
+                                   d.ebest.as<double>(), d.bestbits.as<uint32_t>(), dvar);
           ++launches;
           if (g % m == m - 1 || g == steps - 1) {
             OS_HIP_CHECK(hipEventRecord(d.ev_dec, d.side));
@@ -386,8 +417,38 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, double
       }
       OS_HIP_CHECK(hipEventRecord(b, st));
       // FP64 refresh of fields and running energies; records the fp32 drift
-      if (!xc || (s0 + ns) % refresh == 0 || s0 + ns >= S)
+      if (!(xc || pa) || (s0 + ns) % refresh == 0 || s0 + ns >= S)
         energies(d.xbits.as<uint32_t>(), d.energy.as<double>(), fbuf(cur));
+      if (pa && s0 + ns < S) {
+        // resampling round k on the running energies, then every row moves
+        // to its ancestor's state, fields and energy
+        const int k = s0 / pa->theta;
+        OS_HIP_CHECK(hipMemcpyAsync(e_now.data(), d.energy.ptr, sizeof(double) * T,
+                                    hipMemcpyDeviceToHost, st));
+        c.sync();
+        const double dbeta = 1.0 / (double)pa->tscale[k + 1] - 1.0 / (double)pa->tscale[k];
+        const uint32_t k0 = (uint32_t)p->seed, k1 = (uint32_t)(p->seed >> 32);
+        for (int s = 0; s < I; ++s) {
+          const uint32_t tag = ((p->instance + (uint32_t)s) << 8) | os::TAG_RESAMPLE;
+          const double u = os::resample_uniform(k0, k1, tag, (uint32_t)k);
+          uint32_t *anc = anc_h.data() + (size_t)s * R;
+          os::resample(e_now.data() + rows(s), R, dbeta, u, anc, wsum.data());
+          if (pa->ancestors)
+            std::memcpy(pa->ancestors + ((size_t)s * (pa->K - 1) + k) * R, anc,
+                        sizeof(uint32_t) * R);
+        }
+        OS_HIP_CHECK(hipMemcpyAsync(d.pa_anc.ptr, anc_h.data(), sizeof(uint32_t) * I * R,
+                                    hipMemcpyHostToDevice, st));
+        os::launch_dense_gather(st, I, Rp, R, npw, np, d.pa_anc.as<uint32_t>(),
+                                d.xbits.as<uint32_t>(), fbuf(cur), d.energy.as<double>(),
+                                d.pa_x.as<uint32_t>(), d.pa_f.as<float>(), d.pa_e.as<double>());
+        OS_HIP_CHECK(hipMemcpyAsync(d.xbits.ptr, d.pa_x.ptr, sizeof(uint32_t) * T * npw,
+                                    hipMemcpyDeviceToDevice, st));
+        OS_HIP_CHECK(hipMemcpyAsync(fbuf(cur), d.pa_f.ptr, sizeof(float) * T * np,
+                                    hipMemcpyDeviceToDevice, st));
+        OS_HIP_CHECK(hipMemcpyAsync(d.energy.ptr, d.pa_e.ptr, sizeof(double) * T,
+                                    hipMemcpyDeviceToDevice, st));
+      }
       if (xc && s0 + ns < S) {
         // exchange round on the running energies of the launch (or of the
         // refresh), then the scales of the next segment
@@ -498,7 +559,7 @@ extern "C" {
 int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energies,
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats) {
-  return dense_run(d_, p, nullptr, best_energies, best_states, final_states, best_state,
+  return dense_run(d_, p, nullptr, nullptr, best_energies, best_states, final_states, best_state,
                    best_energy, stats);
 }
 
@@ -521,8 +582,41 @@ int os_dense_tempering(os_dense *d_, const os_tempering_params *t, double *best_
   xc.tscale = t->tscale;
   xc.interval = t->exchange_interval;
   xc.final_slots = final_slots;
-  return dense_run(d_, &p, &xc, best_energies, best_states, final_states, best_state, best_energy,
-                   stats);
+  return dense_run(d_, &p, &xc, nullptr, best_energies, best_states, final_states, best_state,
+                   best_energy, stats);
+}
+
+int os_dense_population(os_dense *d_, const os_population_params *q, double *best_energies,
+                        uint8_t *best_states, uint8_t *final_states, uint32_t *ancestors,
+                        uint8_t *best_state, double *best_energy, os_sweep_stats *stats) {
+  if (!q) return dguard([&] { OS_REQUIRE(false, "null argument"); });
+  if (q->num_steps < 1 || q->sweeps_per_step < 1 || !q->tscale ||
+      (int64_t)q->num_steps * q->sweeps_per_step > (int64_t)INT32_MAX)
+    return dguard([&] { OS_REQUIRE(false, "invalid population schedule"); });
+  // the per-sweep scales: tscale[k] for the sweeps of step k
+  std::vector<float> ts;
+  try {
+    ts.resize((size_t)q->num_steps * q->sweeps_per_step);
+  } catch (...) {
+    return dguard([&] { OS_REQUIRE(false, "population schedule too long"); });
+  }
+  for (int k = 0; k < q->num_steps; ++k)
+    std::fill(ts.begin() + (size_t)k * q->sweeps_per_step,
+              ts.begin() + (size_t)(k + 1) * q->sweeps_per_step, q->tscale[k]);
+  os_sweep_params p{};
+  p.num_replicas = q->num_replicas;
+  p.num_sweeps = q->num_steps * q->sweeps_per_step;
+  p.tscale = ts.data();
+  p.seed = q->seed;
+  p.instance = q->instance;
+  p.refresh_interval = q->refresh_interval;
+  Population pa;
+  pa.K = q->num_steps;
+  pa.theta = q->sweeps_per_step;
+  pa.tscale = q->tscale;
+  pa.ancestors = ancestors;
+  return dense_run(d_, &p, nullptr, &pa, best_energies, best_states, final_states, best_state,
+                   best_energy, stats);
 }
 
 }  // extern "C"

@@ -16,11 +16,13 @@
 //   warm-start draw of spin j, replica r   : (j>>2, r, 0, TAG_WARM)      word j&3
 //   replica exchange, slots (k, k+1) of ladder l in round t:
 //                                            (k, l, t, TAG_EXCHANGE)     word 0
+//   population annealing, resampling round k: (k, 0, 0, TAG_RESAMPLE)   words 0, 1
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <stdexcept>
 #include <string>
 
@@ -38,6 +40,7 @@ enum : uint32_t {
   TAG_SWEEP_THR = 0x11,
   TAG_WARM = 0x12,
   TAG_EXCHANGE = 0x13,
+  TAG_RESAMPLE = 0x14,
 };
 
 __host__ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
@@ -84,6 +87,32 @@ __host__ __device__ __forceinline__ float neg_ln_u(uint32_t w) {
   float l = fmaf(f2, p, f);
   l = fmaf((float)e, 0x1.62e430p-1f, l);
   return -l;
+}
+
+// Population annealing, host side.  The uniform of resampling round k:
+// words 0 and 1 of Philox counter (k, 0, 0, tag), 53 bits.
+inline double resample_uniform(uint32_t k0, uint32_t k1, uint32_t tag, uint32_t k) {
+  const uint4 o = philox4x32_10(make_uint4(k, 0u, 0u, tag), k0, k1);
+  return (double)(((uint64_t)(o.x >> 5) << 26) | (uint64_t)(o.y >> 6)) * 0x1p-53;
+}
+
+// Systematic resampling of R replicas with energies E from scale 1/beta to
+// 1/(beta + dbeta): anc[j] = first i with W_i > (j + u) / R * W_{R-1}, W the
+// running sums of exp(-dbeta (E_i - min E)).  W: R doubles of scratch.
+inline void resample(const double *E, int R, double dbeta, double u, uint32_t *anc, double *W) {
+  double emin = E[0];
+  for (int i = 1; i < R; ++i) emin = E[i] < emin ? E[i] : emin;
+  double acc = 0.0;
+  for (int i = 0; i < R; ++i) {
+    acc += std::exp(-dbeta * (E[i] - emin));
+    W[i] = acc;
+  }
+  int i = 0;
+  for (int j = 0; j < R; ++j) {
+    const double target = ((double)j + u) / (double)R * W[R - 1];
+    while (i + 1 < R && W[i] <= target) ++i;
+    anc[j] = (uint32_t)i;
+  }
 }
 
 // Library error with a return code; caught at the C-ABI boundary.

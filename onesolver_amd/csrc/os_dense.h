@@ -21,6 +21,8 @@ struct Dense {
   DevBuf q64, qpack, qdb;
   DevBuf fields, xbits, bestbits, energy, ebest, tscale, scratch, drift, bytes, marg, status;
   DevBuf tsr;  // replica exchange: per-row temperature scales
+  // population annealing: the gathered rows of states, fields and energies
+  DevBuf pa_x, pa_f, pa_e, pa_anc;
   // stream path: flip masks of the last 3 * group blocks (ring), and the
   // decide launches on `side`, beside the field passes on the context's
   // stream; ev_dec marks the last decide of a group, ev_pass[G % 2] pass G
@@ -28,13 +30,15 @@ struct Dense {
   static constexpr int kGroup = 5;      // default blocks per field pass
   static constexpr int kPassChunk = 2;  // default operand blocks per round trip (pass)
   static constexpr int kPassWgsPerCu = 1;  // default persistent pass work-groups per CU (room for decides)
+  static constexpr int kDecideVariant = 20;  // default decide operand chunking (launch_stream_decide)
   int group = kGroup;
   hipStream_t side = nullptr;
   hipEvent_t ev_seg = nullptr, ev_dec = nullptr, ev_pass[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> pass_ev;  // timing pairs around sampled passes
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
-                    &scratch, &drift, &bytes, &marg, &status, &dblk, &tsr})
+                    &scratch, &drift, &bytes, &marg, &status, &dblk, &tsr, &pa_x, &pa_f,
+                    &pa_e, &pa_anc})
       b->release();
     for (hipEvent_t e : {ev_seg, ev_dec, ev_pass[0], ev_pass[1]})
       if (e) (void)hipEventDestroy(e);

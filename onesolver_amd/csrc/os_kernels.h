@@ -56,6 +56,12 @@ void launch_dense_sweeps(hipStream_t st, int n, int np, int num_instances, const
                          int nsweeps, uint64_t seed, uint32_t instance, int R_pad, float *fields,
                          uint32_t *xbits, uint32_t *bestbits, double *energy, double *ebest,
                          int *status);
+// population annealing: rows r < R of every instance (Rp rows each) take
+// the state words, fields and energy of row anc[instance][r]; rows R..Rp-1
+// keep theirs (out-of-place: x2, f2, e2)
+void launch_dense_gather(hipStream_t st, int num_instances, int Rp, int R, int npw, int np,
+                         const uint32_t *anc, const uint32_t *x, const float *f, const double *e,
+                         uint32_t *x2, float *f2, double *e2);
 // start states of num_instances instances (R rows each, stream ids instance + i;
 // warm start: marginals p[i][n])
 void launch_sweep_init(hipStream_t st, int n, int npw, int R, uint64_t seed, uint32_t instance,
@@ -78,11 +84,13 @@ void launch_stream_pass(hipStream_t st, int I, int np, int Rp, int g0, int cnt, 
                         const uint4 *dring);
 // decide launch: block g, from the fields Fin (after blocks .. gb-1) plus the
 // flips of blocks gb .. g-1
+// variant: 10 * (pending blocks loaded per round trip) + (1 if the next
+// chunk's loads are issued before the current chunk is applied); 20 default
 void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, int gb, int ring,
                           int sweep, float ts, uint64_t seed, uint32_t inst_base,
                           bool last_of_sweep, const uint4 *qpack, const float *qdb,
                           const float *Fin, uint32_t *xbits, uint4 *dring, double *energy,
-                          double *ebest, uint32_t *bestbits);
+                          double *ebest, uint32_t *bestbits, int variant);
 
 // gen.hip
 // q[s] (FP64 n x n, s < I) = SK instance inst_base + s of key seed
@@ -118,7 +126,7 @@ struct CsrRun {
   int32_t init_mode;  // 0: Philox start states, 1: caller's planes
   uint32_t k0, k1;    // Philox key
   uint32_t instance_base;
-  int32_t pad_;
+  int32_t sweep_base;  // number of the launch's first sweep (thresholds, tscale index)
 };
 int csr_lds_bytes(int waves, int max_words);
 // pf: the row-prefetching kernel (batches within csr_pf_eligible)
@@ -133,17 +141,29 @@ void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_w
 // colour-class sweeps: G (4, 8 or 16) sites of 64/G replicas per wave step;
 // items are (instance, group of 64/G replicas); corder lists the sites of
 // each instance class by class (offset spin_off), cstart the class starts
-// (offset class_off)
+// (offset class_off); pell_col / pell_val / pdiag hold the ELL rows and the
+// diagonal in that visiting order (offsets ell_off, spin_off)
 int csr_col_lds_bytes(int waves, int sites, int max_words);
 int csr_col_max_groups_per_cu(int waves, int sites, int max_words);
 void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
                            const CsrInst *inst, const int2 *items, const CsrRun *run,
                            const int4 *ell_col, const float4 *ell_val, const float *diag,
-                           const int32_t *corder, const int32_t *cstart, const float *tscale,
+                           const int32_t *corder, const int32_t *cstart, const int4 *pell_col,
+                           const float4 *pell_val, const float *pdiag, const float *tscale,
                            const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
                            double *e_best, double *e_final, int *queue);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
                        const double *diag64, const uint32_t *bits, double *out);
+// population annealing over the 64-replica items: replica r < R of every
+// instance keeps the step's best planes and energy (eb, best) where they are
+// below its best so far (or all of them when `first`); the gather gives
+// replica j < R the final planes of replica anc[instance][j]
+void launch_csr_pop_best(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                         int Rp, bool first, const double *eb, const uint32_t *best,
+                         double *pop_eb, uint32_t *pop_best);
+void launch_csr_pop_gather(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
+                           int Rp, int R, const uint32_t *anc, const uint32_t *fin,
+                           uint32_t *init);
 
 }  // namespace os

@@ -60,6 +60,17 @@ def lib():
                                          ctypes.c_uint32, _i32p, _u8p, _u8p, _u8p, _f64p,
                                          ctypes.c_int]
         L.or_csr_colour_order.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p]
+        L.or_resample.argtypes = [_f64p, ctypes.c_uint32, ctypes.c_double, ctypes.c_double,
+                                  _u32p]
+        L.or_resample_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32]
+        L.or_resample_uniform.restype = ctypes.c_double
+        L.or_population_dense.argtypes = [ctypes.c_int, _f64p, _f32p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, _u8p,
+                                          _u8p, _f64p, _u32p, ctypes.c_int]
+        L.or_population_csr.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p, _f32p, _f32p,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                        ctypes.c_uint64, ctypes.c_uint32, _i32p, _u8p, _u8p,
+                                        _f64p, _u32p, ctypes.c_int]
         L.or_warm_init.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, ctypes.c_uint64,
                                    ctypes.c_uint32, _u8p]
         L.or_sweep_init.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
@@ -218,6 +229,33 @@ def csr_colour_order(row_ptr, col):
     return order[:n], cs[:k + 1]
 
 
+def resample(E, dbeta, u):
+    """Systematic resampling (or_resample): ancestor of every new replica."""
+    e = np.ascontiguousarray(E, dtype=np.float64)
+    anc = np.zeros(len(e), dtype=np.uint32)
+    lib().or_resample(_p(e, _f64p), len(e), dbeta, u, _p(anc, _u32p))
+    return anc
+
+
+def resample_uniform(seed, instance, k):
+    return float(lib().or_resample_uniform(seed, instance, k))
+
+
+def population_dense(flat, n, tscale, theta, num_replicas, seed=1, instance=0, threads=0):
+    """Population annealing (or_population_dense): (final, best states, best
+    energies, ancestors [(K-1), R])."""
+    flat = np.ascontiguousarray(flat, dtype=np.float64).ravel()
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    K, R = len(ts), int(num_replicas)
+    fin = np.zeros((R, n), dtype=np.uint8)
+    bs = np.zeros((R, n), dtype=np.uint8)
+    be = np.zeros(R, dtype=np.float64)
+    anc = np.zeros((max(K - 1, 1), R), dtype=np.uint32)
+    lib().or_population_dense(n, _p(flat, _f64p), _p(ts, _f32p), K, theta, R, seed, instance,
+                              _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), _p(anc, _u32p), threads)
+    return fin, bs, be, anc[:K - 1]
+
+
 def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0, init=None,
               threads=0, order=None):
     """Sparse sweep-mode chains; `order`: the sites' visiting order per sweep
@@ -237,6 +275,27 @@ def sweep_csr(row_ptr, col, val, diag, tscale, num_replicas, seed=1, instance=0,
                              _p(ts, _f32p), len(ts), num_replicas, seed, instance, _p(od, _i32p),
                              _p(ini, _u8p), _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), threads)
     return fin, bs, be
+
+
+def population_csr(row_ptr, col, val, diag, tscale, theta, num_replicas, seed=1, instance=0,
+                   order=None, threads=0):
+    """Sparse population annealing (or_population_csr): (final, best states,
+    best energies, ancestors [(K-1), R])."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    cl = np.ascontiguousarray(col, dtype=np.int32)
+    vl = np.ascontiguousarray(val, dtype=np.float32)
+    dg = np.ascontiguousarray(diag, dtype=np.float32)
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    n, K, R = len(dg), len(ts), int(num_replicas)
+    fin = np.zeros((R, n), dtype=np.uint8)
+    bs = np.zeros((R, n), dtype=np.uint8)
+    be = np.zeros(R, dtype=np.float64)
+    anc = np.zeros((max(K - 1, 1), R), dtype=np.uint32)
+    od = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+    lib().or_population_csr(n, _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f32p), _p(dg, _f32p),
+                            _p(ts, _f32p), K, theta, R, seed, instance, _p(od, _i32p),
+                            _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), _p(anc, _u32p), threads)
+    return fin, bs, be, anc[:K - 1]
 
 
 def warm_init(p, num_replicas, seed=1, instance=0):

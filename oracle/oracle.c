@@ -66,6 +66,7 @@ enum {
   TAG_SWEEP_THR = 0x11,
   TAG_WARM = 0x12,
   TAG_EXCHANGE = 0x13,
+  TAG_RESAMPLE = 0x14,
 };
 
 static void seed_key(uint64_t seed, uint32_t key[2]) {
@@ -626,6 +627,120 @@ void or_sweep_dense(int n, const double *flat, const float *tscale, int num_swee
   }
 }
 
+/* Systematic resampling of one population of R replicas: weights
+ * w_i = exp(-dbeta (E_i - E_min)), cumulative sums W_i in replica order;
+ * new replica j descends from the smallest i with W_i > (j + u) / R * W_{R-1}
+ * (u in [0, 1)).  Shared by the oracle's population annealer and its tests. */
+void or_resample(const double *E, uint32_t R, double dbeta, double u, uint32_t *anc) {
+  double emin = E[0];
+  for (uint32_t i = 1; i < R; ++i) emin = E[i] < emin ? E[i] : emin;
+  double *W = (double *)malloc(sizeof(double) * R);
+  double acc = 0.0;
+  for (uint32_t i = 0; i < R; ++i) {
+    acc += exp(-dbeta * (E[i] - emin));
+    W[i] = acc;
+  }
+  uint32_t i = 0;
+  for (uint32_t j = 0; j < R; ++j) {
+    const double target = ((double)j + u) / (double)R * W[R - 1];
+    while (i + 1 < R && W[i] <= target) ++i;
+    anc[j] = i;
+  }
+  free(W);
+}
+
+/* The uniform of resampling round k: words 0 and 1 of Philox counter
+ * (k, 0, 0, (instance << 8) | TAG_RESAMPLE), 53 bits.                       */
+double or_resample_uniform(uint64_t seed, uint32_t instance, uint32_t k) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  uint32_t ctr[4] = {k, 0u, 0u, (instance << 8) | TAG_RESAMPLE};
+  uint32_t o[4];
+  or_philox4x32_10(ctr, key, o);
+  return (double)(((uint64_t)(o[0] >> 5) << 26) | (uint64_t)(o[1] >> 6)) * 0x1p-53;
+}
+
+/* Population annealing over sweep-mode replicas (os_dense_population, no
+ * reference counterpart; the reference recommends it,
+ * benchmarks/annealing/performance.md:52-59).  R replicas start from the
+ * sweep-mode random bits.  Step k = 0..K-1 runs theta sweeps at threshold
+ * scale tscale[k] (sweep counters k*theta ..); after every step but the last
+ * the population is resampled (or_resample) with dbeta = 1/tscale[k+1] -
+ * 1/tscale[k], the running energies and u = or_resample_uniform(k): replica
+ * j then continues from the state, fields and running energy of replica
+ * anc[j].  The best state of each replica index is tracked at sweep ends and
+ * never moved.  ancestors: optional [R x (K-1)] ancestor lists per round.   */
+void or_population_dense(int n, const double *flat, const float *tscale, int K, int theta,
+                         uint32_t R, uint64_t seed, uint32_t instance, uint8_t *final_states,
+                         uint8_t *best_states, double *best_energies, uint32_t *ancestors,
+                         int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  uint8_t *xs = (uint8_t *)malloc((size_t)R * n);
+  double *hs = (double *)malloc(sizeof(double) * (size_t)R * n);
+  double *es = (double *)malloc(sizeof(double) * R);
+  uint8_t *xs2 = (uint8_t *)malloc((size_t)R * n);
+  double *hs2 = (double *)malloc(sizeof(double) * (size_t)R * n);
+  double *es2 = (double *)malloc(sizeof(double) * R);
+  uint32_t *anc = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  for (uint32_t r = 0; r < R; ++r) {
+    uint8_t *x = xs + (size_t)r * n;
+    double *h = hs + (size_t)r * n;
+    sweep_init_bits(n, r, instance, key, x);
+    for (int j = 0; j < n; ++j) {
+      double acc = flat[(size_t)j * n + j];
+      for (int l = 0; l < n; ++l)
+        if (l != j && x[l]) acc += flat[(size_t)j * n + l];
+      h[j] = acc;
+    }
+    es[r] = or_energy(flat, x, n);
+    best_energies[r] = es[r];
+    memcpy(best_states + (size_t)r * n, x, (size_t)n);
+  }
+  for (int k = 0; k < K; ++k) {
+    const float ts = tscale[k];
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int64_t rr = 0; rr < (int64_t)R; ++rr) {
+      const uint32_t r = (uint32_t)rr;
+      uint8_t *x = xs + (size_t)r * n;
+      double *h = hs + (size_t)r * n;
+      uint32_t w[4];
+      for (int t = k * theta; t < (k + 1) * theta; ++t) {
+        for (int j = 0; j < n; ++j) {
+          if ((j & 3) == 0) sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
+          float thr = or_neg_ln_u(w[j & 3]) * ts;
+          double de = x[j] ? -h[j] : h[j];
+          if (de < (double)thr) {
+            double d = x[j] ? -1.0 : 1.0;
+            x[j] = (uint8_t)(1 - x[j]);
+            for (int l = 0; l < n; ++l)
+              if (l != j) h[l] += flat[(size_t)l * n + j] * d;
+            es[r] += de;
+          }
+        }
+        if (es[r] < best_energies[r]) {
+          best_energies[r] = es[r];
+          memcpy(best_states + (size_t)r * n, x, (size_t)n);
+        }
+      }
+    }
+    if (k + 1 >= K) break;
+    const double dbeta = 1.0 / (double)tscale[k + 1] - 1.0 / (double)tscale[k];
+    or_resample(es, R, dbeta, or_resample_uniform(seed, instance, (uint32_t)k), anc);
+    if (ancestors) memcpy(ancestors + (size_t)k * R, anc, sizeof(uint32_t) * R);
+    for (uint32_t j = 0; j < R; ++j) {
+      memcpy(xs2 + (size_t)j * n, xs + (size_t)anc[j] * n, (size_t)n);
+      memcpy(hs2 + (size_t)j * n, hs + (size_t)anc[j] * n, sizeof(double) * (size_t)n);
+      es2[j] = es[anc[j]];
+    }
+    uint8_t *tx = xs; xs = xs2; xs2 = tx;
+    double *th = hs; hs = hs2; hs2 = th;
+    double *te = es; es = es2; es2 = te;
+  }
+  if (final_states) memcpy(final_states, xs, (size_t)R * n);
+  free(xs); free(hs); free(es); free(xs2); free(hs2); free(es2); free(anc);
+}
+
 /* Replica exchange over sweep-mode replicas (os_dense_tempering, no reference
  * counterpart).  Replica r = l*K + k of ladder l starts in slot k; it sweeps
  * with the thresholds of or_sweep_dense at scale tscale[slot].  After every
@@ -778,8 +893,44 @@ int or_csr_colour_order(int n, const int32_t *row_ptr, const int32_t *col, int32
 }
 
 /* The sparse sweep with the sites of each sweep visited in order[0..n-1]
- * (NULL: 0..n-1).  Site j keeps its threshold (counter (j>>2, r, t)), so
- * only the order of the updates differs from or_sweep_csr.                 */
+ * (NULL: 0..n-1).  The threshold of the update at position pos of a sweep
+ * is word pos&3 of Philox counter (pos>>2, r, t): for the order 0..n-1 the
+ * stream of or_sweep_csr.                                                  */
+/* Sweeps t0 .. t0+num_sweeps-1 of replica r from state x (updated in place):
+ * the energy starts as csr_energy(x); best/e_best take the start state and
+ * every sweep end below the best so far.  Returns the running energy.      */
+static double csr_chain(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                        const float *diag, const float *tscale, int t0, int num_sweeps,
+                        uint32_t r, uint32_t instance, const uint32_t key[2],
+                        const int32_t *order, uint8_t *x, uint8_t *best, double *e_best) {
+  double e = csr_energy(n, row_ptr, col, val, diag, x);
+  double eb = e;
+  memcpy(best, x, (size_t)n);
+  uint32_t w[4];
+  for (int t = t0; t < t0 + num_sweeps; ++t) {
+    float ts = tscale[t];
+    for (int pos = 0; pos < n; ++pos) {
+      const int j = order ? order[pos] : pos;
+      if ((pos & 3) == 0) sweep_thr_words((uint32_t)(pos >> 2), r, (uint32_t)t, instance, key, w);
+      float thr = or_neg_ln_u(w[pos & 3]) * ts;
+      float h = diag[j];
+      for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
+        if (x[col[k]]) h = h + val[k];
+      float de = x[j] ? -h : h;
+      if (de < thr) {
+        x[j] = (uint8_t)(1 - x[j]);
+        e += (double)de;
+      }
+    }
+    if (e < eb) {
+      eb = e;
+      memcpy(best, x, (size_t)n);
+    }
+  }
+  *e_best = eb;
+  return e;
+}
+
 void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
                         const float *diag, const float *tscale, int num_sweeps,
                         uint32_t num_replicas, uint64_t seed, uint32_t instance,
@@ -791,40 +942,64 @@ void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const
   for (int64_t rr = 0; rr < (int64_t)num_replicas; ++rr) {
     uint32_t r = (uint32_t)rr;
     uint8_t *x = (uint8_t *)malloc((size_t)n);
-    uint8_t *best = best_states + (size_t)r * n;
     if (init)
       memcpy(x, init + (size_t)r * n, (size_t)n);
     else
       sweep_init_bits(n, r, instance, key, x);
-    double e = csr_energy(n, row_ptr, col, val, diag, x);
-    double e_best = e;
-    memcpy(best, x, (size_t)n);
-    uint32_t w[4];
-    for (int t = 0; t < num_sweeps; ++t) {
-      float ts = tscale[t];
-      for (int pos = 0; pos < n; ++pos) {
-        const int j = order ? order[pos] : pos;
-        if (order || (j & 3) == 0)
-          sweep_thr_words((uint32_t)(j >> 2), r, (uint32_t)t, instance, key, w);
-        float thr = or_neg_ln_u(w[j & 3]) * ts;
-        float h = diag[j];
-        for (int k = row_ptr[j]; k < row_ptr[j + 1]; ++k)
-          if (x[col[k]]) h = h + val[k];
-        float de = x[j] ? -h : h;
-        if (de < thr) {
-          x[j] = (uint8_t)(1 - x[j]);
-          e += (double)de;
-        }
-      }
-      if (e < e_best) {
-        e_best = e;
-        memcpy(best, x, (size_t)n);
-      }
-    }
+    csr_chain(n, row_ptr, col, val, diag, tscale, 0, num_sweeps, r, instance, key, order, x,
+              best_states + (size_t)r * n, best_energies + r);
     if (final_states) memcpy(final_states + (size_t)r * n, x, (size_t)n);
-    best_energies[r] = e_best;
     free(x);
   }
+}
+
+/* Population annealing over sparse sweep-mode replicas (os_csr_batch_population,
+ * no reference counterpart).  Step k = 0..K-1 runs csr_chain over sweeps
+ * k*theta .. at tscale[k] from the current states (step 0: the sweep-mode
+ * random bits); the best of replica index r is the step's best where it is
+ * below the best so far (step 0: taken as is).  After every step but the
+ * last, or_resample with dbeta = 1/tscale[k+1] - 1/tscale[k], the running
+ * energies at the step's end and u = or_resample_uniform(k) picks the state
+ * replica j continues from.  ancestors: optional [(K-1) x R].               */
+void or_population_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                       const float *diag, const float *tscale, int K, int theta, uint32_t R,
+                       uint64_t seed, uint32_t instance, const int32_t *order,
+                       uint8_t *final_states, uint8_t *best_states, double *best_energies,
+                       uint32_t *ancestors, int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  float *ts = (float *)malloc(sizeof(float) * (size_t)K * theta);
+  for (int k = 0; k < K; ++k)
+    for (int t = 0; t < theta; ++t) ts[(size_t)k * theta + t] = tscale[k];
+  uint8_t *xs = (uint8_t *)malloc((size_t)R * n);
+  uint8_t *xs2 = (uint8_t *)malloc((size_t)R * n);
+  uint8_t *sb = (uint8_t *)malloc((size_t)R * n);
+  double *es = (double *)malloc(sizeof(double) * R);
+  double *seb = (double *)malloc(sizeof(double) * R);
+  uint32_t *anc = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  for (uint32_t r = 0; r < R; ++r) sweep_init_bits(n, r, instance, key, xs + (size_t)r * n);
+  for (int k = 0; k < K; ++k) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int64_t rr = 0; rr < (int64_t)R; ++rr) {
+      const uint32_t r = (uint32_t)rr;
+      es[r] = csr_chain(n, row_ptr, col, val, diag, ts, k * theta, theta, r, instance, key, order,
+                        xs + (size_t)r * n, sb + (size_t)r * n, seb + r);
+    }
+    for (uint32_t r = 0; r < R; ++r)
+      if (k == 0 || seb[r] < best_energies[r]) {
+        best_energies[r] = seb[r];
+        memcpy(best_states + (size_t)r * n, sb + (size_t)r * n, (size_t)n);
+      }
+    if (k + 1 >= K) break;
+    const double dbeta = 1.0 / (double)tscale[k + 1] - 1.0 / (double)tscale[k];
+    or_resample(es, R, dbeta, or_resample_uniform(seed, instance, (uint32_t)k), anc);
+    if (ancestors) memcpy(ancestors + (size_t)k * R, anc, sizeof(uint32_t) * R);
+    for (uint32_t j = 0; j < R; ++j)
+      memcpy(xs2 + (size_t)j * n, xs + (size_t)anc[j] * n, (size_t)n);
+    uint8_t *tx = xs; xs = xs2; xs2 = tx;
+  }
+  if (final_states) memcpy(final_states, xs, (size_t)R * n);
+  free(ts); free(xs); free(xs2); free(sb); free(es); free(seb); free(anc);
 }
 
 /* ------------------------------------------------------------------------

@@ -113,6 +113,16 @@ void or_sweep_dense(int n, const double *flat, const float *tscale, int num_swee
 /* Replica exchange (os_dense_tempering): num_ladders * K replicas, replica
  * l*K + k starting in slot k, sweeping at tscale[slot]; exchange rounds of
  * adjacent slots after every `interval` sweeps (not after the last).       */
+/* Population annealing (os_dense_population): K steps of theta sweeps at
+ * tscale[k]; systematic resampling between steps with weights
+ * exp(-(1/tscale[k+1] - 1/tscale[k]) (E_i - E_min)) and the uniform of
+ * or_resample_uniform; ancestors optional [(K-1) x R].                     */
+void or_resample(const double *E, uint32_t R, double dbeta, double u, uint32_t *anc);
+double or_resample_uniform(uint64_t seed, uint32_t instance, uint32_t k);
+void or_population_dense(int n, const double *flat, const float *tscale, int K, int theta,
+                         uint32_t R, uint64_t seed, uint32_t instance, uint8_t *final_states,
+                         uint8_t *best_states, double *best_energies, uint32_t *ancestors,
+                         int threads);
 void or_tempering_dense(int n, const double *flat, const float *tscale, int K,
                         uint32_t ladders, int num_sweeps, int interval, uint64_t seed,
                         uint32_t instance, uint8_t *final_states, uint8_t *best_states,
@@ -124,12 +134,23 @@ void or_sweep_csr(int n, const int32_t *row_ptr, const int32_t *col, const float
                   const uint8_t *init, uint8_t *final_states, uint8_t *best_states,
                   double *best_energies, int threads);
 /* or_sweep_csr with the sites of every sweep visited in order[0..n-1]
- * (NULL: 0..n-1); each site keeps its threshold stream.                    */
+ * (NULL: 0..n-1); the update at position pos draws word pos&3 of counter
+ * (pos>>2, r, t), or_sweep_csr's stream for the order 0..n-1.              */
 void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
                         const float *diag, const float *tscale, int num_sweeps,
                         uint32_t num_replicas, uint64_t seed, uint32_t instance,
                         const int32_t *order, const uint8_t *init, uint8_t *final_states,
                         uint8_t *best_states, double *best_energies, int threads);
+/* Population annealing over sparse sweep-mode replicas (os_csr_batch_population):
+ * steps of theta sweeps of or_sweep_csr_order's chains at tscale[k], each
+ * from the FP64-recomputed energy of its start states; systematic
+ * resampling (or_resample) between steps; best per replica index over the
+ * steps.  ancestors: optional [(K-1) x R].                                 */
+void or_population_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                       const float *diag, const float *tscale, int K, int theta, uint32_t R,
+                       uint64_t seed, uint32_t instance, const int32_t *order,
+                       uint8_t *final_states, uint8_t *best_states, double *best_energies,
+                       uint32_t *ancestors, int threads);
 /* Greedy colour classes of a symmetric CSR graph (index order, smallest
  * colour free of lower-index neighbours); order[] by (colour, index),
  * class c = order[cstart[c] .. cstart[c+1]); returns the colour count.     */

@@ -252,3 +252,103 @@ def test_csr_colour_classes_exact_energies_and_tuning(ctx):
     seq = b.anneal(R, ts, seed=5, final_states=True)
     _check_against_oracle(seq, insts, ts, R, 5, 0)
     b.close()
+
+
+# --------------------------------------------------------------------------
+# population annealing on the persistent sparse kernels
+# --------------------------------------------------------------------------
+
+
+def _exact_batch():
+    """Integer couplings and quarter-step diagonals: every dE and energy is
+    exact, so running energies do not depend on summation order."""
+    insts = []
+    for i, (n, d) in enumerate([(64, 3), (333, 6), (1024, 8), (47, 4)]):
+        rp, cl, vl, dg = instances.regular_csr(n, d, seed=4, instance=i)
+        insts.append((rp, cl, vl, np.round(np.asarray(dg, np.float64) * 4) / 4))
+    return insts
+
+
+def _check_population(out, insts, ts, theta, R, seed, base, orders=None):
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        od = None if orders is None else orders[i]
+        fin, bs, be, anc = oracle.population_csr(rp, cl, vl, dg, ts, theta, R, seed=seed,
+                                                 instance=base + i, order=od, threads=8)
+        assert (out["ancestors"][i] == anc).all(), f"ancestors differ, instance {i}"
+        assert (out["final_states"][i] == fin).all(), f"final states differ, instance {i}"
+        assert (out["best_states"][i] == bs).all(), f"best states differ, instance {i}"
+        e64 = oracle.csr_energy(rp, cl, vl, dg, bs)
+        np.testing.assert_allclose(out["best_energies"][i], e64, rtol=1e-9, atol=1e-12)
+        assert min(len(np.unique(a)) for a in anc) < R  # resampling selected
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_csr_population_matches_oracle(ctx, use_graph):
+    """Sequential site order on the mixed batch (real weights, hubs, isolated
+    spins): running energies are the oracle's sums, so ancestors, final and
+    best states equal or_population_csr bit for bit."""
+    insts = _batch()
+    R, theta = 100, 3
+    ts = _tscale(5, 0.2, 3.0)
+    print(f"[gpu] csr population, sequential order, graph={use_graph}", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.population(R, ts, theta, seed=21, instance_base=4, final_states=True,
+                       use_graph=use_graph)
+    _check_population(out, insts, ts, theta, R, 21, 4)
+    b.close()
+
+
+def test_csr_population_colour_classes_match_oracle(ctx):
+    insts = _exact_batch()
+    R, theta = 64, 2
+    ts = _tscale(6, 0.3, 3.0)
+    print("[gpu] csr population, colour classes", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.population(R, ts, theta, seed=8, final_states=True, site_order=1)
+    orders = [b.site_order(i)[0] for i in range(len(insts))]
+    _check_population(out, insts, ts, theta, R, 8, 0, orders)
+    b.close()
+
+
+def test_csr_population_at_equal_temperatures_is_plain_annealing(ctx):
+    insts = _exact_batch()
+    R, K, theta = 64, 4, 3
+    print("[gpu] csr population at equal temperatures", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    a = b.population(R, np.full(K, 0.7, np.float32), theta, seed=2, final_states=True)
+    p = b.anneal(R, np.full(K * theta, 0.7, np.float32), seed=2, final_states=True)
+    b.close()
+    assert (a["ancestors"] == np.arange(R)).all()
+    for i in range(len(insts)):
+        assert (a["final_states"][i] == p["final_states"][i]).all()
+        assert (a["best_states"][i] == p["best_states"][i]).all()
+    assert (a["best_energies"] == p["best_energies"]).all()
+
+
+def test_csr_population_rejects_bad_schedules(ctx):
+    b = onesolver_amd.CsrBatch(ctx, _exact_batch()[:1])
+    ts = np.array([2.0, 1.0], np.float32)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.population(64, ts, 0)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.population(64, np.zeros(0, np.float32), 2)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.population(64, np.array([2.0, 0.0], np.float32), 2)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.population(64, ts, 2, site_order=2)
+    b.close()
+
+
+def test_closing_a_context_closes_what_was_created_on_it():
+    """Objects created on a context use its stream until destroyed; closing
+    the context destroys them first, so none outlives the os_ctx."""
+    c = onesolver_amd.Context(0)
+    b = onesolver_amd.CsrBatch(c, _exact_batch()[:1])
+    p = onesolver_amd.DenseProblem(c, np.eye(8))
+    print("[gpu] context close with live children", flush=True)
+    c.close()
+    assert b._h is None and p._h is None
+    b.close()
+    p.close()
+    with pytest.raises(onesolver_amd.OneSolverError):
+        onesolver_amd.CsrBatch(c, _exact_batch()[:1])

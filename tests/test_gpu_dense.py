@@ -135,6 +135,25 @@ def test_dense_streaming_pass_grouping(ctx, monkeypatch, group, refresh):
         np.testing.assert_allclose(out["best_energies"][i], oracle.energy(qs[i], bs), rtol=1e-12)
 
 
+@pytest.mark.parametrize("variant", [20, 21, 30, 31, 40, 50])
+def test_dense_streaming_decide_chunking(ctx, monkeypatch, variant):
+    """Every operand chunking of the decide launches (OS_STREAM_DECIDE_VARIANT:
+    blocks per round trip, next chunk requested early or not) gives the
+    oracle's chains; 7 blocks per pass leave up to 13 pending blocks."""
+    n, R, S = 1100, 64, 3
+    q = _int_instance(n, 8)
+    ts = _tscale(S)
+    monkeypatch.setenv("OS_STREAM_GROUP", "7")
+    monkeypatch.setenv("OS_STREAM_DECIDE_VARIANT", str(variant))
+    print(f"[gpu] dense streaming decide variant {variant}", flush=True)
+    prob = onesolver_amd.DenseProblem(ctx, q)
+    out = prob.anneal(R, ts, seed=6, instance=1, final_states=True)
+    prob.close()
+    fin, bs, be = oracle.sweep_dense(q, n, ts, R, seed=6, instance=1)
+    assert (out["final_states"] == fin).all()
+    assert (out["best_states"] == bs).all()
+
+
 def test_dense_streaming_sk_energies_and_drift(ctx):
     n, R, S = 2048, 64, 6
     b = onesolver_amd.DenseBatch(ctx, sk=(1, n, 4, 0))

@@ -377,6 +377,162 @@ def test_tempering_keeps_boltzmann_marginals():
 
 
 # --------------------------------------------------------------------------
+# population annealing
+# --------------------------------------------------------------------------
+
+_PA_Q = 2.0 * np.array([[0.3, -1.0, 0.5, 0.0], [-1.0, -0.2, 0.7, 0.4], [0.5, 0.7, 0.1, -0.6],
+                        [0.0, 0.4, -0.6, 0.2]])
+_PA_STATES = ((np.arange(16)[:, None] >> np.arange(4)) & 1).astype(np.uint8)
+
+
+def _boltzmann_chi2(states, q, beta):
+    """Pearson chi^2 (15 dof) of 4-spin states against exp(-beta E)."""
+    e = oracle.energy(q.astype(np.float32).astype(np.float64), _PA_STATES)
+    counts = np.bincount(states @ (1 << np.arange(4)), minlength=16)
+    p = np.exp(-beta * (e - e.min()))
+    p /= p.sum()
+    R = len(states)
+    return ((counts - R * p) ** 2 / (R * p)).sum()
+
+
+def test_resample_at_equal_weights_keeps_every_replica():
+    rng = np.random.default_rng(1)
+    E = rng.normal(size=300)
+    # (u = 0 exactly puts the targets on the cumulative weights, where
+    # rounding of (j + u) / R * W decides the tie)
+    for u in (1e-6, 0.37, 0.999999):
+        assert (oracle.resample(E, 0.0, u) == np.arange(300)).all()
+        assert (oracle.resample(np.full(300, 2.5), 1.7, u) == np.arange(300)).all()
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_resample_is_systematic(seed):
+    # systematic resampling: ancestors ascend, replica i gets floor or ceil of
+    # R p_i offspring, and over u the mean count is R p_i exactly, where p_i is
+    # the importance weight exp(-dbeta E_i) normalised
+    rng = np.random.default_rng(seed)
+    R = 500
+    E = rng.normal(scale=3.0, size=R)
+    dbeta = 0.6
+    p = np.exp(-dbeta * (E - E.min()))
+    p /= p.sum()
+    mean = np.zeros(R)
+    M = 2000
+    for m in range(M):
+        anc = oracle.resample(E, dbeta, (m + 0.5) / M)
+        assert (np.diff(anc.astype(np.int64)) >= 0).all()
+        c = np.bincount(anc, minlength=R)
+        assert c.sum() == R
+        assert ((c >= np.floor(R * p - 1e-9)) & (c <= np.ceil(R * p + 1e-9))).all()
+        mean += c
+    assert np.abs(mean / M - R * p).max() < 2.0 / M * 2 + 1e-9
+
+
+@pytest.mark.parametrize("b0,b1", [(0.5, 1.0), (1.0, 2.0), (0.2, 1.5)])
+def test_resample_moves_boltzmann_samples_to_the_colder_temperature(b0, b1):
+    # i.i.d. samples of exp(-b0 E) resampled with dbeta = b1 - b0 follow
+    # exp(-b1 E); the same samples not resampled do not
+    e = oracle.energy(_PA_Q.astype(np.float32).astype(np.float64), _PA_STATES)
+    p0 = np.exp(-b0 * (e - e.min()))
+    p0 /= p0.sum()
+    rng = np.random.default_rng(int(10 * b0 + b1))
+    idx = rng.choice(16, size=60000, p=p0)
+    anc = oracle.resample(e[idx], b1 - b0, rng.random())
+    # chi^2 inflated by the weights' effective sample size: 70 for 15 dof
+    assert _boltzmann_chi2(_PA_STATES[idx[anc]], _PA_Q, b1) < 70.0
+    assert _boltzmann_chi2(_PA_STATES[idx], _PA_Q, b1) > 1000.0
+
+
+def test_resample_uniform_is_a_53_bit_uniform():
+    u = np.array([oracle.resample_uniform(7, 3, k) for k in range(20000)])
+    assert ((u >= 0.0) & (u < 1.0)).all()
+    assert (u * 2.0 ** 53 == np.floor(u * 2.0 ** 53)).all()
+    assert (u * 2.0 ** 29 != np.floor(u * 2.0 ** 29)).any()  # more than 32 bits
+    assert abs(u.mean() - 0.5) < 0.01
+    assert np.abs(np.sort(u) - (np.arange(20000) + 0.5) / 20000).max() < 0.015  # KS
+    assert oracle.resample_uniform(7, 4, 0) != oracle.resample_uniform(7, 3, 0)
+
+
+def test_population_at_equal_temperatures_is_plain_sweeps():
+    rng = np.random.default_rng(6)
+    n = 40
+    q = rng.integers(-3, 4, (n, n)).astype(np.float64)
+    q = np.triu(q) + np.triu(q, 1).T
+    K, theta, R = 5, 3, 48
+    fin, bs, be, anc = oracle.population_dense(q, n, np.full(K, 1.2, np.float32), theta, R,
+                                               seed=3, instance=5)
+    fin2, bs2, be2 = oracle.sweep_dense(q, n, np.full(K * theta, 1.2, np.float32), R, seed=3,
+                                        instance=5)
+    assert (fin == fin2).all() and (bs == bs2).all() and (be == be2).all()
+    assert anc.shape == (K - 1, R) and (anc == np.arange(R)).all()
+
+
+def test_population_anneal_ends_in_boltzmann():
+    # one sweep per temperature from beta 0.1 to 3: the resampled population
+    # ends distributed as exp(-3 E); plain annealing on the same schedule
+    # (no resampling) is far from it
+    betas = np.geomspace(0.1, 3.0, 20)
+    ts = (1.0 / betas).astype(np.float32)
+    R = 60000
+    fin, _, _, anc = oracle.population_dense(_PA_Q, 4, ts, 1, R, seed=1, threads=8)
+    assert _boltzmann_chi2(fin, _PA_Q, 3.0) < 45.0  # 15 dof: p ~ 1e-4
+    assert len(np.unique(anc[-1])) < R  # resampling did select
+    sa, _, _ = oracle.sweep_dense(_PA_Q, 4, ts, R, seed=1, threads=8)
+    assert _boltzmann_chi2(sa, _PA_Q, 3.0) > 1000.0
+
+
+@pytest.mark.parametrize("coloured", [False, True])
+def test_population_csr_at_equal_temperatures_is_plain_sweeps(coloured):
+    # half-integer diagonal and +-1 weights: energies exact, so the FP64
+    # recompute at every step start equals the running energy
+    rp, col, val, diag = _random_csr(80, 5, 7, integer_diag=True)
+    order = oracle.csr_colour_order(rp, col)[0] if coloured else None
+    K, theta, R = 4, 3, 40
+    fin, bs, be, anc = oracle.population_csr(rp, col, val, diag, np.full(K, 0.9, np.float32),
+                                             theta, R, seed=3, instance=2, order=order)
+    fin2, bs2, be2 = oracle.sweep_csr(rp, col, val, diag, np.full(K * theta, 0.9, np.float32), R,
+                                      seed=3, instance=2, order=order)
+    assert (anc == np.arange(R)).all()
+    assert (fin == fin2).all() and (bs == bs2).all() and (be == be2).all()
+
+
+def test_population_csr_anneal_ends_in_boltzmann():
+    # the 4-spin instance as CSR: the sparse population ends distributed as
+    # exp(-3 E); plain sparse sweeps on the same schedule do not
+    q = _PA_Q
+    rp, col, val = [0], [], []
+    for j in range(4):
+        for l in range(4):
+            if l != j:
+                col.append(l)
+                val.append(q[j, l])
+        rp.append(len(col))
+    diag = np.diag(q)
+    betas = np.geomspace(0.1, 3.0, 20)
+    ts = (1.0 / betas).astype(np.float32)
+    R = 60000
+    fin, _, _, anc = oracle.population_csr(rp, col, val, diag, ts, 1, R, seed=2, threads=8)
+    assert _boltzmann_chi2(fin, q, 3.0) < 45.0  # 15 dof: p ~ 1e-4
+    assert len(np.unique(anc[-1])) < R
+    sa, _, _ = oracle.sweep_csr(rp, col, val, diag, ts, R, seed=2, threads=8)
+    assert _boltzmann_chi2(sa, q, 3.0) > 1000.0
+
+
+def test_population_csr_best_is_the_lowest_over_the_steps():
+    # best energies are attained by the best states and never above the
+    # step-0 best (a plain run of the first step's sweeps)
+    rp, col, val, diag = _random_csr(120, 4, 9)
+    ts = (1.0 / np.geomspace(0.3, 3.0, 6)).astype(np.float32)
+    R, theta = 64, 4
+    fin, bs, be, anc = oracle.population_csr(rp, col, val, diag, ts, theta, R, seed=5)
+    e = oracle.csr_energy(rp, col, val, diag, bs)
+    assert np.allclose(e, be, rtol=0, atol=1e-5)  # running sums of fp32 dE
+    _, _, be0 = oracle.sweep_csr(rp, col, val, diag, np.full(theta, ts[0], np.float32), R, seed=5)
+    assert (be <= be0).all() and (be < be0).any()
+    assert anc.shape == (5, R) and (np.diff(anc.astype(np.int64), axis=1) >= 0).all()
+
+
+# --------------------------------------------------------------------------
 # sparse sweeps in colour-class order
 # --------------------------------------------------------------------------
 
