@@ -42,7 +42,12 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
-TRAFFIC_CSR_C3_COLOUR = None  # csr_sweep_col_kernel: not measured yet
+TRAFFIC_CSR_C3_COLOUR = (2 * 83827832.0 + 5959958.1) * 1024  # csr_sweep_col_kernel, 2048 sweeps
+# csr_sweep_col_kernel, one 2048-sweep launch of the config-3 batch (PMC passes): L2 requests and
+# hits, LDS instructions (wave level), LDS bank-conflict and active cycles, VALU instructions
+COUNTERS_CSR_C3_COLOUR = {"tcc_req": 7305888835.0, "tcc_hit": 5704927429.0,
+                          "sq_insts_lds": 25849097501.0, "sq_lds_bank_conflict": 88772708309.0,
+                          "sq_lds_idx_active": 143560519727.0, "sq_insts_valu": 414048828908.0}
 TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
 TRAFFIC_INGEST_8192 = (2 * 643639.7 + 558661.2) * 1024   # parse_lines_kernel<true>, N=8192 text
 
@@ -361,6 +366,24 @@ def run_csr_c3(args):
                      "avg_launch_ms": avg_launch_s * 1e3,
                      "algorithmic_bytes_per_launch": bytes_alg},
     }
+    if order and S == 2048:
+        # beside the algorithmic figure: what the counters of the same launch
+        # give at this run's launch time
+        k = COUNTERS_CSR_C3_COLOUR
+        res["roofline"]["measured"] = {
+            "hbm_GBs": TRAFFIC_CSR_C3_COLOUR / avg_launch_s / 1e9,
+            "l2_requests_per_s": k["tcc_req"] / avg_launch_s,
+            "l2_hit_rate": k["tcc_hit"] / k["tcc_req"],
+            "lds_GBs": k["sq_insts_lds"] * 64 * 4 / avg_launch_s / 1e9,
+            "lds_bank_conflict_cycle_frac": k["sq_lds_bank_conflict"] / k["sq_lds_idx_active"],
+            "valu_lane_instructions_per_attempt": k["sq_insts_valu"] * 64 / (attempts / args.steps),
+            "valu_wave_instructions_per_simd_cycle":
+                k["sq_insts_valu"] / avg_launch_s / (256 * 4 * 2.4e9),
+            "bound": "instruction issue (VALU), see profiles/csr_sweep_col_kernel_sparse512.md"}
+    if order:
+        sites = b.colour_sites()
+        res["config"]["instances_by_sites_per_step"] = {
+            str(g): int((sites == g).sum()) for g in (4, 8, 16)}
     imax = int(np.argmax(ns))
     b.close()
     ctx.close()
@@ -430,6 +453,56 @@ def run_copy_calibration(args):
     return {"metric": "copy bandwidth", "value": 2 * nbytes * args.steps / el / 1e9, "unit": "GB/s",
             "n_gpus": 1, "steps": args.steps, "warmup": 1, "higher_is_better": True,
             "bytes_read_per_copy": nbytes, "bytes_written_per_copy": nbytes}
+
+
+def run_tempering_sparse(args):
+    """Replica exchange on the config-3 batch (512 sparse instances): 16
+    ladders of 16 temperature slots (beta 0.1 .. 3.0 geometric) = 256
+    replicas per instance, 2048 sweeps, an exchange round every --pt-interval
+    sweeps, colour-class site order; plain annealing with the config-3
+    schedule and budget runs in the same call for comparison."""
+    import onesolver_amd
+    from onesolver_amd import instances
+
+    insts = instances.batch_c3(512)
+    S, K, L, iv = 2048, 16, 16, args.pt_interval
+    ts_slots = onesolver_amd.tscale_from_beta(np.geomspace(0.1, 3.0, K))
+    ctx = onesolver_amd.Context(0)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    kw = dict(seed=1234, best_states=False, site_order=1)
+    for _ in range(args.warmup):
+        b.tempering(L, ts_slots, S, iv, **kw)
+    t0 = time.perf_counter()
+    best = np.inf
+    for i in range(args.steps):
+        out = b.tempering(L, ts_slots, S, iv, **kw)
+        best = min(best, float(np.sum(out["inst_best_energy"])))
+        log(f"step {i}: tempering sum of best energies {best:.4f}")
+    elapsed = time.perf_counter() - t0
+    ts_sa = onesolver_amd.tscale_from_beta(np.geomspace(0.1, 3.0, S))
+    b.anneal(K * L, ts_sa, **kw)
+    t1 = time.perf_counter()
+    sa = b.anneal(K * L, ts_sa, **kw)
+    sa_s = time.perf_counter() - t1
+    b.close()
+    ctx.close()
+    sum_n = int(sum(len(d) for (_, _, _, d) in insts))
+    attempts = float(K * L) * S * sum_n * args.steps
+    return {
+        "metric": "spin-flip attempts/s", "value": attempts / elapsed, "unit": "attempts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 fields, fp64 energies",
+        "data": "synthetic (seeded random d-regular graphs)",
+        "config": {"workload": f"replica exchange, config-3 sparse batch (512 instances), {L} "
+                               f"ladders x {K} slots (beta 0.1..3.0), {S} sweeps, exchange every "
+                               f"{iv} sweeps, colour-class order",
+                   "ladders": L, "slots": K, "sweeps": S, "exchange_interval": iv},
+        "roofline": None,
+        "sum_best_energy": best,
+        "annealing_same_budget": {"sum_best_energy": float(np.sum(sa["inst_best_energy"])),
+                                  "ms": sa_s * 1e3},
+    }
 
 
 def run_tempering(args):
@@ -826,6 +899,7 @@ def main():
     ap.add_argument("--pa-path", choices=["dense", "sparse"], default="dense")
     ap.add_argument("--ingest-n", type=int, default=8192)
     ap.add_argument("--pt-interval", type=int, default=16)
+    ap.add_argument("--pt-path", choices=["dense", "sparse"], default="dense")
     ap.add_argument("--exh-n", type=int, default=32)
     ap.add_argument("--c5-replicas", type=int, default=4096)
     ap.add_argument("--c5-sweeps", type=int, default=1024)
@@ -868,7 +942,8 @@ def main():
         print(json.dumps(run_exhaustive(args)))
         return
     if args.workload == "pt":
-        print(json.dumps(run_tempering(args)))
+        print(json.dumps(run_tempering_sparse(args) if args.pt_path == "sparse"
+                         else run_tempering(args)))
         return
     if args.workload == "pa":
         print(json.dumps(run_population(args)))

@@ -340,6 +340,40 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
                         double *best_energies, double *run_best_energies, uint8_t *best_states,
                         uint8_t *final_states, double *inst_best_energy,
                         int32_t *inst_best_replica, os_sweep_stats *stats);
+/* Replica exchange on a sparse batch (os_csr_batch_tempering): every
+ * instance runs num_ladders ladders of num_temperatures slots; replica
+ * r = l * num_temperatures + k starts in slot k of ladder l and sweeps at
+ * the scale of its slot, with the thresholds and site order of
+ * os_csr_batch_anneal.  Segments of exchange_interval sweeps each start from
+ * the FP64 energy of their start states; after every segment but the last,
+ * round t pairs slots (k, k+1), k = t mod 2, ..., of every ladder exactly as
+ * os_dense_tempering does (u from Philox counter (k, l, t, ((instance_base +
+ * i) << 8) | 0x13)), on the running energies at the segment's end.  The best
+ * state of a replica is its lowest over the segments (sweep ends and segment
+ * starts; the earliest on ties).  No reference counterpart: the reference
+ * recommends replica exchange (benchmarks/annealing/performance.md:52-59)
+ * but anneals only. */
+typedef struct os_csr_tempering_params {
+  uint32_t num_ladders;
+  uint32_t num_temperatures; /* >= 2 slots per ladder */
+  const float *tscale;       /* [num_temperatures] host array, > 0 */
+  int32_t num_sweeps;        /* >= 1 */
+  int32_t exchange_interval; /* sweeps between exchange rounds, >= 1 */
+  uint64_t seed;             /* Philox key */
+  uint32_t instance_base;    /* stream id of instance 0 (< 2^24) */
+  int32_t site_order;        /* as in os_csr_params */
+  int32_t use_graph;         /* replay every segment from the captured hipGraph */
+  int32_t tune;              /* pick the launch shape by timing candidates on first use */
+} os_csr_tempering_params;
+
+/* Replica exchange (see os_csr_tempering_params).  Outputs optional, as in
+ * os_csr_batch_anneal, indexed by replica; best_energies are the FP64
+ * energies of the best states; final_slots: per instance [R] the slot of
+ * every replica at the end. */
+int os_csr_batch_tempering(os_csr_batch *batch, const os_csr_tempering_params *params,
+                           double *best_energies, uint8_t *best_states, uint8_t *final_states,
+                           uint32_t *final_slots, double *inst_best_energy,
+                           int32_t *inst_best_replica, os_sweep_stats *stats);
 /* Population annealing (see os_csr_population_params).  Outputs optional, as
  * in os_csr_batch_anneal; best_energies are the FP64 energies of the best
  * states; ancestors: per instance [num_steps - 1][num_replicas]. */
@@ -352,11 +386,17 @@ int os_csr_batch_population(os_csr_batch *batch, const os_csr_population_params 
  * rows have at most 8 couplings), 0 for the general one. */
 int os_csr_batch_launch_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
                               int32_t *prefetch);
-/* The colour-class order's launch shape in use (0 before its first run):
- * a wave updates sites_per_step sites of each of its 64/sites_per_step
- * replicas per step. */
+/* The colour-class order's launch shape in use (0 before its first run).
+ * A wave updates G sites of each of its 64/G replicas per step; every
+ * instance has its own G (4, 8 or 16), chosen with the launch shape: larger
+ * instances take more sites per step, so that their work items, which set
+ * the length of a run, take fewer steps.  sites_per_step: the common G, or 0
+ * when the instances differ.  OS_CSR_COLOUR_SITES (4, 8 or 16) in the
+ * environment fixes one G for all. */
 int os_csr_batch_colour_shape(os_csr_batch *batch, int32_t *waves_per_group, int32_t *groups,
                               int32_t *sites_per_step);
+/* G of every instance ([num_instances], 0 before the first colour-class run). */
+int os_csr_batch_colour_sites(os_csr_batch *batch, int32_t *sites_per_step);
 /* The site order of site_order 1 for one instance (outputs optional):
  * order [n] lists the sites class by class, each class ascending;
  * class_start [num_classes + 1] bounds the classes.  Greedy colouring in

@@ -597,6 +597,15 @@ class CsrPopulationParams(ctypes.Structure):
                 ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32)]
 
 
+class CsrTemperingParams(ctypes.Structure):
+    """os_csr_tempering_params of include/onesolver.h."""
+    _fields_ = [("num_ladders", ctypes.c_uint32), ("num_temperatures", ctypes.c_uint32),
+                ("tscale", _vp), ("num_sweeps", ctypes.c_int32),
+                ("exchange_interval", ctypes.c_int32), ("seed", ctypes.c_uint64),
+                ("instance_base", ctypes.c_uint32), ("site_order", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32)]
+
+
 def _csr_lib():
     L = lib()
     if not getattr(L, "_csr_ready", False):
@@ -608,8 +617,12 @@ def _csr_lib():
         L.os_csr_batch_population.argtypes = [_vp, ctypes.POINTER(CsrPopulationParams), _f64p,
                                               _u8p, _u8p, ctypes.POINTER(ctypes.c_uint32),
                                               _f64p, _i32p, ctypes.POINTER(SweepStats)]
+        L.os_csr_batch_tempering.argtypes = [_vp, ctypes.POINTER(CsrTemperingParams), _f64p,
+                                             _u8p, _u8p, ctypes.POINTER(ctypes.c_uint32),
+                                             _f64p, _i32p, ctypes.POINTER(SweepStats)]
         L.os_csr_batch_launch_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
         L.os_csr_batch_colour_shape.argtypes = [_vp, _i32p, _i32p, _i32p]
+        L.os_csr_batch_colour_sites.argtypes = [_vp, _i32p]
         L.os_csr_batch_site_order.argtypes = [_vp, ctypes.c_int32, _i32p, _i32p, _i32p]
         L._csr_ready = True
     return L
@@ -657,11 +670,18 @@ class CsrBatch:
 
     def colour_shape(self):
         """(waves per work-group, resident work-groups, sites per step) of the
-        colour-class order; zeros before its first run."""
+        colour-class order -- sites per step 0 when the instances differ
+        (colour_sites()); zeros before its first run."""
         w, g, s = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
         _check(_csr_lib().os_csr_batch_colour_shape(self._h, ctypes.byref(w), ctypes.byref(g),
                                                     ctypes.byref(s)))
         return w.value, g.value, s.value
+
+    def colour_sites(self):
+        """Sites per wave step of every instance in the colour-class order."""
+        out = np.zeros(len(self.ns), dtype=np.int32)
+        _check(_csr_lib().os_csr_batch_colour_sites(self._h, _p(out, _i32p)))
+        return out
 
     def site_order(self, instance):
         """(order [n], class starts) of the colour-class order of an instance."""
@@ -714,6 +734,32 @@ class CsrBatch:
             out.append(flat[at:at + R * n].reshape(R, n))
             at += R * n
         return out
+
+    def tempering(self, ladders, tscale, num_sweeps, exchange_interval, seed=1, instance_base=0,
+                  best_states=True, final_states=False, use_graph=True, tune=True,
+                  site_order=0):
+        """Replica exchange (os_csr_batch_tempering): `ladders` ladders of
+        len(tscale) slots per instance; final_slots [I, R]."""
+        ts = np.ascontiguousarray(tscale, dtype=np.float32)
+        I, K = len(self.ns), len(ts)
+        R = int(ladders) * K
+        prm = CsrTemperingParams(ladders, K, ts.ctypes.data, num_sweeps, exchange_interval, seed,
+                                 instance_base, int(site_order), int(use_graph), int(tune))
+        tot = int(self.ns.sum()) * R
+        be = np.zeros((I, R), dtype=np.float64)
+        bs = np.zeros(tot, dtype=np.uint8) if best_states else None
+        fs = np.zeros(tot, dtype=np.uint8) if final_states else None
+        sl = np.zeros((I, R), dtype=np.uint32)
+        ib = np.zeros(I, dtype=np.float64)
+        ir = np.zeros(I, dtype=np.int32)
+        st = SweepStats()
+        _check(_csr_lib().os_csr_batch_tempering(
+            self._h, ctypes.byref(prm), _p(be, _f64p), _p(bs, _u8p), _p(fs, _u8p),
+            sl.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), _p(ib, _f64p), _p(ir, _i32p),
+            ctypes.byref(st)))
+        return {"best_energies": be, "best_states": self._split(bs, R),
+                "final_states": self._split(fs, R), "final_slots": sl, "inst_best_energy": ib,
+                "inst_best_replica": ir, "stats": st.as_dict()}
 
     def population(self, num_replicas, tscale, sweeps_per_step, seed=1, instance_base=0,
                    best_states=True, final_states=False, use_graph=True, tune=True,

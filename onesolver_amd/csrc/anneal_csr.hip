@@ -61,10 +61,10 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
     const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
     const float4 *__restrict__ ell_val, const float *__restrict__ diag,
-    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
-    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
-    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
-    int max_words) {
+    const float *__restrict__ tscale, const float *__restrict__ tsr,
+    const uint32_t *__restrict__ init_bits, uint32_t *__restrict__ best_bits,
+    uint32_t *__restrict__ final_bits, double *__restrict__ e_best,
+    double *__restrict__ e_final, int *__restrict__ queue, int max_words) {
   extern __shared__ uint32_t lds[];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -74,6 +74,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
   const int tb = run->sweep_base;
+  const int use_tsr = run->use_tsr;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(queue, 1);
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_kernel(
     double eb = e;
     for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
     for (int t = 0; t < S; ++t) {
-      const float ts = tscale[tb + t];
+      const float ts = use_tsr ? tsr[d.rep_off + r] : tscale[tb + t];
       for (int j4 = 0; j4 < n; j4 += 4) {
         const uint4 o = philox4x32_10(
             make_uint4((uint32_t)(j4 >> 2), r, (uint32_t)(tb + t), id | TAG_SWEEP_THR), k0, k1);
@@ -213,10 +214,10 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
     const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
     const float4 *__restrict__ ell_val, const float *__restrict__ diag,
-    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
-    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
-    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
-    int max_words) {
+    const float *__restrict__ tscale, const float *__restrict__ tsr,
+    const uint32_t *__restrict__ init_bits, uint32_t *__restrict__ best_bits,
+    uint32_t *__restrict__ final_bits, double *__restrict__ e_best,
+    double *__restrict__ e_final, int *__restrict__ queue, int max_words) {
   extern __shared__ uint32_t lds[];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -226,6 +227,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
   const int tb = run->sweep_base;
+  const int use_tsr = run->use_tsr;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(queue, 1);
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
     double eb = e;
     for (int wd = 0; wd < words; ++wd) bb[(size_t)wd * Rp] = xs[wd * 64 + lane];
     for (int t = 0; t < S; ++t) {
-      const float ts = tscale[tb + t];
+      const float ts = use_tsr ? tsr[d.rep_off + r] : tscale[tb + t];
       SiteRow row = load_row(ecol, eval, dg, 0, chunks, n);
       // the state word of the current sites (j >> 5), kept in a register:
       // flips update it and are written through to LDS for the gathers
@@ -318,11 +320,197 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
 // order draws word pos&3 of Philox counter (pos>>2, r, t) -- for the order
 // 0..n-1 the stream of the sequential kernels -- and each site keeps its
 // field order and dE; the FP64 running energy adds the dE of a sweep per
-// lane and then over the G lanes of a replica.  Thresholds are computed for
+// lane and then over the G lanes of a replica (and the start energy, per
+// site as the oracle computes it, sums sites s, s+G, ... in lane s, then over
+// the G lanes).  Thresholds are computed for
 // windows of 4G positions (one Philox call per lane, four logarithms) into
 // LDS.  Flips are atomic XORs on the LDS planes, since sites of one class
 // can share a word.
-template <int kW, int G>
+
+// One work item of the colour-class kernel: replicas [wi.y * 64/G, +64/G) of
+// instance d, G sites per step; xs: this wave's LDS planes, then the
+// threshold window.
+template <int G>
+__device__ __forceinline__ void col_item(
+    const CsrInst &d, int2 wi, uint32_t *xs, int plane_words, int lane, int S, int Rp,
+    uint32_t k0, uint32_t k1, uint32_t base, int init_mode, int tb, int use_tsr,
+    const float *__restrict__ tsr,
+    const int4 *__restrict__ ell_col, const float4 *__restrict__ ell_val,
+    const float *__restrict__ diag, const int32_t *__restrict__ corder,
+    const int32_t *__restrict__ cstart, const int4 *__restrict__ pell_col,
+    const float4 *__restrict__ pell_val, const float *__restrict__ pdiag,
+    const float *__restrict__ tscale, const uint32_t *__restrict__ init_bits,
+    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
+    double *__restrict__ e_best, double *__restrict__ e_final) {
+  static_assert(G == 4 || G == 8 || G == 16, "sites per step");
+  constexpr int RW = 64 / G;  // replicas per wave
+  const int rr = lane / G, s = lane % G;
+  // thresholds of the positions [win, win + 4G) for the RW replicas
+  float *thw = reinterpret_cast<float *>(xs + plane_words);
+  auto bit = [&](int j) { return (xs[(j >> 5) * RW + rr] >> (j & 31)) & 1u; };
+  const int n = d.n, words = d.words, chunks = d.width >> 2, ncol = d.ncol;
+  const uint32_t r = (uint32_t)(wi.y * RW + rr);
+  const uint32_t id = (base + (uint32_t)wi.x) << 8;
+  const int4 *ecol = ell_col + (d.ell_off >> 2);
+  const float4 *eval = ell_val + (d.ell_off >> 2);
+  const float *dg = diag + d.spin_off;
+  const int32_t *ord = corder + d.spin_off;
+  const int32_t *cs = cstart + d.class_off;
+  // the rows and diagonal in visiting order (position pos: site ord[pos])
+  const int4 *pcol = pell_col + (d.ell_off >> 2);
+  const float4 *pval = pell_val + (d.ell_off >> 2);
+  const float *pdg = pdiag + d.spin_off;
+  uint32_t *bb = best_bits + d.state_off + r;
+  // start state, as in csr_sweep_kernel; the G lanes of a replica share
+  // its words
+  for (int wd = s; wd < words; wd += G) {
+    uint32_t v;
+    if (init_mode) {
+      v = init_bits[d.state_off + (size_t)wd * Rp + r];
+    } else {
+      const uint4 o = philox4x32_10(make_uint4((uint32_t)(wd >> 2), r, 0u, id | TAG_SWEEP_INIT),
+                                    k0, k1);
+      v = pick(o, (uint32_t)wd & 3u);
+    }
+    const int valid = n - 32 * wd;
+    if (valid < 32) v = valid <= 0 ? 0u : (v & ((1u << valid) - 1u));
+    xs[wd * RW + rr] = v;
+  }
+  wave_sync();
+  // FP64 energy: lane s of a replica sums sites s, s + G, ... (each site as
+  // the oracle does: diag + half its set couplings), then the G partial
+  // sums are added
+  double e = 0.0;
+  for (int j = s; j < n; j += G) {
+    double off = 0.0;
+    for (int c = 0; c < chunks; ++c) {
+      const int4 cc = ecol[(size_t)j * chunks + c];
+      const float4 vv = eval[(size_t)j * chunks + c];
+      off = bit(cc.x) ? off + (double)vv.x : off;
+      off = bit(cc.y) ? off + (double)vv.y : off;
+      off = bit(cc.z) ? off + (double)vv.z : off;
+      off = bit(cc.w) ? off + (double)vv.w : off;
+    }
+    if (bit(j)) e += (double)dg[j] + 0.5 * off;
+  }
+#pragma unroll
+  for (int o = 1; o < G; o <<= 1) e += __shfl_xor(e, o, 64);
+  double eb = e;
+  for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
+  for (int t = 0; t < S; ++t) {
+    const float ts = use_tsr ? tsr[d.rep_off + r] : tscale[tb + t];
+    double de_sum = 0.0;
+    int win = -4 * G;  // no window yet
+    // steps of the sweep: positions [p0, p0 + G) of class c (ending at
+    // c1); the rows of the next step are requested before this one runs
+    int c = 0, p0 = cs[0], c1 = cs[1];
+    // the first two chunks of a row travel with the step; rows of more
+    // than 8 couplings read the rest when the step runs
+    struct Step {
+      int j, pos;
+      float dj;
+      int4 c0, c1;
+      float4 v0, v1;
+    };
+    auto fetch = [&](int q0) {
+      Step st;
+      const int pos = min(q0 + s, n - 1);
+      st.pos = pos;
+      st.j = ord[pos];
+      st.dj = pdg[pos];
+      st.c0 = pcol[(size_t)pos * chunks];
+      st.v0 = pval[(size_t)pos * chunks];
+      st.c1 = make_int4(n, n, n, n);
+      st.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (chunks > 1) {
+        st.c1 = pcol[(size_t)pos * chunks + 1];
+        st.v1 = pval[(size_t)pos * chunks + 1];
+      }
+      return st;
+    };
+    Step cur = fetch(p0);
+    for (;;) {
+      int np0 = p0 + G, nc = c, nc1 = c1;
+      if (np0 >= c1) {
+        nc = c + 1;
+        if (nc < ncol) {
+          np0 = cs[nc];
+          nc1 = cs[nc + 1];
+        }
+      }
+      const bool more = nc < ncol;
+      Step nxt = cur;
+      if (more) nxt = fetch(np0);
+      if (p0 + G > win + 4 * G) {
+        // next window: lane (rr, s) draws positions win + 4s .. +3
+        win = p0 & ~3;
+        const uint4 o = philox4x32_10(make_uint4((uint32_t)((win >> 2) + s), r,
+                                                 (uint32_t)(tb + t), id | TAG_SWEEP_THR),
+                                      k0, k1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) thw[(4 * s + q) * RW + rr] = neg_ln_u(pick(o, (uint32_t)q)) * ts;
+        wave_sync();
+      }
+      const int pos = p0 + s;
+      const bool active = pos < c1;
+      const float thr = thw[(pos - win) * RW + rr];
+      float h = cur.dj;
+      h = bit(cur.c0.x) ? h + cur.v0.x : h;
+      h = bit(cur.c0.y) ? h + cur.v0.y : h;
+      h = bit(cur.c0.z) ? h + cur.v0.z : h;
+      h = bit(cur.c0.w) ? h + cur.v0.w : h;
+      if (chunks > 1) {
+        h = bit(cur.c1.x) ? h + cur.v1.x : h;
+        h = bit(cur.c1.y) ? h + cur.v1.y : h;
+        h = bit(cur.c1.z) ? h + cur.v1.z : h;
+        h = bit(cur.c1.w) ? h + cur.v1.w : h;
+      }
+      for (int cc = 2; cc < chunks; ++cc) {
+        const int4 ci = pcol[(size_t)cur.pos * chunks + cc];
+        const float4 vi = pval[(size_t)cur.pos * chunks + cc];
+        h = bit(ci.x) ? h + vi.x : h;
+        h = bit(ci.y) ? h + vi.y : h;
+        h = bit(ci.z) ? h + vi.z : h;
+        h = bit(ci.w) ? h + vi.w : h;
+      }
+      const int j = cur.j;
+      const float de = bit(j) ? -h : h;
+      if (active && de < thr) {
+        atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
+        de_sum += (double)de;
+      }
+      if (!more) break;
+      // a new class reads this class's flips
+      if (nc != c) wave_sync();
+      cur = nxt;
+      p0 = np0;
+      c1 = nc1;
+      c = nc;
+    }
+    wave_sync();
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) de_sum += __shfl_xor(de_sum, off, 64);
+    e += de_sum;
+    if (e < eb) {
+      eb = e;
+      for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
+    }
+  }
+  uint32_t *fb = final_bits + d.state_off + r;
+  for (int wd = s; wd < words; wd += G) fb[(size_t)wd * Rp] = xs[wd * RW + rr];
+  if (s == 0) {
+    e_best[d.rep_off + r] = eb;
+    e_final[d.rep_off + r] = e;
+  }
+  // the next item's start state overwrites these planes
+  wave_sync();
+}
+
+// Colour-class sweeps over a batch whose instances each have their own sites
+// per step (CsrInst::sites: 4, 8 or 16).  plane_words: LDS words per wave for
+// the state planes (the largest words * 64/sites of the batch), followed by
+// 4*64 threshold words.
+template <int kW>
 __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
     const CsrRun *__restrict__ run, const int4 *__restrict__ ell_col,
@@ -330,24 +518,20 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     const int32_t *__restrict__ corder, const int32_t *__restrict__ cstart,
     const int4 *__restrict__ pell_col, const float4 *__restrict__ pell_val,
     const float *__restrict__ pdiag, const float *__restrict__ tscale,
-    const uint32_t *__restrict__ init_bits, uint32_t *__restrict__ best_bits,
-    uint32_t *__restrict__ final_bits, double *__restrict__ e_best, double *__restrict__ e_final,
-    int *__restrict__ queue, int max_words) {
-  static_assert(G == 4 || G == 8 || G == 16, "sites per step");
-  constexpr int RW = 64 / G;  // replicas per wave
+    const float *__restrict__ tsr, const uint32_t *__restrict__ init_bits,
+    uint32_t *__restrict__ best_bits, uint32_t *__restrict__ final_bits,
+    double *__restrict__ e_best, double *__restrict__ e_final, int *__restrict__ queue,
+    int plane_words) {
   extern __shared__ uint32_t lds[];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int rr = lane / G, s = lane % G;
-  uint32_t *xs = lds + (size_t)w * (max_words * RW + 4 * 64);
-  // thresholds of the positions [win, win + 4G) for the RW replicas
-  float *thw = reinterpret_cast<float *>(xs + max_words * RW);
+  uint32_t *xs = lds + (size_t)w * (plane_words + 4 * 64);
   const int num_items = run->num_items, S = run->num_sweeps, Rp = run->Rp;
   const uint32_t k0 = run->k0, k1 = run->k1;
   const uint32_t base = run->instance_base;
   const int init_mode = run->init_mode;
   const int tb = run->sweep_base;
-  auto bit = [&](int j) { return (xs[(j >> 5) * RW + rr] >> (j & 31)) & 1u; };
+  const int use_tsr = run->use_tsr;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(queue, 1);
@@ -355,158 +539,19 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
     if (it >= num_items) break;
     const int2 wi = items[it];
     const CsrInst d = inst[wi.x];
-    const int n = d.n, words = d.words, chunks = d.width >> 2, ncol = d.ncol;
-    const uint32_t r = (uint32_t)(wi.y * RW + rr);
-    const uint32_t id = (base + (uint32_t)wi.x) << 8;
-    const int4 *ecol = ell_col + (d.ell_off >> 2);
-    const float4 *eval = ell_val + (d.ell_off >> 2);
-    const float *dg = diag + d.spin_off;
-    const int32_t *ord = corder + d.spin_off;
-    const int32_t *cs = cstart + d.class_off;
-    // the rows and diagonal in visiting order (position pos: site ord[pos])
-    const int4 *pcol = pell_col + (d.ell_off >> 2);
-    const float4 *pval = pell_val + (d.ell_off >> 2);
-    const float *pdg = pdiag + d.spin_off;
-    uint32_t *bb = best_bits + d.state_off + r;
-    // start state, as in csr_sweep_kernel; the G lanes of a replica share
-    // its words
-    for (int wd = s; wd < words; wd += G) {
-      uint32_t v;
-      if (init_mode) {
-        v = init_bits[d.state_off + (size_t)wd * Rp + r];
-      } else {
-        const uint4 o = philox4x32_10(make_uint4((uint32_t)(wd >> 2), r, 0u, id | TAG_SWEEP_INIT),
-                                      k0, k1);
-        v = pick(o, (uint32_t)wd & 3u);
-      }
-      const int valid = n - 32 * wd;
-      if (valid < 32) v = valid <= 0 ? 0u : (v & ((1u << valid) - 1u));
-      xs[wd * RW + rr] = v;
-    }
-    wave_sync();
-    // FP64 energy in the oracle's order (every lane of a replica computes it)
-    double e = 0.0;
-    for (int j = 0; j < n; ++j) {
-      double off = 0.0;
-      for (int c = 0; c < chunks; ++c) {
-        const int4 cc = ecol[(size_t)j * chunks + c];
-        const float4 vv = eval[(size_t)j * chunks + c];
-        off = bit(cc.x) ? off + (double)vv.x : off;
-        off = bit(cc.y) ? off + (double)vv.y : off;
-        off = bit(cc.z) ? off + (double)vv.z : off;
-        off = bit(cc.w) ? off + (double)vv.w : off;
-      }
-      if (bit(j)) e += (double)dg[j] + 0.5 * off;
-    }
-    double eb = e;
-    for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
-    for (int t = 0; t < S; ++t) {
-      const float ts = tscale[tb + t];
-      double de_sum = 0.0;
-      int win = -4 * G;  // no window yet
-      // steps of the sweep: positions [p0, p0 + G) of class c (ending at
-      // c1); the rows of the next step are requested before this one runs
-      int c = 0, p0 = cs[0], c1 = cs[1];
-      // the first two chunks of a row travel with the step; rows of more
-      // than 8 couplings read the rest when the step runs
-      struct Step {
-        int j, pos;
-        float dj;
-        int4 c0, c1;
-        float4 v0, v1;
-      };
-      auto fetch = [&](int q0) {
-        Step st;
-        const int pos = min(q0 + s, n - 1);
-        st.pos = pos;
-        st.j = ord[pos];
-        st.dj = pdg[pos];
-        st.c0 = pcol[(size_t)pos * chunks];
-        st.v0 = pval[(size_t)pos * chunks];
-        st.c1 = make_int4(n, n, n, n);
-        st.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (chunks > 1) {
-          st.c1 = pcol[(size_t)pos * chunks + 1];
-          st.v1 = pval[(size_t)pos * chunks + 1];
-        }
-        return st;
-      };
-      Step cur = fetch(p0);
-      for (;;) {
-        int np0 = p0 + G, nc = c, nc1 = c1;
-        if (np0 >= c1) {
-          nc = c + 1;
-          if (nc < ncol) {
-            np0 = cs[nc];
-            nc1 = cs[nc + 1];
-          }
-        }
-        const bool more = nc < ncol;
-        Step nxt = cur;
-        if (more) nxt = fetch(np0);
-        if (p0 + G > win + 4 * G) {
-          // next window: lane (rr, s) draws positions win + 4s .. +3
-          win = p0 & ~3;
-          const uint4 o = philox4x32_10(make_uint4((uint32_t)((win >> 2) + s), r,
-                                                   (uint32_t)(tb + t), id | TAG_SWEEP_THR),
-                                        k0, k1);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) thw[(4 * s + q) * RW + rr] = neg_ln_u(pick(o, (uint32_t)q)) * ts;
-          wave_sync();
-        }
-        const int pos = p0 + s;
-        const bool active = pos < c1;
-        const float thr = thw[(pos - win) * RW + rr];
-        float h = cur.dj;
-        h = bit(cur.c0.x) ? h + cur.v0.x : h;
-        h = bit(cur.c0.y) ? h + cur.v0.y : h;
-        h = bit(cur.c0.z) ? h + cur.v0.z : h;
-        h = bit(cur.c0.w) ? h + cur.v0.w : h;
-        if (chunks > 1) {
-          h = bit(cur.c1.x) ? h + cur.v1.x : h;
-          h = bit(cur.c1.y) ? h + cur.v1.y : h;
-          h = bit(cur.c1.z) ? h + cur.v1.z : h;
-          h = bit(cur.c1.w) ? h + cur.v1.w : h;
-        }
-        for (int cc = 2; cc < chunks; ++cc) {
-          const int4 ci = pcol[(size_t)cur.pos * chunks + cc];
-          const float4 vi = pval[(size_t)cur.pos * chunks + cc];
-          h = bit(ci.x) ? h + vi.x : h;
-          h = bit(ci.y) ? h + vi.y : h;
-          h = bit(ci.z) ? h + vi.z : h;
-          h = bit(ci.w) ? h + vi.w : h;
-        }
-        const int j = cur.j;
-        const float de = bit(j) ? -h : h;
-        if (active && de < thr) {
-          atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
-          de_sum += (double)de;
-        }
-        if (!more) break;
-        // a new class reads this class's flips
-        if (nc != c) wave_sync();
-        cur = nxt;
-        p0 = np0;
-        c1 = nc1;
-        c = nc;
-      }
-      wave_sync();
-#pragma unroll
-      for (int off = 1; off < G; off <<= 1) de_sum += __shfl_xor(de_sum, off, 64);
-      e += de_sum;
-      if (e < eb) {
-        eb = e;
-        for (int wd = s; wd < words; wd += G) bb[(size_t)wd * Rp] = xs[wd * RW + rr];
-      }
-    }
-    uint32_t *fb = final_bits + d.state_off + r;
-    for (int wd = s; wd < words; wd += G) fb[(size_t)wd * Rp] = xs[wd * RW + rr];
-    if (s == 0) {
-      e_best[d.rep_off + r] = eb;
-      e_final[d.rep_off + r] = e;
-    }
-    // the next item's start state overwrites these planes
-    wave_sync();
+    const int g = __builtin_amdgcn_readfirstlane(d.sites);
+#define OS_COL_ITEM(GG)                                                                         \
+  col_item<GG>(d, wi, xs, plane_words, lane, S, Rp, k0, k1, base, init_mode, tb, use_tsr, tsr, \
+               ell_col,                                                                        \
+               ell_val, diag, corder, cstart, pell_col, pell_val, pdiag, tscale, init_bits,    \
+               best_bits, final_bits, e_best, e_final)
+    if (g == 4)
+      OS_COL_ITEM(4);
+    else if (g == 8)
+      OS_COL_ITEM(8);
+    else
+      OS_COL_ITEM(16);
+#undef OS_COL_ITEM
   }
 }
 
@@ -602,8 +647,9 @@ bool csr_pf_eligible(int max_width) { return max_width <= 8; }
 void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_words,
                        const CsrInst *inst, const int2 *items, const CsrRun *run,
                        const int4 *ell_col, const float4 *ell_val, const float *diag,
-                       const float *tscale, const uint32_t *init_bits, uint32_t *best_bits,
-                       uint32_t *final_bits, double *e_best, double *e_final, int *queue) {
+                       const float *tscale, const float *tsr, const uint32_t *init_bits,
+                       uint32_t *best_bits, uint32_t *final_bits, double *e_best,
+                       double *e_final, int *queue) {
   const size_t lds = (size_t)csr_lds_bytes(waves, max_words);
   switch (waves) {
 #define OS_CSR_CASE(W)                                                                       \
@@ -614,7 +660,8 @@ void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_w
                                          hipFuncAttributeMaxDynamicSharedMemorySize,         \
                                          (int)lds));                                         \
       csr_sweep_pf_kernel<W><<<groups, 64 * W, lds, st>>>(inst, items, run, ell_col, ell_val, \
-                                                          diag, tscale, init_bits, best_bits, \
+                                                          diag, tscale, tsr, init_bits,       \
+                                                          best_bits,                          \
                                                           final_bits, e_best, e_final, queue, \
                                                           max_words);                         \
       break;                                                                                 \
@@ -623,7 +670,8 @@ void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_w
       OS_HIP_CHECK(hipFuncSetAttribute((const void *)csr_sweep_kernel<W>,                    \
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
     csr_sweep_kernel<W><<<groups, 64 * W, lds, st>>>(inst, items, run, ell_col, ell_val, diag,  \
-                                                     tscale, init_bits, best_bits, final_bits,  \
+                                                     tscale, tsr, init_bits, best_bits,         \
+                                                     final_bits,                                \
                                                      e_best, e_final, queue, max_words);        \
     break;
     OS_CSR_CASE(1)
@@ -674,69 +722,53 @@ int csr_max_groups_per_cu(bool pf, int waves, int max_words) {
   return blocks;
 }
 
-int csr_col_lds_bytes(int waves, int sites, int max_words) {
-  return waves * (max_words * (64 / sites) + 4 * 64) * 4;
-}
+int csr_col_lds_bytes(int waves, int plane_words) { return waves * (plane_words + 4 * 64) * 4; }
 
 namespace {
 
-template <int kW, int G>
-const void *col_kernel() {
-  return (const void *)csr_sweep_col_kernel<kW, G>;
-}
-
-const void *col_kernel_for(int waves, int sites) {
-#define OS_COL_PICK(W, GG) \
-  if (waves == W && sites == GG) return col_kernel<W, GG>();
-  OS_COL_PICK(1, 4)
-  OS_COL_PICK(1, 8)
-  OS_COL_PICK(1, 16)
-  OS_COL_PICK(2, 4)
-  OS_COL_PICK(2, 8)
-  OS_COL_PICK(2, 16)
-  OS_COL_PICK(4, 4)
-  OS_COL_PICK(4, 8)
-  OS_COL_PICK(4, 16)
-#undef OS_COL_PICK
-  throw Error(-3, "csr colour sweep kernel: unsupported shape");
+const void *col_kernel_for(int waves) {
+  switch (waves) {
+    case 1: return (const void *)csr_sweep_col_kernel<1>;
+    case 2: return (const void *)csr_sweep_col_kernel<2>;
+    case 4: return (const void *)csr_sweep_col_kernel<4>;
+    default: throw Error(-3, "csr colour sweep kernel: unsupported waves per group");
+  }
 }
 
 }  // namespace
 
-int csr_col_max_groups_per_cu(int waves, int sites, int max_words) {
-  const size_t lds = (size_t)csr_col_lds_bytes(waves, sites, max_words);
+int csr_col_max_groups_per_cu(int waves, int plane_words) {
+  const size_t lds = (size_t)csr_col_lds_bytes(waves, plane_words);
   int blocks = 0;
-  OS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &blocks, col_kernel_for(waves, sites), 64 * waves, lds));
+  OS_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, col_kernel_for(waves),
+                                                           64 * waves, lds));
   return blocks;
 }
 
-void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
+void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_words,
                            const CsrInst *inst, const int2 *items, const CsrRun *run,
                            const int4 *ell_col, const float4 *ell_val, const float *diag,
                            const int32_t *corder, const int32_t *cstart, const int4 *pell_col,
                            const float4 *pell_val, const float *pdiag, const float *tscale,
-                           const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
-                           double *e_best, double *e_final, int *queue) {
-  const size_t lds = (size_t)csr_col_lds_bytes(waves, sites, max_words);
-  const void *k = col_kernel_for(waves, sites);
+                           const float *tsr, const uint32_t *init_bits, uint32_t *best_bits,
+                           uint32_t *final_bits, double *e_best, double *e_final, int *queue) {
+  const size_t lds = (size_t)csr_col_lds_bytes(waves, plane_words);
+  const void *k = col_kernel_for(waves);
   if (lds > 65536)
     OS_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-#define OS_COL_CASE(W, GG)                                                                      \
-  if (waves == W && sites == GG) {                                                              \
-    csr_sweep_col_kernel<W, GG><<<groups, 64 * W, lds, st>>>(                                   \
+#define OS_COL_CASE(W)                                                                          \
+  case W:                                                                                       \
+    csr_sweep_col_kernel<W><<<groups, 64 * W, lds, st>>>(                                       \
         inst, items, run, ell_col, ell_val, diag, corder, cstart, pell_col, pell_val, pdiag,    \
-        tscale, init_bits, best_bits, final_bits, e_best, e_final, queue, max_words);          \
+        tscale, tsr, init_bits, best_bits, final_bits, e_best, e_final, queue, plane_words);   \
+    break;
+  switch (waves) {
+    OS_COL_CASE(1)
+    OS_COL_CASE(2)
+    OS_COL_CASE(4)
+    default:
+      throw Error(-3, "csr colour sweep kernel: unsupported waves per group");
   }
-  OS_COL_CASE(1, 4)
-  OS_COL_CASE(1, 8)
-  OS_COL_CASE(1, 16)
-  OS_COL_CASE(2, 4)
-  OS_COL_CASE(2, 8)
-  OS_COL_CASE(2, 16)
-  OS_COL_CASE(4, 4)
-  OS_COL_CASE(4, 8)
-  OS_COL_CASE(4, 16)
 #undef OS_COL_CASE
   OS_HIP_CHECK(hipGetLastError());
 }

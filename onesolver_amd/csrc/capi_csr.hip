@@ -46,25 +46,32 @@ struct CsrBatch {
   // population annealing: best planes and energies of every replica index
   // over the steps, ancestors of a resampling round
   DevBuf pop_best, pop_eb, pop_anc;
+  // replica exchange: per-replica temperature scales ([I][Rp], read when
+  // CsrRun::use_tsr is set)
+  DevBuf tsr;
   int num_items = 0;
   // launch shape chosen by the tuner (0: not tuned); reg: row-prefetching kernel
   int waves = 0, groups = 0;
   bool reg = false;
   int max_width = 0;
-  // colour-class site order: its launch shape (0: not tuned) and work items
-  // ((instance, group of 64/sites replicas), for sites = items_sites)
-  int c_waves = 0, c_groups = 0, c_sites = 0;
+  // colour-class site order: its launch shape (0: not tuned), the policy
+  // that gives each instance its sites per step (c_uniform: one value for
+  // all, else the smallest of 4, 8, 16 with at most c_limit steps per class
+  // sweep of n sites), and the work items ((instance, group of 64/sites_i
+  // replicas)) built for (items_uniform, items_limit, items_c_Rp)
+  int c_waves = 0, c_groups = 0, c_uniform = 0, c_limit = 0;
   DevBuf items_c;
-  int num_items_c = 0, items_sites = 0, items_c_Rp = 0;
+  int num_items_c = 0, items_uniform = -1, items_limit = -1, items_c_Rp = 0;
+  int c_plane_words = 0;  // LDS state words per wave: max of words_i * 64/sites_i
   std::vector<int> order_by_work;  // instances, largest first
   // the run that enqueue() launches: 0 sequential, 1 colour classes
   int mode = 0;
   // captured graph and what it was captured for
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int g_waves = 0, g_groups = 0, g_Rp = 0, g_mode = 0, g_sites = 0;
+  int g_waves = 0, g_groups = 0, g_Rp = 0, g_mode = 0, g_uniform = 0, g_limit = 0;
   bool g_reg = false;
-  const void *g_tscale = nullptr;
+  const void *g_tscale = nullptr, *g_items_c = nullptr;
 
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -77,30 +84,30 @@ struct CsrBatch {
     for (auto *b : {&d_inst, &ell_col, &ell_val, &diag, &diag64, &row_ptr, &col, &val64, &items,
                     &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &corder,
                     &cstart, &items_c, &pell_col, &pell_val, &pdiag, &pop_best, &pop_eb,
-                    &pop_anc})
+                    &pop_anc, &tsr})
       b->release();
   }
 
   // mode 0: sequential sweeps with shape (rg, w, g); mode 1: colour classes
-  // with w waves, g work-groups and `sites` sites per step (items_c built
-  // for that many sites)
-  void enqueue(bool rg, int w, int g, bool energy, int sites = 0) {
+  // with w waves and g work-groups over the items in items_c
+  void enqueue(bool rg, int w, int g, bool energy) {
     hipStream_t st = ctx->stream;
     launch_csr_queue_reset(st, queue.as<int>());
     if (mode == 1)
-      launch_csr_col_sweeps(st, w, sites, g, max_words, d_inst.as<CsrInst>(), items_c.as<int2>(),
+      launch_csr_col_sweeps(st, w, g, c_plane_words, d_inst.as<CsrInst>(), items_c.as<int2>(),
                             run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(),
                             diag.as<float>(), corder.as<int32_t>(), cstart.as<int32_t>(),
                             pell_col.as<int4>(), pell_val.as<float4>(), pdiag.as<float>(),
-                            tscale.as<float>(), init.as<uint32_t>(), best.as<uint32_t>(),
+                            tscale.as<float>(), tsr.as<float>(), init.as<uint32_t>(),
+                            best.as<uint32_t>(),
                             fin.as<uint32_t>(), eb.as<double>(), ef.as<double>(),
                             queue.as<int>());
     else
       launch_csr_sweeps(st, rg, w, g, max_words, d_inst.as<CsrInst>(), items.as<int2>(),
                         run.as<CsrRun>(), ell_col.as<int4>(), ell_val.as<float4>(),
-                        diag.as<float>(), tscale.as<float>(), init.as<uint32_t>(),
-                        best.as<uint32_t>(), fin.as<uint32_t>(), eb.as<double>(),
-                        ef.as<double>(), queue.as<int>());
+                        diag.as<float>(), tscale.as<float>(), tsr.as<float>(),
+                        init.as<uint32_t>(), best.as<uint32_t>(), fin.as<uint32_t>(),
+                        eb.as<double>(), ef.as<double>(), queue.as<int>());
     if (energy)
       launch_csr_energy(st, d_inst.as<CsrInst>(), items.as<int2>(), num_items, Rp,
                         row_ptr.as<int32_t>(), col.as<int32_t>(), val64.as<double>(),
@@ -129,7 +136,7 @@ void shape_for(CsrBatch &b, int Rp) {
     return wa > wc;
   });
   b.order_by_work = order;
-  b.items_sites = 0;
+  b.items_uniform = -1;
   const int groups = Rp / 64;
   std::vector<int2> items;
   items.reserve((size_t)b.I * groups);
@@ -151,6 +158,7 @@ void shape_for(CsrBatch &b, int Rp) {
   b.eb.ensure(reps);
   b.ef.ensure(reps);
   b.e64.ensure(reps);
+  b.tsr.ensure(sizeof(float) * (size_t)std::max<int64_t>(rep, 1));
   b.queue.ensure(sizeof(int) * 64);
   b.run.ensure(sizeof(CsrRun));
   OS_HIP_CHECK(hipStreamSynchronize(s));
@@ -158,21 +166,47 @@ void shape_for(CsrBatch &b, int Rp) {
   b.drop_graph();
 }
 
-// Work items of the colour kernel for `sites` sites per step: (instance,
-// group of 64/sites replicas), largest instances first.
-void col_items(CsrBatch &b, int sites) {
-  if (b.items_sites == sites && b.items_c_Rp == b.Rp) return;
-  const int per = 64 / sites, groups = b.Rp / per;
+// Sites per step of an n-site instance under (uniform, limit).
+int sites_for(int n, int uniform, int limit) {
+  if (uniform) return uniform;
+  if ((n + 3) / 4 <= limit) return 4;
+  if ((n + 7) / 8 <= limit) return 8;
+  return 16;
+}
+
+// Sets every instance's sites per step for the policy (uniform, limit) and
+// builds the colour kernel's work items: (instance, group of 64/sites_i
+// replicas), the items of longer estimated runtime first (steps per sweep
+// times row cost).
+void col_items(CsrBatch &b, int uniform, int limit) {
+  if (b.items_uniform == uniform && b.items_limit == limit && b.items_c_Rp == b.Rp) return;
+  std::vector<int> order(b.I);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<int64_t> est(b.I);
+  int plane = 0;
+  for (int i = 0; i < b.I; ++i) {
+    CsrInst &d = b.inst[i];
+    d.sites = sites_for(d.n, uniform, limit);
+    est[i] = ((int64_t)(d.n + d.sites - 1) / d.sites + d.ncol) * (d.width + 8);
+    plane = std::max(plane, d.words * (64 / d.sites));
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return est[a] > est[c]; });
   std::vector<int2> items;
-  items.reserve((size_t)b.I * groups);
-  for (int i : b.order_by_work)
+  for (int i : order) {
+    const int groups = b.Rp / (64 / b.inst[i].sites);
     for (int gidx = 0; gidx < groups; ++gidx) items.push_back(make_int2(i, gidx));
+  }
+  hipStream_t s = b.ctx->stream;
+  OS_HIP_CHECK(hipMemcpyAsync(b.d_inst.ptr, b.inst.data(), sizeof(CsrInst) * b.I,
+                              hipMemcpyHostToDevice, s));
   b.items_c.ensure(sizeof(int2) * std::max<size_t>(items.size(), 1));
   OS_HIP_CHECK(hipMemcpyAsync(b.items_c.ptr, items.data(), sizeof(int2) * items.size(),
-                              hipMemcpyHostToDevice, b.ctx->stream));
-  OS_HIP_CHECK(hipStreamSynchronize(b.ctx->stream));
+                              hipMemcpyHostToDevice, s));
+  OS_HIP_CHECK(hipStreamSynchronize(s));
   b.num_items_c = (int)items.size();
-  b.items_sites = sites;
+  b.c_plane_words = plane;
+  b.items_uniform = uniform;
+  b.items_limit = limit;
   b.items_c_Rp = b.Rp;
 }
 
@@ -183,11 +217,11 @@ void put_run(CsrBatch &b, CsrRun r) {
                               b.ctx->stream));
 }
 
-float time_launch(CsrBatch &b, bool rg, int w, int g, int sites = 0) {
+float time_launch(CsrBatch &b, bool rg, int w, int g) {
   Ctx &c = *b.ctx;
-  b.enqueue(rg, w, g, false, sites);  // warm-up
+  b.enqueue(rg, w, g, false);  // warm-up
   OS_HIP_CHECK(hipEventRecord(c.ev_start, c.stream));
-  b.enqueue(rg, w, g, false, sites);
+  b.enqueue(rg, w, g, false);
   OS_HIP_CHECK(hipEventRecord(c.ev_stop, c.stream));
   OS_HIP_CHECK(hipEventSynchronize(c.ev_stop));
   float ms = 0.f;
@@ -195,29 +229,33 @@ float time_launch(CsrBatch &b, bool rg, int w, int g, int sites = 0) {
   return ms;
 }
 
-// Times the colour kernel's candidate shapes (sites per step, waves per
-// work-group, resident work-groups) on a short run of the batch itself.
+// Times the colour kernel's candidates -- the policy for sites per step (all
+// 4; at most 512, 256, 128 or 64 steps per class sweep; all 16), waves per
+// work-group, resident work-groups -- on a run of up to 8 sweeps of the
+// batch itself: long enough that the items of the largest instances, which
+// set the length of a run, weigh as they do in a full one.
 void tune_col(CsrBatch &b, const CsrRun &full) {
   Ctx &c = *b.ctx;
   CsrRun probe = full;
-  probe.num_sweeps = std::min(full.num_sweeps, 2);
+  probe.num_sweeps = std::min(full.num_sweeps, 8);
   float best_ms = 0.f;
-  for (int sites : {4, 8, 16}) {
-    col_items(b, sites);
+  for (int limit : {1 << 30, 512, 256, 128, 64, 0}) {
+    col_items(b, 0, limit);
     put_run(b, probe);
     for (int w : {1, 2, 4}) {
-      const int maxb = csr_col_max_groups_per_cu(w, sites, b.max_words);
+      const int maxb = csr_col_max_groups_per_cu(w, b.c_plane_words);
       if (maxb < 1) continue;
       std::vector<int> gpcs{maxb};
       if (maxb > 1) gpcs.push_back(std::max(1, maxb / 2));
       for (int gpc : gpcs) {
         const int g = std::max(1, std::min(c.num_cus * gpc, (b.num_items_c + w - 1) / w));
-        const float ms = time_launch(b, false, w, g, sites);
+        const float ms = time_launch(b, false, w, g);
         if (b.c_waves == 0 || ms < best_ms) {
           best_ms = ms;
           b.c_waves = w;
           b.c_groups = g;
-          b.c_sites = sites;
+          b.c_uniform = 0;
+          b.c_limit = limit;
         }
       }
     }
@@ -266,7 +304,8 @@ void put_tscale(CsrBatch &b, const float *ts, int S) {
 
 struct Shape {
   bool rg;
-  int waves, groups, sites;
+  int waves, groups;
+  int uniform, limit;  // colour-class policy (mode 1)
 };
 
 // The launch shape of a run in `site_order` (tuned on first use when `tune`).
@@ -289,33 +328,34 @@ Shape select_shape(CsrBatch &b, const CsrRun &run, int site_order, bool tune_it)
     const int forced = senv ? std::atoi(senv) : 0;
     OS_REQUIRE(!senv || forced == 4 || forced == 8 || forced == 16,
                "OS_CSR_COLOUR_SITES must be 4, 8 or 16");
-    if (forced && forced != b.c_sites) {
-      b.c_sites = forced;
+    if (forced && forced != b.c_uniform) {
+      b.c_uniform = forced;
+      b.c_limit = 0;
       b.c_waves = 1;
-      col_items(b, b.c_sites);
+      col_items(b, b.c_uniform, b.c_limit);
       b.c_groups = std::max(
-          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
-                      b.num_items_c));
+          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_plane_words), b.num_items_c));
     }
   }
   if (b.mode == 1 && b.c_waves == 0) {
     if (tune_it) {
       tune_col(b, run);
     } else {
-      b.c_sites = 8;
+      b.c_uniform = 0;
+      b.c_limit = 128;
       b.c_waves = 1;
-      col_items(b, b.c_sites);
+      col_items(b, b.c_uniform, b.c_limit);
       b.c_groups = std::max(
-          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_sites, b.max_words),
-                      b.num_items_c));
+          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_plane_words), b.num_items_c));
     }
   }
-  if (b.mode == 1) col_items(b, b.c_sites);
+  if (b.mode == 1) col_items(b, b.c_uniform, b.c_limit);
   Shape sh;
   sh.rg = b.mode == 0 && b.reg;
   sh.waves = b.mode == 1 ? b.c_waves : b.waves;
   sh.groups = b.mode == 1 ? b.c_groups : b.groups;
-  sh.sites = b.mode == 1 ? b.c_sites : 0;
+  sh.uniform = b.mode == 1 ? b.c_uniform : 0;
+  sh.limit = b.mode == 1 ? b.c_limit : 0;
   return sh;
 }
 
@@ -324,12 +364,13 @@ Shape select_shape(CsrBatch &b, const CsrRun &run, int site_order, bool tune_it)
 void ensure_graph(CsrBatch &b, const Shape &sh) {
   hipStream_t s = b.ctx->stream;
   if (b.gexec && b.g_mode == b.mode && b.g_reg == sh.rg && b.g_waves == sh.waves &&
-      b.g_groups == sh.groups && b.g_sites == sh.sites && b.g_Rp == b.Rp)
+      b.g_groups == sh.groups && b.g_uniform == sh.uniform && b.g_limit == sh.limit &&
+      b.g_items_c == b.items_c.ptr && b.g_Rp == b.Rp)
     return;
   b.drop_graph();
   OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    b.enqueue(sh.rg, sh.waves, sh.groups, true, sh.sites);
+    b.enqueue(sh.rg, sh.waves, sh.groups, true);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(s, &g);
@@ -342,7 +383,9 @@ void ensure_graph(CsrBatch &b, const Shape &sh) {
   b.g_reg = sh.rg;
   b.g_waves = sh.waves;
   b.g_groups = sh.groups;
-  b.g_sites = sh.sites;
+  b.g_uniform = sh.uniform;
+  b.g_limit = sh.limit;
+  b.g_items_c = b.items_c.ptr;
   b.g_Rp = b.Rp;
   b.g_tscale = b.tscale.ptr;
 }
@@ -614,7 +657,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     if (p->use_graph)
       OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
     else
-      b.enqueue(sh.rg, sh.waves, sh.groups, true, sh.sites);
+      b.enqueue(sh.rg, sh.waves, sh.groups, true);
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
     os::collect(b, R, b.best, b.fin, run_best_energies ? &b.eb : nullptr, best_energies,
                 run_best_energies, best_states, final_states, inst_best_energy,
@@ -688,7 +731,7 @@ int os_csr_batch_population(os_csr_batch *b_, const os_csr_population_params *p,
       if (p->use_graph)
         OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
       else
-        b.enqueue(sh.rg, sh.waves, sh.groups, false, sh.sites);
+        b.enqueue(sh.rg, sh.waves, sh.groups, false);
       // best of every replica index: the step's best where it is lower
       os::launch_csr_pop_best(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
                               k == 0, b.eb.as<double>(), b.best.as<uint32_t>(),
@@ -734,6 +777,135 @@ int os_csr_batch_population(os_csr_batch *b_, const os_csr_population_params *p,
   });
 }
 
+int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
+                           double *best_energies, uint8_t *best_states, uint8_t *final_states,
+                           uint32_t *final_slots, double *inst_best_energy,
+                           int32_t *inst_best_replica, os_sweep_stats *stats) {
+  return os::guarded([&] {
+    OS_REQUIRE(b_ && p, "null argument");
+    OS_REQUIRE(p->num_ladders >= 1 && p->num_temperatures >= 2 && p->tscale &&
+                   (uint64_t)p->num_ladders * p->num_temperatures <= (1u << 24),
+               "invalid ladder shape");
+    OS_REQUIRE(p->num_sweeps >= 1, "num_sweeps must be positive");
+    OS_REQUIRE(p->exchange_interval >= 1, "exchange_interval must be positive");
+    OS_REQUIRE(p->site_order == 0 || p->site_order == 1, "site_order must be 0 or 1");
+    const int K = (int)p->num_temperatures;
+    for (int k = 0; k < K; ++k)
+      OS_REQUIRE(p->tscale[k] > 0.0f && std::isfinite(p->tscale[k]),
+                 "temperature scales must be positive");
+    CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+    OS_REQUIRE((uint64_t)p->instance_base + (uint64_t)b.I <= (1u << 24),
+               "instance ids must stay below 2^24");
+    Ctx &c = *b.ctx;
+    c.activate();
+    hipStream_t s = c.stream;
+    const int L = (int)p->num_ladders;
+    const int R = K * L;
+    const int Rp = (R + 63) / 64 * 64;
+    const int S = p->num_sweeps, iv = p->exchange_interval;
+    os::shape_for(b, Rp);
+    // per-sweep scales are not read (use_tsr); the array still covers S sweeps
+    std::vector<float> ones((size_t)S, 1.0f);
+    os::put_tscale(b, ones.data(), S);
+    os::CsrRun run{};
+    run.num_items = b.num_items;
+    run.Rp = Rp;
+    run.k0 = (uint32_t)p->seed;
+    run.k1 = (uint32_t)(p->seed >> 32);
+    run.instance_base = p->instance_base;
+    run.num_sweeps = std::min(S, iv);
+    const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
+    if (p->use_graph) os::ensure_graph(b, sh);
+    const size_t planes = sizeof(uint32_t) * (size_t)std::max<int64_t>(b.total_words * Rp, 1);
+    b.pop_best.ensure(planes);
+    b.pop_eb.ensure(sizeof(double) * (size_t)b.I * Rp);
+    // slot of every replica and replica at every slot, per instance; the
+    // scale of every row (padding rows: slot 0)
+    std::vector<uint32_t> slot_of((size_t)b.I * R), rep_at((size_t)b.I * R);
+    for (size_t i = 0; i < slot_of.size(); ++i) {
+      slot_of[i] = (uint32_t)(i % R % K);
+      rep_at[i] = (uint32_t)(i % R);
+    }
+    std::vector<float> tsr_h((size_t)b.I * Rp, p->tscale[0]);
+    auto upload_tsr = [&]() {
+      for (int i = 0; i < b.I; ++i)
+        for (int r = 0; r < R; ++r)
+          tsr_h[(size_t)b.inst[i].rep_off + r] = p->tscale[slot_of[(size_t)i * R + r]];
+      OS_HIP_CHECK(hipMemcpyAsync(b.tsr.ptr, tsr_h.data(), sizeof(float) * tsr_h.size(),
+                                  hipMemcpyHostToDevice, s));
+    };
+    upload_tsr();
+    std::vector<double> ef((size_t)b.I * Rp);
+    uint32_t round = 0;
+    int segments = 0;
+    OS_HIP_CHECK(hipEventRecord(c.ev_start, s));
+    for (int s0 = 0; s0 < S; s0 += iv) {
+      const int ns = std::min(iv, S - s0);
+      run.sweep_base = s0;
+      run.num_sweeps = ns;
+      run.init_mode = s0 > 0 ? 1 : 0;
+      run.use_tsr = 1;
+      os::put_run(b, run);
+      if (p->use_graph)
+        OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
+      else
+        b.enqueue(sh.rg, sh.waves, sh.groups, false);
+      ++segments;
+      os::launch_csr_pop_best(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
+                              s0 == 0, b.eb.as<double>(), b.best.as<uint32_t>(),
+                              b.pop_eb.as<double>(), b.pop_best.as<uint32_t>());
+      if (s0 + ns >= S) break;
+      // exchange round on the running energies at the segment's end
+      OS_HIP_CHECK(hipMemcpyAsync(ef.data(), b.ef.ptr, sizeof(double) * ef.size(),
+                                  hipMemcpyDeviceToHost, s));
+      OS_HIP_CHECK(hipStreamSynchronize(s));
+      for (int i = 0; i < b.I; ++i) {
+        const uint32_t tag = ((p->instance_base + (uint32_t)i) << 8) | os::TAG_EXCHANGE;
+        const double *e = ef.data() + b.inst[i].rep_off;
+        for (int l = 0; l < L; ++l) {
+          uint32_t *ra = rep_at.data() + (size_t)i * R + (size_t)l * K;
+          for (int k = (int)(round & 1u); k + 1 < K; k += 2) {
+            const uint32_t a = ra[k], bb = ra[k + 1];
+            const double d =
+                (1.0 / (double)p->tscale[k] - 1.0 / (double)p->tscale[k + 1]) * (e[a] - e[bb]);
+            const uint4 o = os::philox4x32_10(
+                make_uint4((uint32_t)k, (uint32_t)l, round, tag), run.k0, run.k1);
+            const double u = (double)(o.x >> 8) * 0x1p-24;
+            if (d >= 0.0 || u < std::exp(d)) {
+              ra[k] = bb;
+              ra[k + 1] = a;
+              slot_of[(size_t)i * R + a] = (uint32_t)(k + 1);
+              slot_of[(size_t)i * R + bb] = (uint32_t)k;
+            }
+          }
+        }
+      }
+      ++round;
+      upload_tsr();
+      // the next segment continues from the final states
+      OS_HIP_CHECK(hipMemcpyAsync(b.init.ptr, b.fin.ptr, planes, hipMemcpyDeviceToDevice, s));
+    }
+    OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
+    os::launch_csr_energy(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
+                          b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
+                          b.diag64.as<double>(), b.pop_best.as<uint32_t>(), b.e64.as<double>());
+    os::collect(b, R, b.pop_best, b.fin, nullptr, best_energies, nullptr, best_states,
+                final_states, inst_best_energy, inst_best_replica);
+    if (final_slots) std::memcpy(final_slots, slot_of.data(), sizeof(uint32_t) * slot_of.size());
+    float ms = 0.f;
+    OS_HIP_CHECK(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
+    if (stats) {
+      stats->kernel_ms = ms;
+      stats->total_ms = ms;
+      stats->kernel_launches = segments;
+      stats->max_field_drift = 0.0;
+      stats->attempts = (uint64_t)b.total_spins * (uint64_t)R * (uint64_t)S;
+      stats->pass_avg_ms = 0.0;
+      stats->pass_launches = 0;
+    }
+  });
+}
+
 int os_csr_batch_launch_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_t *groups,
                               int32_t *prefetch) {
   return os::guarded([&] {
@@ -752,7 +924,22 @@ int os_csr_batch_colour_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_
     const CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
     if (waves_per_group) *waves_per_group = b.c_waves;
     if (groups) *groups = b.c_groups;
-    if (sites_per_step) *sites_per_step = b.c_sites;
+    if (sites_per_step) {
+      // the common value, or 0 when the instances differ (or before a run)
+      int v = b.items_uniform >= 0 ? b.inst[0].sites : 0;
+      for (const auto &d : b.inst)
+        if (d.sites != v) v = 0;
+      *sites_per_step = v;
+    }
+  });
+}
+
+int os_csr_batch_colour_sites(os_csr_batch *b_, int32_t *sites_per_step) {
+  return os::guarded([&] {
+    OS_REQUIRE(b_ && sites_per_step, "null argument");
+    const CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+    for (int i = 0; i < b.I; ++i)
+      sites_per_step[i] = b.items_uniform >= 0 ? b.inst[i].sites : 0;
   });
 }
 

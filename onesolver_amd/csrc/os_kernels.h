@@ -116,6 +116,8 @@ struct CsrInst {
   int64_t state_off;  // first word of the instance's [words][Rp] state planes
   int64_t rep_off;    // first replica in per-replica arrays (Rp per instance)
   int64_t class_off;  // first entry in the class starts (ncol + 1 per instance)
+  int32_t sites;      // colour-class kernel: sites per wave step (4, 8 or 16)
+  int32_t pad_;
 };
 // Parameters of one batched run, read by the kernel from device memory so a
 // captured graph replays with new values.
@@ -127,6 +129,8 @@ struct CsrRun {
   uint32_t k0, k1;    // Philox key
   uint32_t instance_base;
   int32_t sweep_base;  // number of the launch's first sweep (thresholds, tscale index)
+  int32_t use_tsr;     // 1: replica r sweeps at tsr[rep_off + r] (replica exchange)
+  int32_t pad_;
 };
 int csr_lds_bytes(int waves, int max_words);
 // pf: the row-prefetching kernel (batches within csr_pf_eligible)
@@ -136,22 +140,25 @@ void launch_csr_queue_reset(hipStream_t st, int *queue);
 void launch_csr_sweeps(hipStream_t st, bool pf, int waves, int groups, int max_words,
                        const CsrInst *inst, const int2 *items, const CsrRun *run,
                        const int4 *ell_col, const float4 *ell_val, const float *diag,
-                       const float *tscale, const uint32_t *init_bits, uint32_t *best_bits,
-                       uint32_t *final_bits, double *e_best, double *e_final, int *queue);
-// colour-class sweeps: G (4, 8 or 16) sites of 64/G replicas per wave step;
-// items are (instance, group of 64/G replicas); corder lists the sites of
-// each instance class by class (offset spin_off), cstart the class starts
-// (offset class_off); pell_col / pell_val / pdiag hold the ELL rows and the
-// diagonal in that visiting order (offsets ell_off, spin_off)
-int csr_col_lds_bytes(int waves, int sites, int max_words);
-int csr_col_max_groups_per_cu(int waves, int sites, int max_words);
-void launch_csr_col_sweeps(hipStream_t st, int waves, int sites, int groups, int max_words,
+                       const float *tscale, const float *tsr, const uint32_t *init_bits,
+                       uint32_t *best_bits, uint32_t *final_bits, double *e_best,
+                       double *e_final, int *queue);
+// colour-class sweeps: instance i updates G_i = CsrInst::sites (4, 8 or 16)
+// sites of 64/G_i replicas per wave step; items are (instance, group of
+// 64/G_i replicas); corder lists the sites of each instance class by class
+// (offset spin_off), cstart the class starts (offset class_off); pell_col /
+// pell_val / pdiag hold the ELL rows and the diagonal in that visiting order
+// (offsets ell_off, spin_off).  plane_words: LDS words per wave for the
+// state planes, the batch's largest words_i * 64/G_i.
+int csr_col_lds_bytes(int waves, int plane_words);
+int csr_col_max_groups_per_cu(int waves, int plane_words);
+void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_words,
                            const CsrInst *inst, const int2 *items, const CsrRun *run,
                            const int4 *ell_col, const float4 *ell_val, const float *diag,
                            const int32_t *corder, const int32_t *cstart, const int4 *pell_col,
                            const float4 *pell_val, const float *pdiag, const float *tscale,
-                           const uint32_t *init_bits, uint32_t *best_bits, uint32_t *final_bits,
-                           double *e_best, double *e_final, int *queue);
+                           const float *tsr, const uint32_t *init_bits, uint32_t *best_bits,
+                           uint32_t *final_bits, double *e_best, double *e_final, int *queue);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
                        const double *diag64, const uint32_t *bits, double *out);

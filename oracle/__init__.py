@@ -71,6 +71,10 @@ def lib():
                                         ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                         ctypes.c_uint64, ctypes.c_uint32, _i32p, _u8p, _u8p,
                                         _f64p, _u32p, ctypes.c_int]
+        L.or_tempering_csr.argtypes = [ctypes.c_int, _i32p, _i32p, _f32p, _f32p, _f32p,
+                                       ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_uint64, ctypes.c_uint32, _i32p, _u8p, _u8p,
+                                       _f64p, _u32p, ctypes.c_int]
         L.or_warm_init.argtypes = [ctypes.c_int, _f32p, ctypes.c_uint32, ctypes.c_uint64,
                                    ctypes.c_uint32, _u8p]
         L.or_sweep_init.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
@@ -296,6 +300,29 @@ def population_csr(row_ptr, col, val, diag, tscale, theta, num_replicas, seed=1,
                             _p(ts, _f32p), K, theta, R, seed, instance, _p(od, _i32p),
                             _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p), _p(anc, _u32p), threads)
     return fin, bs, be, anc[:K - 1]
+
+
+def tempering_csr(row_ptr, col, val, diag, tscale, ladders, num_sweeps, interval, seed=1,
+                  instance=0, order=None, threads=0):
+    """Sparse replica exchange (or_tempering_csr): (final, best states, best
+    energies, final slots); replicas = ladders * len(tscale)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    cl = np.ascontiguousarray(col, dtype=np.int32)
+    vl = np.ascontiguousarray(val, dtype=np.float32)
+    dg = np.ascontiguousarray(diag, dtype=np.float32)
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    n, K = len(dg), len(ts)
+    R = ladders * K
+    fin = np.zeros((R, n), dtype=np.uint8)
+    bs = np.zeros((R, n), dtype=np.uint8)
+    be = np.zeros(R, dtype=np.float64)
+    sl = np.zeros(R, dtype=np.uint32)
+    od = None if order is None else np.ascontiguousarray(order, dtype=np.int32)
+    lib().or_tempering_csr(n, _p(rp, _i32p), _p(cl, _i32p), _p(vl, _f32p), _p(dg, _f32p),
+                           _p(ts, _f32p), K, ladders, num_sweeps, interval, seed, instance,
+                           _p(od, _i32p), _p(fin, _u8p), _p(bs, _u8p), _p(be, _f64p),
+                           _p(sl, _u32p), threads)
+    return fin, bs, be, sl
 
 
 def warm_init(p, num_replicas, seed=1, instance=0):

@@ -896,19 +896,20 @@ int or_csr_colour_order(int n, const int32_t *row_ptr, const int32_t *col, int32
  * (NULL: 0..n-1).  The threshold of the update at position pos of a sweep
  * is word pos&3 of Philox counter (pos>>2, r, t): for the order 0..n-1 the
  * stream of or_sweep_csr.                                                  */
-/* Sweeps t0 .. t0+num_sweeps-1 of replica r from state x (updated in place):
- * the energy starts as csr_energy(x); best/e_best take the start state and
- * every sweep end below the best so far.  Returns the running energy.      */
+/* Sweeps t0 .. t0+num_sweeps-1 of replica r from state x (updated in place)
+ * at scale tscale[t] (tscale NULL: ts_fixed for every sweep): the energy
+ * starts as csr_energy(x); best/e_best take the start state and every sweep
+ * end below the best so far.  Returns the running energy.                  */
 static double csr_chain(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
-                        const float *diag, const float *tscale, int t0, int num_sweeps,
-                        uint32_t r, uint32_t instance, const uint32_t key[2],
+                        const float *diag, const float *tscale, float ts_fixed, int t0,
+                        int num_sweeps, uint32_t r, uint32_t instance, const uint32_t key[2],
                         const int32_t *order, uint8_t *x, uint8_t *best, double *e_best) {
   double e = csr_energy(n, row_ptr, col, val, diag, x);
   double eb = e;
   memcpy(best, x, (size_t)n);
   uint32_t w[4];
   for (int t = t0; t < t0 + num_sweeps; ++t) {
-    float ts = tscale[t];
+    float ts = tscale ? tscale[t] : ts_fixed;
     for (int pos = 0; pos < n; ++pos) {
       const int j = order ? order[pos] : pos;
       if ((pos & 3) == 0) sweep_thr_words((uint32_t)(pos >> 2), r, (uint32_t)t, instance, key, w);
@@ -946,8 +947,8 @@ void or_sweep_csr_order(int n, const int32_t *row_ptr, const int32_t *col, const
       memcpy(x, init + (size_t)r * n, (size_t)n);
     else
       sweep_init_bits(n, r, instance, key, x);
-    csr_chain(n, row_ptr, col, val, diag, tscale, 0, num_sweeps, r, instance, key, order, x,
-              best_states + (size_t)r * n, best_energies + r);
+    csr_chain(n, row_ptr, col, val, diag, tscale, 0.0f, 0, num_sweeps, r, instance, key, order,
+              x, best_states + (size_t)r * n, best_energies + r);
     if (final_states) memcpy(final_states + (size_t)r * n, x, (size_t)n);
     free(x);
   }
@@ -982,8 +983,8 @@ void or_population_csr(int n, const int32_t *row_ptr, const int32_t *col, const 
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
     for (int64_t rr = 0; rr < (int64_t)R; ++rr) {
       const uint32_t r = (uint32_t)rr;
-      es[r] = csr_chain(n, row_ptr, col, val, diag, ts, k * theta, theta, r, instance, key, order,
-                        xs + (size_t)r * n, sb + (size_t)r * n, seb + r);
+      es[r] = csr_chain(n, row_ptr, col, val, diag, ts, 0.0f, k * theta, theta, r, instance, key,
+                        order, xs + (size_t)r * n, sb + (size_t)r * n, seb + r);
     }
     for (uint32_t r = 0; r < R; ++r)
       if (k == 0 || seb[r] < best_energies[r]) {
@@ -1000,6 +1001,73 @@ void or_population_csr(int n, const int32_t *row_ptr, const int32_t *col, const 
   }
   if (final_states) memcpy(final_states, xs, (size_t)R * n);
   free(ts); free(xs); free(xs2); free(sb); free(es); free(seb); free(anc);
+}
+
+/* Replica exchange over sparse sweep-mode replicas (os_csr_batch_tempering,
+ * no reference counterpart).  Replica r = l*K + k of ladder l starts in
+ * slot k.  Segments of `interval` sweeps (the last may be shorter) run
+ * csr_chain at tscale[slot[r]] from the current states (segment 0: the
+ * sweep-mode random bits); the best of replica r is the segment's best
+ * where it is below the best so far (segment 0: taken as is).  After every
+ * segment but the last, round t pairs slots (k, k+1), k = t%2, t%2+2, ...
+ * of every ladder, with the running energies at the segment's end, exactly
+ * as or_tempering_dense does.  num_sweeps >= 1.                            */
+void or_tempering_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                      const float *diag, const float *tscale, int K, uint32_t ladders,
+                      int num_sweeps, int interval, uint64_t seed, uint32_t instance,
+                      const int32_t *order, uint8_t *final_states, uint8_t *best_states,
+                      double *best_energies, uint32_t *final_slots, int threads) {
+  uint32_t key[2];
+  seed_key(seed, key);
+  const uint32_t R = ladders * (uint32_t)K;
+  uint8_t *xs = (uint8_t *)malloc((size_t)R * n);
+  uint8_t *sb = (uint8_t *)malloc((size_t)R * n);
+  double *es = (double *)malloc(sizeof(double) * R);
+  double *seb = (double *)malloc(sizeof(double) * R);
+  uint32_t *slot = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  uint32_t *at = (uint32_t *)malloc(sizeof(uint32_t) * R);
+  for (uint32_t r = 0; r < R; ++r) {
+    sweep_init_bits(n, r, instance, key, xs + (size_t)r * n);
+    slot[r] = r % (uint32_t)K;
+    at[r] = r;
+  }
+  uint32_t round = 0;
+  for (int s0 = 0; s0 < num_sweeps; s0 += interval) {
+    const int ns = num_sweeps - s0 < interval ? num_sweeps - s0 : interval;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 0 ? threads : 1)
+    for (int64_t rr = 0; rr < (int64_t)R; ++rr) {
+      const uint32_t r = (uint32_t)rr;
+      es[r] = csr_chain(n, row_ptr, col, val, diag, NULL, tscale[slot[r]], s0, ns, r, instance,
+                        key, order, xs + (size_t)r * n, sb + (size_t)r * n, seb + r);
+    }
+    for (uint32_t r = 0; r < R; ++r)
+      if (s0 == 0 || seb[r] < best_energies[r]) {
+        best_energies[r] = seb[r];
+        memcpy(best_states + (size_t)r * n, sb + (size_t)r * n, (size_t)n);
+      }
+    if (s0 + ns >= num_sweeps) break;
+    for (uint32_t l = 0; l < ladders; ++l) {
+      uint32_t *ra = at + (size_t)l * K;
+      for (int k = (int)(round & 1u); k + 1 < K; k += 2) {
+        const uint32_t a = ra[k], b = ra[k + 1];
+        const double d = (1.0 / (double)tscale[k] - 1.0 / (double)tscale[k + 1]) * (es[a] - es[b]);
+        uint32_t ctr[4] = {(uint32_t)k, l, round, (instance << 8) | TAG_EXCHANGE};
+        uint32_t o[4];
+        or_philox4x32_10(ctr, key, o);
+        const double u = (double)(o[0] >> 8) * 0x1p-24;
+        if (d >= 0.0 || u < exp(d)) {
+          ra[k] = b;
+          ra[k + 1] = a;
+          slot[a] = (uint32_t)(k + 1);
+          slot[b] = (uint32_t)k;
+        }
+      }
+    }
+    ++round;
+  }
+  if (final_states) memcpy(final_states, xs, (size_t)R * n);
+  if (final_slots) memcpy(final_slots, slot, sizeof(uint32_t) * R);
+  free(xs); free(sb); free(es); free(seb); free(slot); free(at);
 }
 
 /* ------------------------------------------------------------------------

@@ -151,6 +151,16 @@ void or_population_csr(int n, const int32_t *row_ptr, const int32_t *col, const 
                        uint64_t seed, uint32_t instance, const int32_t *order,
                        uint8_t *final_states, uint8_t *best_states, double *best_energies,
                        uint32_t *ancestors, int threads);
+/* Replica exchange over sparse sweep-mode replicas (os_csr_batch_tempering):
+ * segments of `interval` sweeps of or_sweep_csr_order's chains at the scale
+ * of each replica's slot, each from the FP64-recomputed energy of its start
+ * states; or_tempering_dense's exchange rounds between segments; best per
+ * replica over the segments.                                                */
+void or_tempering_csr(int n, const int32_t *row_ptr, const int32_t *col, const float *val,
+                      const float *diag, const float *tscale, int K, uint32_t ladders,
+                      int num_sweeps, int interval, uint64_t seed, uint32_t instance,
+                      const int32_t *order, uint8_t *final_states, uint8_t *best_states,
+                      double *best_energies, uint32_t *final_slots, int threads);
 /* Greedy colour classes of a symmetric CSR graph (index order, smallest
  * colour free of lower-index neighbours); order[] by (colour, index),
  * class c = order[cstart[c] .. cstart[c+1]); returns the colour count.     */

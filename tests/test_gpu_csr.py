@@ -229,6 +229,24 @@ def test_csr_colour_classes_match_oracle(ctx, monkeypatch, sites):
     b.close()
 
 
+def test_csr_colour_mixed_sites_per_step_match_oracle(ctx):
+    """The default policy (no tuning: at most 128 steps per class sweep)
+    gives small instances 4 sites per step, mid-size 8 and large 16, in one
+    launch; every instance still follows the oracle's class-order chains."""
+    insts = [instances.regular_csr(n, d, seed=3, instance=i)
+             for i, (n, d) in enumerate([(64, 3), (700, 5), (1500, 8), (300, 4), (2048, 3)])]
+    insts.append(_irregular(70, 1))
+    R, S = 64, 5
+    ts = _tscale(S)
+    print("[gpu] csr colour classes, mixed sites per step", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.anneal(R, ts, seed=17, instance_base=2, final_states=True, site_order=1, tune=False)
+    assert list(b.colour_sites()) == [4, 8, 16, 4, 16, 4]
+    assert b.colour_shape()[2] == 0
+    _check_colour_against_oracle(b, out, insts, ts, R, 17, 2)
+    b.close()
+
+
 def test_csr_colour_classes_exact_energies_and_tuning(ctx):
     """Integer couplings and quarter-step diagonals: the running energies are
     exact, so they equal the oracle's; the tuner picks a shape, and the
@@ -248,7 +266,8 @@ def test_csr_colour_classes_exact_energies_and_tuning(ctx):
         assert (out["final_states"][i] == fin).all() and (out["best_states"][i] == bs).all()
         assert (out["run_best_energies"][i] == be).all()
     w, g, sites = b.colour_shape()
-    assert w in (1, 2, 4) and g >= 1 and sites in (4, 8, 16)
+    assert w in (1, 2, 4) and g >= 1 and sites in (0, 4, 8, 16)
+    assert set(b.colour_sites()) <= {4, 8, 16}
     seq = b.anneal(R, ts, seed=5, final_states=True)
     _check_against_oracle(seq, insts, ts, R, 5, 0)
     b.close()
@@ -352,3 +371,79 @@ def test_closing_a_context_closes_what_was_created_on_it():
     p.close()
     with pytest.raises(onesolver_amd.OneSolverError):
         onesolver_amd.CsrBatch(c, _exact_batch()[:1])
+
+
+# --------------------------------------------------------------------------
+# replica exchange on the persistent sparse kernels
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_csr_tempering_matches_oracle(ctx, use_graph):
+    """Sequential site order on the mixed batch: running energies are the
+    oracle's sums, so final slots, final and best states equal
+    or_tempering_csr bit for bit (a partial last segment included)."""
+    insts = _batch()
+    K, L, S, iv = 4, 16, 11, 3
+    ts = _tscale(K, 0.3, 3.0)
+    print(f"[gpu] csr tempering, sequential order, graph={use_graph}", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.tempering(L, ts, S, iv, seed=23, instance_base=6, final_states=True,
+                      use_graph=use_graph)
+    b.close()
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        fin, bs, be, sl = oracle.tempering_csr(rp, cl, vl, dg, ts, L, S, iv, seed=23,
+                                               instance=6 + i, threads=8)
+        assert (out["final_slots"][i] == sl).all(), f"slots differ, instance {i}"
+        assert (out["final_states"][i] == fin).all(), f"final states differ, instance {i}"
+        assert (out["best_states"][i] == bs).all(), f"best states differ, instance {i}"
+        np.testing.assert_allclose(out["best_energies"][i], oracle.csr_energy(rp, cl, vl, dg, bs),
+                                   rtol=1e-9, atol=1e-12)
+    moved = sum(int((out["final_slots"][i] != np.arange(K * L) % K).sum())
+                for i in range(len(insts)))
+    assert moved > 0  # exchanges happened
+
+
+def test_csr_tempering_colour_classes_match_oracle(ctx):
+    insts = _exact_batch()
+    K, L, S, iv = 8, 8, 8, 2
+    ts = _tscale(K, 0.3, 3.0)
+    print("[gpu] csr tempering, colour classes", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    out = b.tempering(L, ts, S, iv, seed=4, final_states=True, site_order=1)
+    for i, (rp, cl, vl, dg) in enumerate(insts):
+        order = b.site_order(i)[0]
+        fin, bs, be, sl = oracle.tempering_csr(rp, cl, vl, dg, ts, L, S, iv, seed=4, instance=i,
+                                               order=order, threads=8)
+        assert (out["final_slots"][i] == sl).all(), f"slots differ, instance {i}"
+        assert (out["final_states"][i] == fin).all(), f"final states differ, instance {i}"
+        assert (out["best_states"][i] == bs).all(), f"best states differ, instance {i}"
+    b.close()
+
+
+def test_csr_tempering_with_equal_temperatures_is_plain_annealing(ctx):
+    insts = _exact_batch()
+    K, L, S = 4, 16, 9
+    print("[gpu] csr tempering at equal temperatures", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    a = b.tempering(L, np.full(K, 0.9, np.float32), S, 2, seed=7, final_states=True)
+    p = b.anneal(K * L, np.full(S, 0.9, np.float32), seed=7, final_states=True)
+    b.close()
+    for i in range(len(insts)):
+        assert (a["final_states"][i] == p["final_states"][i]).all()
+        assert (a["best_states"][i] == p["best_states"][i]).all()
+    assert (a["best_energies"] == p["best_energies"]).all()
+
+
+def test_csr_tempering_rejects_bad_shapes(ctx):
+    b = onesolver_amd.CsrBatch(ctx, _exact_batch()[:1])
+    ts = np.array([2.0, 1.0], np.float32)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.tempering(4, ts, 8, 0)                                   # no exchange interval
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.tempering(4, np.array([2.0], np.float32), 8, 2)          # one slot
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.tempering(4, np.array([2.0, -1.0], np.float32), 8, 2)
+    with pytest.raises(onesolver_amd.OneSolverError):
+        b.tempering(4, ts, 0, 2)                                   # no sweeps
+    b.close()

@@ -500,14 +500,7 @@ def test_population_csr_anneal_ends_in_boltzmann():
     # the 4-spin instance as CSR: the sparse population ends distributed as
     # exp(-3 E); plain sparse sweeps on the same schedule do not
     q = _PA_Q
-    rp, col, val = [0], [], []
-    for j in range(4):
-        for l in range(4):
-            if l != j:
-                col.append(l)
-                val.append(q[j, l])
-        rp.append(len(col))
-    diag = np.diag(q)
+    rp, col, val, diag = _pa_q_csr()
     betas = np.geomspace(0.1, 3.0, 20)
     ts = (1.0 / betas).astype(np.float32)
     R = 60000
@@ -530,6 +523,66 @@ def test_population_csr_best_is_the_lowest_over_the_steps():
     _, _, be0 = oracle.sweep_csr(rp, col, val, diag, np.full(theta, ts[0], np.float32), R, seed=5)
     assert (be <= be0).all() and (be < be0).any()
     assert anc.shape == (5, R) and (np.diff(anc.astype(np.int64), axis=1) >= 0).all()
+
+
+def _pa_q_csr():
+    """The 4-spin population-test instance as CSR (both (j,l) and (l,j))."""
+    rp, col, val = [0], [], []
+    for j in range(4):
+        for l in range(4):
+            if l != j:
+                col.append(l)
+                val.append(_PA_Q[j, l])
+        rp.append(len(col))
+    return rp, col, val, np.diag(_PA_Q)
+
+
+@pytest.mark.parametrize("coloured", [False, True])
+def test_tempering_csr_with_equal_temperatures_is_plain_sweeps(coloured):
+    rp, col, val, diag = _random_csr(80, 5, 11, integer_diag=True)
+    order = oracle.csr_colour_order(rp, col)[0] if coloured else None
+    K, L, S = 4, 8, 14
+    fin, bs, be, sl = oracle.tempering_csr(rp, col, val, diag, np.full(K, 1.3, np.float32), L, S,
+                                           4, seed=3, instance=5, order=order)
+    fin2, bs2, be2 = oracle.sweep_csr(rp, col, val, diag, np.full(S, 1.3, np.float32), L * K,
+                                      seed=3, instance=5, order=order)
+    assert (fin == fin2).all() and (bs == bs2).all() and (be == be2).all()
+    for l in range(L):
+        assert sorted(sl[l * K:(l + 1) * K]) == list(range(K))
+
+
+def test_tempering_csr_exchange_decisions():
+    # one exchange round after one sweep between a hot and a cold slot: pairs
+    # with d >= 0 swap, pairs with d < -15 keep their slots, and in between
+    # the swap count matches sum exp(d)
+    rp, col, val, diag = _random_csr(40, 4, 12, integer_diag=True)
+    ts = np.array([5.0, 1.0], np.float32)
+    L = 512
+    _, _, _, sl = oracle.tempering_csr(rp, col, val, diag, ts, L, 2, 1, seed=11, instance=2)
+    hot, _, _ = oracle.sweep_csr(rp, col, val, diag, np.full(1, ts[0]), 2 * L, seed=11,
+                                 instance=2)
+    cold, _, _ = oracle.sweep_csr(rp, col, val, diag, np.full(1, ts[1]), 2 * L, seed=11,
+                                  instance=2)
+    e_hot = oracle.csr_energy(rp, col, val, diag, hot[0::2])
+    e_cold = oracle.csr_energy(rp, col, val, diag, cold[1::2])
+    d = (1.0 / ts[0] - 1.0 / ts[1]) * (e_hot - e_cold)
+    swapped = sl[0::2] == 1
+    assert (swapped[d >= 0]).all() and (d >= 0).sum() > 10
+    assert not (swapped[d < -12]).any() and (d < -12).sum() > 10
+    mid = (d < 0) & (d >= -12)
+    p = np.exp(d[mid])
+    assert mid.sum() > 300
+    assert abs(swapped[mid].sum() - p.sum()) < 4 * np.sqrt((p * (1 - p)).sum()) + 1
+
+
+def test_tempering_csr_keeps_boltzmann_marginals():
+    rp, col, val, diag = _pa_q_csr()
+    betas = np.array([0.4, 1.3])
+    ts = (1.0 / betas).astype(np.float32)
+    L = 60000
+    fin, _, _, sl = oracle.tempering_csr(rp, col, val, diag, ts, L, 30, 1, seed=21, threads=8)
+    for k, beta in enumerate(betas):
+        assert _boltzmann_chi2(fin[sl == k], _PA_Q, beta) < 45.0, k  # 15 dof: p ~ 1e-4
 
 
 # --------------------------------------------------------------------------
