@@ -42,12 +42,12 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
-TRAFFIC_CSR_C3_COLOUR = (2 * 83827832.0 + 5959958.1) * 1024  # csr_sweep_col_kernel, 2048 sweeps
+TRAFFIC_CSR_C3_COLOUR = (2 * 63019529.1 + 6022566.6) * 1024  # csr_sweep_col_kernel, 2048 sweeps
 # csr_sweep_col_kernel, one 2048-sweep launch of the config-3 batch (PMC passes): L2 requests and
 # hits, LDS instructions (wave level), LDS bank-conflict and active cycles, VALU instructions
-COUNTERS_CSR_C3_COLOUR = {"tcc_req": 7305888835.0, "tcc_hit": 5704927429.0,
+COUNTERS_CSR_C3_COLOUR = {"tcc_req": 6854772925.0, "tcc_hit": 5868742601.0,
                           "sq_insts_lds": 25849097501.0, "sq_lds_bank_conflict": 88772708309.0,
-                          "sq_lds_idx_active": 143560519727.0, "sq_insts_valu": 414048828908.0}
+                          "sq_lds_idx_active": 143560519727.0, "sq_insts_valu": 274165825036.0}
 TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
 TRAFFIC_INGEST_8192 = (2 * 643639.7 + 558661.2) * 1024   # parse_lines_kernel<true>, N=8192 text
 
@@ -379,7 +379,8 @@ def run_csr_c3(args):
             "valu_lane_instructions_per_attempt": k["sq_insts_valu"] * 64 / (attempts / args.steps),
             "valu_wave_instructions_per_simd_cycle":
                 k["sq_insts_valu"] / avg_launch_s / (256 * 4 * 2.4e9),
-            "bound": "instruction issue (VALU), see profiles/csr_sweep_col_kernel_sparse512.md"}
+            "bound": "LDS latency per wave step (neighbour-bit gathers, 62% bank-conflict "
+                     "cycles), see profiles/csr_sweep_col_kernel_sparse512.md"}
     if order:
         sites = b.colour_sites()
         res["config"]["instances_by_sites_per_step"] = {
@@ -721,7 +722,7 @@ def run_batch_cli(args):
             f.write(text)
     cmd = [onesolver_amd.BATCH_CLI_PATH, "--input", inp, "--output", out, "--num-iter", str(S),
            "--num-tries", str(R), "--beta-min", "0.1", "--beta-max", "3", "--beta-kind",
-           "inverse", "--path", "sparse"]
+           "inverse", "--path", "sparse", "--site-order", "colour"]
     for _ in range(args.warmup):
         subprocess.run(cmd, check=True, capture_output=True, timeout=300)
     phases = []
@@ -747,7 +748,8 @@ def run_batch_cli(args):
         "vs_baseline": None, "dtype": "fp32 fields, fp64 energies",
         "data": "synthetic (seeded random d-regular graphs, written as .qubo text)",
         "config": {"workload": f"one-solver-batch: 512 .qubo files ({nbytes / 1e6:.1f} MB) -> "
-                               f"512 CSVs, sparse path, {R} replicas, {S} sweeps",
+                               f"512 CSVs, sparse path (colour-class order), {R} replicas, "
+                               f"{S} sweeps",
                    "instances": len(insts), "replicas": R, "sweeps": S,
                    "sum_n": int(ns.sum()), "text_bytes": nbytes},
         "attempts_per_s": attempts / elapsed,

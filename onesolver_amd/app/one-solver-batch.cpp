@@ -14,7 +14,10 @@
  * together in sweep mode:
  *   - dense (os_dense_batch_*): every instance has the same n <= 16384 and at
  *     least a quarter of its couplings set; or --path dense;
- *   - sparse (os_csr_batch_*): otherwise; or --path sparse.
+ *   - sparse (os_csr_batch_*): otherwise; or --path sparse.  Sweeps visit
+ *     the sites class by class of a greedy colouring (--site-order colour,
+ *     the default: several independent sites per wave step) or in index
+ *     order (--site-order sequential).
  * Instance k draws its random numbers as stream k.  The beta schedule is the
  * reference's (linear or geometric over --num-iter values, one sweep per
  * value); --beta-kind temperature (default) uses beta as the reference does,
@@ -58,11 +61,12 @@ const char *kHelp =
     "  --beta-max arg (=1)                   maximum value of beta\n"
     "  --beta-kind arg (=temperature)        temperature (as the reference) or inverse\n"
     "  --seed arg (=1234)                    seed of the Philox4x32-10 streams\n"
-    "  --path arg (=auto)                    auto, dense or sparse\n";
+    "  --path arg (=auto)                    auto, dense or sparse\n"
+    "  --site-order arg (=colour)            sparse path: colour or sequential\n";
 
 const std::vector<std::string> kNames = {"help",      "input",       "output",   "num-iter",
                                          "num-tries", "schedule-type", "beta-min", "beta-max",
-                                         "beta-kind", "seed",        "path"};
+                                         "beta-kind", "seed",        "path",     "site-order"};
 
 bool is_dir(const std::string &p) {
   struct stat st;
@@ -209,6 +213,7 @@ int main(int argc, char *argv[]) {
     const std::string beta_kind = cli::convert<std::string>(opt, "beta-kind", "temperature");
     const uint64_t seed = cli::convert<uint64_t>(opt, "seed", 1234u);
     const std::string path = cli::convert<std::string>(opt, "path", "auto");
+    const std::string site_order = cli::convert<std::string>(opt, "site-order", "colour");
     if (!opt.values.count("input")) {
       std::cerr << "No input file provided." << std::endl;
       return -1;
@@ -232,6 +237,10 @@ int main(int argc, char *argv[]) {
     }
     if (beta_kind != "temperature" && beta_kind != "inverse") {
       std::cerr << "Unknown beta kind: " << beta_kind << std::endl;
+      return -1;
+    }
+    if (site_order != "colour" && site_order != "sequential") {
+      std::cerr << "Unknown site order: " << site_order << std::endl;
       return -1;
     }
     if (path != "auto" && path != "dense" && path != "sparse") {
@@ -285,7 +294,9 @@ int main(int argc, char *argv[]) {
         dense = dense && in.n == insts[0].n && in.n <= 16384 &&
                 (double)in.nq * 4.0 >= 0.5 * (double)in.n * (in.n - 1);
     }
-    std::cout << "Path: " << (dense ? "dense" : "sparse") << std::endl;
+    std::cout << "Path: " << (dense ? "dense" : "sparse")
+              << (dense ? "" : site_order == "colour" ? " (colour-class order)" : " (sequential order)")
+              << std::endl;
     std::vector<double> best_e(I);
     std::vector<std::vector<uint8_t>> best_x(I);
     if (dense) {
@@ -335,6 +346,7 @@ int main(int argc, char *argv[]) {
       p.instance_base = 0;
       p.use_graph = 1;
       p.tune = 0;
+      p.site_order = site_order == "colour" ? 1 : 0;
       size_t tot = 0;
       for (int32_t n : ns) tot += (size_t)n * R;
       std::vector<double> be((size_t)I * R);

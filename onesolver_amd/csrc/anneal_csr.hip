@@ -32,9 +32,18 @@ __device__ __forceinline__ void wave_sync() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
-// Bit j of replica lane in a wave's transposed planes.
+// Bit j of replica lane in a wave's transposed planes (the bit-field extract
+// reads only the low 5 bits of its offset).
 __device__ __forceinline__ uint32_t plane_bit(const uint32_t *xs, int j, int lane) {
-  return (xs[(j >> 5) * 64 + lane] >> (j & 31)) & 1u;
+  return __builtin_amdgcn_ubfe(xs[(j >> 5) * 64 + lane], (uint32_t)j, 1u);
+}
+
+// h + w if neighbour l's bit is set, else h: fma(1, w, h) is the rounded sum
+// and fma(0, w, h) is h (up to the sign of a zero h, which no comparison or
+// later sum sees), with one instruction fewer than a select.
+__device__ __forceinline__ float add_if_set(const uint32_t *xs, int l, int lane, float w,
+                                            float h) {
+  return fmaf((float)plane_bit(xs, l, lane), w, h);
 }
 
 __device__ __forceinline__ float row_field(const uint32_t *xs, int lane, const int4 *ecol,
@@ -42,10 +51,10 @@ __device__ __forceinline__ float row_field(const uint32_t *xs, int lane, const i
   for (int c = 0; c < chunks; ++c) {
     const int4 cc = ecol[c];
     const float4 vv = eval[c];
-    h = plane_bit(xs, cc.x, lane) ? h + vv.x : h;
-    h = plane_bit(xs, cc.y, lane) ? h + vv.y : h;
-    h = plane_bit(xs, cc.z, lane) ? h + vv.z : h;
-    h = plane_bit(xs, cc.w, lane) ? h + vv.w : h;
+    h = add_if_set(xs, cc.x, lane, vv.x, h);
+    h = add_if_set(xs, cc.y, lane, vv.y, h);
+    h = add_if_set(xs, cc.z, lane, vv.z, h);
+    h = add_if_set(xs, cc.w, lane, vv.w, h);
   }
   return h;
 }
@@ -166,15 +175,15 @@ struct SiteRow {
   // field in the oracle's order: diag, then + w for each set neighbour
   __device__ __forceinline__ float field(const uint32_t *xs, int lane, int chunks) const {
     float h = dj;
-    h = plane_bit(xs, c0.x, lane) ? h + v0.x : h;
-    h = plane_bit(xs, c0.y, lane) ? h + v0.y : h;
-    h = plane_bit(xs, c0.z, lane) ? h + v0.z : h;
-    h = plane_bit(xs, c0.w, lane) ? h + v0.w : h;
+    h = add_if_set(xs, c0.x, lane, v0.x, h);
+    h = add_if_set(xs, c0.y, lane, v0.y, h);
+    h = add_if_set(xs, c0.z, lane, v0.z, h);
+    h = add_if_set(xs, c0.w, lane, v0.w, h);
     if (chunks > 1) {
-      h = plane_bit(xs, c1.x, lane) ? h + v1.x : h;
-      h = plane_bit(xs, c1.y, lane) ? h + v1.y : h;
-      h = plane_bit(xs, c1.z, lane) ? h + v1.z : h;
-      h = plane_bit(xs, c1.w, lane) ? h + v1.w : h;
+      h = add_if_set(xs, c1.x, lane, v1.x, h);
+      h = add_if_set(xs, c1.y, lane, v1.y, h);
+      h = add_if_set(xs, c1.z, lane, v1.z, h);
+      h = add_if_set(xs, c1.w, lane, v1.w, h);
     }
     return h;
   }
@@ -347,7 +356,9 @@ __device__ __forceinline__ void col_item(
   const int rr = lane / G, s = lane % G;
   // thresholds of the positions [win, win + 4G) for the RW replicas
   float *thw = reinterpret_cast<float *>(xs + plane_words);
-  auto bit = [&](int j) { return (xs[(j >> 5) * RW + rr] >> (j & 31)) & 1u; };
+  // bit j of this lane's replica (the bit-field extract reads only the low 5
+  // bits of its offset)
+  auto bit = [&](int j) { return __builtin_amdgcn_ubfe(xs[(j >> 5) * RW + rr], (uint32_t)j, 1u); };
   const int n = d.n, words = d.words, chunks = d.width >> 2, ncol = d.ncol;
   const uint32_t r = (uint32_t)(wi.y * RW + rr);
   const uint32_t id = (base + (uint32_t)wi.x) << 8;
@@ -412,24 +423,31 @@ __device__ __forceinline__ void col_item(
       int4 c0, c1;
       float4 v0, v1;
     };
+    // loads at a uniform base plus a 32-bit byte offset
+    auto at16 = [](const void *base, uint32_t off) {
+      return reinterpret_cast<const char *>(base) + off;
+    };
     auto fetch = [&](int q0) {
       Step st;
       const int pos = min(q0 + s, n - 1);
+      const uint32_t ro = (uint32_t)pos * (uint32_t)chunks * 16u;  // row's byte offset
       st.pos = pos;
-      st.j = ord[pos];
-      st.dj = pdg[pos];
-      st.c0 = pcol[(size_t)pos * chunks];
-      st.v0 = pval[(size_t)pos * chunks];
+      st.j = *reinterpret_cast<const int32_t *>(at16(ord, (uint32_t)pos * 4u));
+      st.dj = *reinterpret_cast<const float *>(at16(pdg, (uint32_t)pos * 4u));
+      st.c0 = *reinterpret_cast<const int4 *>(at16(pcol, ro));
+      st.v0 = *reinterpret_cast<const float4 *>(at16(pval, ro));
       st.c1 = make_int4(n, n, n, n);
       st.v1 = make_float4(0.f, 0.f, 0.f, 0.f);
       if (chunks > 1) {
-        st.c1 = pcol[(size_t)pos * chunks + 1];
-        st.v1 = pval[(size_t)pos * chunks + 1];
+        st.c1 = *reinterpret_cast<const int4 *>(at16(pcol, ro + 16u));
+        st.v1 = *reinterpret_cast<const float4 *>(at16(pval, ro + 16u));
       }
       return st;
     };
-    Step cur = fetch(p0);
-    for (;;) {
+    // one step: decides the sites of `cur` after requesting the next
+    // step's rows into `nxt`; false after the sweep's last step.  Two
+    // buffers in turn (no copies between steps).
+    auto step = [&](const Step &cur, Step &nxt) -> bool {
       int np0 = p0 + G, nc = c, nc1 = c1;
       if (np0 >= c1) {
         nc = c + 1;
@@ -439,7 +457,6 @@ __device__ __forceinline__ void col_item(
         }
       }
       const bool more = nc < ncol;
-      Step nxt = cur;
       if (more) nxt = fetch(np0);
       if (p0 + G > win + 4 * G) {
         // next window: lane (rr, s) draws positions win + 4s .. +3
@@ -454,24 +471,28 @@ __device__ __forceinline__ void col_item(
       const int pos = p0 + s;
       const bool active = pos < c1;
       const float thr = thw[(pos - win) * RW + rr];
+      // h + w for a set neighbour bit, h for a clear one: fma(1, w, h) is the
+      // rounded sum, fma(0, w, h) is h (up to the sign of a zero h, which no
+      // comparison or later sum sees)
       float h = cur.dj;
-      h = bit(cur.c0.x) ? h + cur.v0.x : h;
-      h = bit(cur.c0.y) ? h + cur.v0.y : h;
-      h = bit(cur.c0.z) ? h + cur.v0.z : h;
-      h = bit(cur.c0.w) ? h + cur.v0.w : h;
+      h = fmaf((float)bit(cur.c0.x), cur.v0.x, h);
+      h = fmaf((float)bit(cur.c0.y), cur.v0.y, h);
+      h = fmaf((float)bit(cur.c0.z), cur.v0.z, h);
+      h = fmaf((float)bit(cur.c0.w), cur.v0.w, h);
       if (chunks > 1) {
-        h = bit(cur.c1.x) ? h + cur.v1.x : h;
-        h = bit(cur.c1.y) ? h + cur.v1.y : h;
-        h = bit(cur.c1.z) ? h + cur.v1.z : h;
-        h = bit(cur.c1.w) ? h + cur.v1.w : h;
+        h = fmaf((float)bit(cur.c1.x), cur.v1.x, h);
+        h = fmaf((float)bit(cur.c1.y), cur.v1.y, h);
+        h = fmaf((float)bit(cur.c1.z), cur.v1.z, h);
+        h = fmaf((float)bit(cur.c1.w), cur.v1.w, h);
       }
       for (int cc = 2; cc < chunks; ++cc) {
-        const int4 ci = pcol[(size_t)cur.pos * chunks + cc];
-        const float4 vi = pval[(size_t)cur.pos * chunks + cc];
-        h = bit(ci.x) ? h + vi.x : h;
-        h = bit(ci.y) ? h + vi.y : h;
-        h = bit(ci.z) ? h + vi.z : h;
-        h = bit(ci.w) ? h + vi.w : h;
+        const uint32_t ro = ((uint32_t)cur.pos * (uint32_t)chunks + (uint32_t)cc) * 16u;
+        const int4 ci = *reinterpret_cast<const int4 *>(at16(pcol, ro));
+        const float4 vi = *reinterpret_cast<const float4 *>(at16(pval, ro));
+        h = fmaf((float)bit(ci.x), vi.x, h);
+        h = fmaf((float)bit(ci.y), vi.y, h);
+        h = fmaf((float)bit(ci.z), vi.z, h);
+        h = fmaf((float)bit(ci.w), vi.w, h);
       }
       const int j = cur.j;
       const float de = bit(j) ? -h : h;
@@ -479,13 +500,16 @@ __device__ __forceinline__ void col_item(
         atomicXor(&xs[(j >> 5) * RW + rr], 1u << (j & 31));
         de_sum += (double)de;
       }
-      if (!more) break;
+      if (!more) return false;
       // a new class reads this class's flips
       if (nc != c) wave_sync();
-      cur = nxt;
       p0 = np0;
       c1 = nc1;
       c = nc;
+      return true;
+    };
+    Step sa = fetch(p0), sb = sa;
+    while (step(sa, sb) && step(sb, sa)) {
     }
     wave_sync();
 #pragma unroll

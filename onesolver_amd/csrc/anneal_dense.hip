@@ -624,6 +624,53 @@ void launch_dense_gather(hipStream_t st, int num_instances, int Rp, int R, int n
   OS_HIP_CHECK(hipGetLastError());
 }
 
+// Replica exchange round `round` on the GPU, for the dense and the sparse
+// paths (replica r of instance i at row i * Rp + r): thread = (instance,
+// ladder, pair p); slots k = round % 2 + 2p and k + 1 of the ladder swap
+// their replicas a, b iff d = (1/ts_k - 1/ts_{k+1}) (E_a - E_b) >= 0 or
+// u < exp(d), u = (w >> 8) 2^-24 from word 0 of Philox counter (k, l, round,
+// ((instance_base + i) << 8) | TAG_EXCHANGE); the swapped replicas take their
+// new slots' scales in tsr.  The pairs of a round are disjoint.
+__global__ __launch_bounds__(64) void exchange_round_kernel(
+    int I, int L, int K, int R, int Rp, uint32_t round, uint32_t k0, uint32_t k1,
+    uint32_t inst_base, const float *__restrict__ ts_slots, const double *__restrict__ e,
+    uint32_t *__restrict__ rep_at, uint32_t *__restrict__ slot_of, float *__restrict__ tsr) {
+  const int pairs = K / 2;
+  const int idx = blockIdx.x * 64 + threadIdx.x;
+  if (idx >= I * L * pairs) return;
+  const int i = idx / (L * pairs), l = (idx / pairs) % L, p = idx % pairs;
+  const int k = (int)(round & 1u) + 2 * p;
+  if (k + 1 >= K) return;
+  const size_t ro = (size_t)i * Rp;
+  uint32_t *ra = rep_at + (size_t)i * R + (size_t)l * K;
+  const uint32_t a = ra[k], b = ra[k + 1];
+  const double d =
+      (1.0 / (double)ts_slots[k] - 1.0 / (double)ts_slots[k + 1]) * (e[ro + a] - e[ro + b]);
+  const uint4 o = philox4x32_10(
+      make_uint4((uint32_t)k, (uint32_t)l, round, ((inst_base + (uint32_t)i) << 8) | TAG_EXCHANGE),
+      k0, k1);
+  const double u = (double)(o.x >> 8) * 0x1p-24;
+  if (d >= 0.0 || u < exp(d)) {
+    ra[k] = b;
+    ra[k + 1] = a;
+    slot_of[(size_t)i * R + a] = (uint32_t)(k + 1);
+    slot_of[(size_t)i * R + b] = (uint32_t)k;
+    tsr[ro + a] = ts_slots[k + 1];
+    tsr[ro + b] = ts_slots[k];
+  }
+}
+
+void launch_exchange_round(hipStream_t st, int I, int L, int K, int R, int Rp, uint32_t round,
+                           uint64_t seed, uint32_t inst_base, const float *ts_slots,
+                           const double *e, uint32_t *rep_at, uint32_t *slot_of, float *tsr) {
+  const int threads = I * L * (K / 2);
+  if (threads == 0) return;
+  exchange_round_kernel<<<(threads + 63) / 64, 64, 0, st>>>(
+      I, L, K, R, Rp, round, (uint32_t)seed, (uint32_t)(seed >> 32), inst_base, ts_slots, e,
+      rep_at, slot_of, tsr);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
 int dense_max_n() { return kWaves * 32 * 8; }
 
 // Column tiles (32 columns) per wave are instantiated for these counts; n is

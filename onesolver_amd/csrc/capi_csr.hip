@@ -49,6 +49,8 @@ struct CsrBatch {
   // replica exchange: per-replica temperature scales ([I][Rp], read when
   // CsrRun::use_tsr is set)
   DevBuf tsr;
+  // replica exchange rounds on the GPU: slots, replicas at slots, scales
+  DevBuf pt_slots;
   int num_items = 0;
   // launch shape chosen by the tuner (0: not tuned); reg: row-prefetching kernel
   int waves = 0, groups = 0;
@@ -84,7 +86,7 @@ struct CsrBatch {
     for (auto *b : {&d_inst, &ell_col, &ell_val, &diag, &diag64, &row_ptr, &col, &val64, &items,
                     &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &corder,
                     &cstart, &items_c, &pell_col, &pell_val, &pdiag, &pop_best, &pop_eb,
-                    &pop_anc, &tsr})
+                    &pop_anc, &tsr, &pt_slots})
       b->release();
   }
 
@@ -827,15 +829,23 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
       rep_at[i] = (uint32_t)(i % R);
     }
     std::vector<float> tsr_h((size_t)b.I * Rp, p->tscale[0]);
-    auto upload_tsr = [&]() {
-      for (int i = 0; i < b.I; ++i)
-        for (int r = 0; r < R; ++r)
-          tsr_h[(size_t)b.inst[i].rep_off + r] = p->tscale[slot_of[(size_t)i * R + r]];
-      OS_HIP_CHECK(hipMemcpyAsync(b.tsr.ptr, tsr_h.data(), sizeof(float) * tsr_h.size(),
-                                  hipMemcpyHostToDevice, s));
-    };
-    upload_tsr();
-    std::vector<double> ef((size_t)b.I * Rp);
+    for (int i = 0; i < b.I; ++i)
+      for (int r = 0; r < R; ++r)
+        tsr_h[(size_t)b.inst[i].rep_off + r] = p->tscale[slot_of[(size_t)i * R + r]];
+    // the exchange rounds run on the GPU between segments: slots, replicas
+    // at slots, and the slots' scales in device memory
+    b.pt_slots.ensure(sizeof(uint32_t) * slot_of.size() * 2 + sizeof(float) * K);
+    uint32_t *d_slot = b.pt_slots.as<uint32_t>();
+    uint32_t *d_rep = d_slot + slot_of.size();
+    float *d_ts = reinterpret_cast<float *>(d_rep + rep_at.size());
+    OS_HIP_CHECK(hipMemcpyAsync(d_slot, slot_of.data(), sizeof(uint32_t) * slot_of.size(),
+                                hipMemcpyHostToDevice, s));
+    OS_HIP_CHECK(hipMemcpyAsync(d_rep, rep_at.data(), sizeof(uint32_t) * rep_at.size(),
+                                hipMemcpyHostToDevice, s));
+    OS_HIP_CHECK(hipMemcpyAsync(d_ts, p->tscale, sizeof(float) * K, hipMemcpyHostToDevice, s));
+    OS_HIP_CHECK(hipMemcpyAsync(b.tsr.ptr, tsr_h.data(), sizeof(float) * tsr_h.size(),
+                                hipMemcpyHostToDevice, s));
+    OS_HIP_CHECK(hipStreamSynchronize(s));
     uint32_t round = 0;
     int segments = 0;
     OS_HIP_CHECK(hipEventRecord(c.ev_start, s));
@@ -855,34 +865,11 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
                               s0 == 0, b.eb.as<double>(), b.best.as<uint32_t>(),
                               b.pop_eb.as<double>(), b.pop_best.as<uint32_t>());
       if (s0 + ns >= S) break;
-      // exchange round on the running energies at the segment's end
-      OS_HIP_CHECK(hipMemcpyAsync(ef.data(), b.ef.ptr, sizeof(double) * ef.size(),
-                                  hipMemcpyDeviceToHost, s));
-      OS_HIP_CHECK(hipStreamSynchronize(s));
-      for (int i = 0; i < b.I; ++i) {
-        const uint32_t tag = ((p->instance_base + (uint32_t)i) << 8) | os::TAG_EXCHANGE;
-        const double *e = ef.data() + b.inst[i].rep_off;
-        for (int l = 0; l < L; ++l) {
-          uint32_t *ra = rep_at.data() + (size_t)i * R + (size_t)l * K;
-          for (int k = (int)(round & 1u); k + 1 < K; k += 2) {
-            const uint32_t a = ra[k], bb = ra[k + 1];
-            const double d =
-                (1.0 / (double)p->tscale[k] - 1.0 / (double)p->tscale[k + 1]) * (e[a] - e[bb]);
-            const uint4 o = os::philox4x32_10(
-                make_uint4((uint32_t)k, (uint32_t)l, round, tag), run.k0, run.k1);
-            const double u = (double)(o.x >> 8) * 0x1p-24;
-            if (d >= 0.0 || u < std::exp(d)) {
-              ra[k] = bb;
-              ra[k + 1] = a;
-              slot_of[(size_t)i * R + a] = (uint32_t)(k + 1);
-              slot_of[(size_t)i * R + bb] = (uint32_t)k;
-            }
-          }
-        }
-      }
-      ++round;
-      upload_tsr();
+      // exchange round on the running energies at the segment's end, and
       // the next segment continues from the final states
+      os::launch_exchange_round(s, b.I, L, K, R, Rp, round, p->seed, p->instance_base, d_ts,
+                                b.ef.as<double>(), d_rep, d_slot, b.tsr.as<float>());
+      ++round;
       OS_HIP_CHECK(hipMemcpyAsync(b.init.ptr, b.fin.ptr, planes, hipMemcpyDeviceToDevice, s));
     }
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
@@ -891,7 +878,11 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
                           b.diag64.as<double>(), b.pop_best.as<uint32_t>(), b.e64.as<double>());
     os::collect(b, R, b.pop_best, b.fin, nullptr, best_energies, nullptr, best_states,
                 final_states, inst_best_energy, inst_best_replica);
-    if (final_slots) std::memcpy(final_slots, slot_of.data(), sizeof(uint32_t) * slot_of.size());
+    if (final_slots) {
+      OS_HIP_CHECK(hipMemcpyAsync(final_slots, d_slot, sizeof(uint32_t) * slot_of.size(),
+                                  hipMemcpyDeviceToHost, s));
+      OS_HIP_CHECK(hipStreamSynchronize(s));
+    }
     float ms = 0.f;
     OS_HIP_CHECK(hipEventElapsedTime(&ms, c.ev_start, c.ev_stop));
     if (stats) {

@@ -239,6 +239,16 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       OS_HIP_CHECK(hipMemcpyAsync(d.tsr.ptr, tsr_h.data(), sizeof(float) * T,
                                   hipMemcpyHostToDevice, c.stream));
     };
+    // the exchange rounds run on the GPU: slots, replicas at slots and the
+    // slots' scales in device memory (uploaded with the first scales)
+    uint32_t *d_slot = nullptr, *d_rep = nullptr;
+    float *d_ts = nullptr;
+    if (xc) {
+      d.pt_slots.ensure(sizeof(uint32_t) * slot_of.size() * 2 + sizeof(float) * K);
+      d_slot = d.pt_slots.as<uint32_t>();
+      d_rep = d_slot + slot_of.size();
+      d_ts = reinterpret_cast<float *>(d_rep + rep_at.size());
+    }
     d.fields.ensure(sizeof(float) * T * np * (d.stream ? 2 : 1));
     d.xbits.ensure(sizeof(uint32_t) * T * npw);
     d.bestbits.ensure(sizeof(uint32_t) * T * npw);
@@ -275,8 +285,17 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
     if (S > 0 && !xc)
       OS_HIP_CHECK(hipMemcpyAsync(d.tscale.ptr, p->tscale, sizeof(float) * S,
                                   hipMemcpyHostToDevice, st));
-    if (xc) upload_tsr();
-    std::vector<double> e_now(xc || pa ? T : 0);
+    if (xc) {
+      upload_tsr();
+      OS_HIP_CHECK(hipMemcpyAsync(d_slot, slot_of.data(), sizeof(uint32_t) * slot_of.size(),
+                                  hipMemcpyHostToDevice, st));
+      OS_HIP_CHECK(hipMemcpyAsync(d_rep, rep_at.data(), sizeof(uint32_t) * rep_at.size(),
+                                  hipMemcpyHostToDevice, st));
+      OS_HIP_CHECK(hipMemcpyAsync(d_ts, xc->tscale, sizeof(float) * K, hipMemcpyHostToDevice,
+                                  st));
+      c.sync();
+    }
+    std::vector<double> e_now(pa ? T : 0);
     std::vector<uint32_t> anc_h(pa ? (size_t)I * R : 0);
     std::vector<double> wsum(pa ? R : 0);
     if (pa) {
@@ -451,35 +470,10 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       }
       if (xc && s0 + ns < S) {
         // exchange round on the running energies of the launch (or of the
-        // refresh), then the scales of the next segment
-        OS_HIP_CHECK(hipMemcpyAsync(e_now.data(), d.energy.ptr, sizeof(double) * T,
-                                    hipMemcpyDeviceToHost, st));
-        c.sync();
-        const uint32_t k0 = (uint32_t)p->seed, k1 = (uint32_t)(p->seed >> 32);
-        for (int s = 0; s < I; ++s) {
-          const uint32_t tag = ((p->instance + (uint32_t)s) << 8) | os::TAG_EXCHANGE;
-          for (int l = 0; l < L; ++l) {
-            uint32_t *ra = rep_at.data() + (size_t)s * R + (size_t)l * K;
-            for (int k = round & 1; k + 1 < K; k += 2) {
-              const uint32_t a = ra[k], b = ra[k + 1];  // replicas of the instance
-              const double ea = e_now[(size_t)s * Rp + a];
-              const double eb = e_now[(size_t)s * Rp + b];
-              const double delta = (1.0 / (double)xc->tscale[k] - 1.0 / (double)xc->tscale[k + 1]) *
-                                   (ea - eb);
-              const uint4 o = os::philox4x32_10(
-                  make_uint4((uint32_t)k, (uint32_t)l, (uint32_t)round, tag), k0, k1);
-              const double u = (double)(o.x >> 8) * 0x1p-24;
-              if (delta >= 0.0 || u < std::exp(delta)) {
-                ra[k] = b;
-                ra[k + 1] = a;
-                slot_of[(size_t)s * R + a] = (uint32_t)(k + 1);
-                slot_of[(size_t)s * R + b] = (uint32_t)k;
-              }
-            }
-          }
-        }
+        // refresh); the next launch reads the new scales
+        os::launch_exchange_round(st, I, L, K, R, Rp, (uint32_t)round, p->seed, p->instance,
+                                  d_ts, d.energy.as<double>(), d_rep, d_slot, d.tsr.as<float>());
         ++round;
-        upload_tsr();
       }
     }
     // FP64 energies of the best states (the returned energies)
@@ -548,7 +542,9 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       stats->pass_avg_ms = timed > 0 ? pms / timed : 0.0;
       stats->pass_launches = passes;
     }
-    if (xc && xc->final_slots) std::memcpy(xc->final_slots, slot_of.data(), sizeof(uint32_t) * I * R);
+    if (xc && xc->final_slots)
+      OS_HIP_CHECK(hipMemcpy(xc->final_slots, d_slot, sizeof(uint32_t) * I * R,
+                             hipMemcpyDeviceToHost));
   });
 }
 

@@ -23,6 +23,8 @@ struct Dense {
   DevBuf tsr;  // replica exchange: per-row temperature scales
   // population annealing: the gathered rows of states, fields and energies
   DevBuf pa_x, pa_f, pa_e, pa_anc;
+  // replica exchange rounds on the GPU: slots, replicas at slots, scales
+  DevBuf pt_slots;
   // stream path: flip masks of the last 3 * group blocks (ring), and the
   // decide launches on `side`, beside the field passes on the context's
   // stream; ev_dec marks the last decide of a group, ev_pass[G % 2] pass G
@@ -38,7 +40,7 @@ struct Dense {
   ~Dense() {
     for (auto *b : {&q64, &qpack, &qdb, &fields, &xbits, &bestbits, &energy, &ebest, &tscale,
                     &scratch, &drift, &bytes, &marg, &status, &dblk, &tsr, &pa_x, &pa_f,
-                    &pa_e, &pa_anc})
+                    &pa_e, &pa_anc, &pt_slots})
       b->release();
     for (hipEvent_t e : {ev_seg, ev_dec, ev_pass[0], ev_pass[1]})
       if (e) (void)hipEventDestroy(e);

@@ -169,6 +169,12 @@ void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, i
 void launch_csr_pop_best(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                          int Rp, bool first, const double *eb, const uint32_t *best,
                          double *pop_eb, uint32_t *pop_best);
+// replica exchange round on the GPU (exchange_round_kernel, dense and
+// sparse paths): rep_at / slot_of per instance [R], ts_slots [K], e and tsr
+// indexed by row i * Rp + r
+void launch_exchange_round(hipStream_t st, int I, int L, int K, int R, int Rp, uint32_t round,
+                           uint64_t seed, uint32_t inst_base, const float *ts_slots,
+                           const double *e, uint32_t *rep_at, uint32_t *slot_of, float *tsr);
 void launch_csr_pop_gather(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                            int Rp, int R, const uint32_t *anc, const uint32_t *fin,
                            uint32_t *init);

@@ -57,7 +57,8 @@ def _sparse_text(n, d, seed):
     return "p qubo 0 %d %d %d\n" % (n, n, len(edges)) + "".join(lines)
 
 
-def test_batch_cli_sparse_instances_match_oracle(ctx, tmp_path):
+@pytest.mark.parametrize("order", ["sequential", "colour"])
+def test_batch_cli_sparse_instances_match_oracle(ctx, tmp_path, order):
     sizes = [(20, 3), (64, 4), (97, 5), (150, 3), (33, 6)]
     texts = [_sparse_text(n, d, 10 + k) for k, (n, d) in enumerate(sizes)]
     inp = tmp_path / "in"
@@ -69,16 +70,17 @@ def test_batch_cli_sparse_instances_match_oracle(ctx, tmp_path):
     with onesolver_amd.Qubo(ctx, texts[-1]) as q:      # one instance as a binary file
         (inp / ("inst%02d.osqb" % (len(texts) - 1))).write_bytes(q.to_binary())
     S, R, seed = 40, 64, 7
-    print("[gpu] one-solver-batch sparse", flush=True)
+    print(f"[gpu] one-solver-batch sparse, {order} order", flush=True)
     r = _run(["--input", str(inp), "--output", str(out), "--num-iter", str(S), "--num-tries",
-              str(R), "--seed", str(seed), "--beta-max", "3"])
+              str(R), "--seed", str(seed), "--beta-max", "3", "--site-order", order])
     assert r.returncode == 0, r.stderr
-    assert "Path: sparse" in r.stdout
+    assert f"Path: sparse ({'colour-class' if order == 'colour' else 'sequential'} order)" in r.stdout
     ts = oracle.geometric_schedule(0.1, 3.0, S).astype(np.float32)   # beta as a temperature
     for k, t in enumerate(texts):
         _, _, res = oracle.qubo_parse(t.encode())
         rp, cl, vl, dg = _csr(res)
-        _, bs, _ = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=seed, instance=k)
+        od = oracle.csr_colour_order(rp, cl)[0] if order == "colour" else None
+        _, bs, _ = oracle.sweep_csr(rp, cl, vl, dg, ts, R, seed=seed, instance=k, order=od)
         e64 = oracle.csr_energy(rp, cl, vl, dg, bs)
         b = int(np.argmin(e64))
         state, energy = _read_csv(out / ("inst%02d.csv" % k))
