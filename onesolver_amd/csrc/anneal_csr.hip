@@ -61,8 +61,11 @@ __device__ __forceinline__ float row_field(const uint32_t *xs, int lane, const i
 
 }  // namespace
 
+// Work-queue counters: queue[0] for the sequential kernels, queue[0..7] (one
+// per XCD) for the colour-class kernel, whose item lists start at queue[8+x]
+// and hold queue[16+x] items (written by the host, kept across resets).
 __global__ void csr_queue_reset_kernel(int *queue) {
-  if (threadIdx.x == 0) queue[0] = 0;
+  if (threadIdx.x < 8) queue[threadIdx.x] = 0;
 }
 
 template <int kW>
@@ -556,11 +559,31 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
   const int init_mode = run->init_mode;
   const int tb = run->sweep_base;
   const int use_tsr = run->use_tsr;
+  (void)num_items;
+  // XCD-aware queue: work-groups are dealt to the 8 XCDs round robin, and the
+  // host gave each XCD's list whole instances, so an instance's rows stay in
+  // one XCD's L2.  A wave takes items from its XCD's list, then from the
+  // others in turn; it leaves once every list is exhausted (each list is
+  // tried once more after it ran out, so every wave exits).
+  const int x0 = blockIdx.x & 7;
+  uint32_t done = 0u;  // lists this wave found exhausted (lane 0)
   for (;;) {
-    int it = 0;
-    if (lane == 0) it = atomicAdd(queue, 1);
+    int it = -1;
+    if (lane == 0) {
+      for (int q = 0; q < 8; ++q) {
+        const int x = (x0 + q) & 7;
+        if (done & (1u << x)) continue;
+        const int cnt = queue[16 + x];
+        const int v = cnt > 0 ? atomicAdd(&queue[x], 1) : cnt;
+        if (v < cnt) {
+          it = queue[8 + x] + v;
+          break;
+        }
+        done |= 1u << x;
+      }
+    }
     it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
-    if (it >= num_items) break;
+    if (it < 0) break;
     const int2 wi = items[it];
     const CsrInst d = inst[wi.x];
     const int g = __builtin_amdgcn_readfirstlane(d.sites);

@@ -193,10 +193,29 @@ void col_items(CsrBatch &b, int uniform, int limit) {
     plane = std::max(plane, d.words * (64 / d.sites));
   }
   std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return est[a] > est[c]; });
+  // whole instances to the 8 XCD lists, largest total work first onto the
+  // least loaded list; within a list, the longest items first
+  std::vector<int> by_total(order);
+  auto groups_of = [&](int i) { return b.Rp / (64 / b.inst[i].sites); };
+  std::stable_sort(by_total.begin(), by_total.end(), [&](int a, int c) {
+    return est[a] * groups_of(a) > est[c] * groups_of(c);
+  });
+  int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<int> xcd_of(b.I);
+  for (int i : by_total) {
+    const int x = (int)(std::min_element(load, load + 8) - load);
+    xcd_of[i] = x;
+    load[x] += est[i] * groups_of(i);
+  }
   std::vector<int2> items;
-  for (int i : order) {
-    const int groups = b.Rp / (64 / b.inst[i].sites);
-    for (int gidx = 0; gidx < groups; ++gidx) items.push_back(make_int2(i, gidx));
+  int meta[16];
+  for (int x = 0; x < 8; ++x) {
+    meta[x] = (int)items.size();
+    for (int i : order) {
+      if (xcd_of[i] != x) continue;
+      for (int gidx = 0; gidx < groups_of(i); ++gidx) items.push_back(make_int2(i, gidx));
+    }
+    meta[8 + x] = (int)items.size() - meta[x];
   }
   hipStream_t s = b.ctx->stream;
   OS_HIP_CHECK(hipMemcpyAsync(b.d_inst.ptr, b.inst.data(), sizeof(CsrInst) * b.I,
@@ -204,6 +223,9 @@ void col_items(CsrBatch &b, int uniform, int limit) {
   b.items_c.ensure(sizeof(int2) * std::max<size_t>(items.size(), 1));
   OS_HIP_CHECK(hipMemcpyAsync(b.items_c.ptr, items.data(), sizeof(int2) * items.size(),
                               hipMemcpyHostToDevice, s));
+  // list starts and lengths behind the 8 counters (queue[8..23])
+  OS_HIP_CHECK(hipMemcpyAsync(b.queue.as<int>() + 8, meta, sizeof(meta), hipMemcpyHostToDevice,
+                              s));
   OS_HIP_CHECK(hipStreamSynchronize(s));
   b.num_items_c = (int)items.size();
   b.c_plane_words = plane;
