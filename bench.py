@@ -42,12 +42,12 @@ PEAK_FP64_TFLOPS = 78.6    # MI355X FP64 vector
 # bytes = WRITE_SIZE).  Only reported when the run uses that configuration.
 TRAFFIC_DENSE_512 = (2 * 2967756.3 + 252689.6) * 1024   # dense_sweep_kernel, 512 sweeps
 TRAFFIC_CSR_C3 = (2 * 9356018.9 + 7374460.7) * 1024     # csr_sweep_pf_kernel, 2048 sweeps
-TRAFFIC_CSR_C3_COLOUR = (2 * 63019529.1 + 6022566.6) * 1024  # csr_sweep_col_kernel, 2048 sweeps
+TRAFFIC_CSR_C3_COLOUR = (2 * 20880.6 + 4838077.6) * 1024  # csr_sweep_col_kernel, 2048 sweeps
 # csr_sweep_col_kernel, one 2048-sweep launch of the config-3 batch (PMC passes): L2 requests and
 # hits, LDS instructions (wave level), LDS bank-conflict and active cycles, VALU instructions
-COUNTERS_CSR_C3_COLOUR = {"tcc_req": 6854772925.0, "tcc_hit": 5868742601.0,
+COUNTERS_CSR_C3_COLOUR = {"tcc_req": 6640911323.0, "tcc_hit": 6640351949.0,
                           "sq_insts_lds": 25849097501.0, "sq_lds_bank_conflict": 88772708309.0,
-                          "sq_lds_idx_active": 143560519727.0, "sq_insts_valu": 274165825036.0}
+                          "sq_lds_idx_active": 143560519727.0, "sq_insts_valu": 274166665953.0}
 TRAFFIC_PASS_C5 = (2 * 107418.5 + 131072.0) * 1024      # stream_pass_kernel, 5 blocks
 TRAFFIC_INGEST_8192 = (2 * 643639.7 + 558661.2) * 1024   # parse_lines_kernel<true>, N=8192 text
 

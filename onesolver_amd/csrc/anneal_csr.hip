@@ -218,9 +218,9 @@ __device__ __forceinline__ SiteRow load_row(const int4 *ecol, const float4 *eval
 
 // Prefetching variant for batches whose ELL rows have at most 8 entries
 // (1 or 2 chunks): the same kernel, but the row, weights and diagonal of site
-// j+1 (uniform, scalar loads) are requested before site j is decided, so the
-// per-site chain no longer waits for a row load.  Same arithmetic, order and
-// results as csr_sweep_kernel.
+// j+1 (uniform values, vector loads: load_row) are requested before site j is
+// decided, so the per-site chain no longer waits for a row load.  Same
+// arithmetic, order and results as csr_sweep_kernel.
 template <int kW>
 __global__ __launch_bounds__(64 * kW) void csr_sweep_pf_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items,
