@@ -604,34 +604,72 @@ __global__ __launch_bounds__(64 * kW) void csr_sweep_col_kernel(
 
 // FP64 energy of states given as planes: E = sum_j x_j (Q_jj + sum_{l>j} Q_jl x_l),
 // the reference's energy function (upper triangle incl. diagonal), with the
-// double-precision couplings of the CSR input.  Thread = (item, lane).
-__global__ __launch_bounds__(64) void csr_energy_kernel(
+// double-precision couplings of the CSR input.  Work-group = item (64
+// replicas of one instance, lane = replica): its state planes are staged in
+// LDS (words x 64, dynamic), then wave w sums rows
+// [n w / kEnergyWaves, n (w+1) / kEnergyWaves) in ascending order, and the
+// partial sums are added in wave order, so the result is the same on every
+// run.  The neighbour loop takes four entries at a time, so four bit reads
+// are in flight per lane.
+constexpr int kEnergyWaves = 8;
+
+__global__ __launch_bounds__(64 * kEnergyWaves) void csr_energy_kernel(
     const CsrInst *__restrict__ inst, const int2 *__restrict__ items, int num_items, int Rp,
     const int32_t *__restrict__ row_ptr, const int32_t *__restrict__ col,
     const double *__restrict__ val, const double *__restrict__ diag64,
     const uint32_t *__restrict__ bits, double *__restrict__ out) {
-  const int it = blockIdx.x;
-  if (it >= num_items) return;
-  const int lane = threadIdx.x;
+  __shared__ double part[kEnergyWaves][64];
+  extern __shared__ uint32_t eplanes[];  // [words][64]
+  const int it = blockIdx.x;  // < num_items: the grid has one work-group per item
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
   const int2 wi = items[it];
   const CsrInst d = inst[wi.x];
   const uint32_t r = (uint32_t)(wi.y * 64 + lane);
-  const uint32_t *x = bits + d.state_off + r;
+  {
+    const uint32_t *x = bits + d.state_off + (uint32_t)wi.y * 64;
+    for (int i = threadIdx.x; i < d.words * 64; i += 64 * kEnergyWaves)
+      eplanes[i] = x[(size_t)(i >> 6) * Rp + (i & 63)];
+  }
+  __syncthreads();
   const int32_t *rp = row_ptr + d.row_off;
   const int32_t *cl = col + d.csr_off;
   const double *vl = val + d.csr_off;
   const double *dg = diag64 + d.spin_off;
+  const int j0 = (int)((int64_t)d.n * w / kEnergyWaves);
+  const int j1 = (int)((int64_t)d.n * (w + 1) / kEnergyWaves);
+  auto bit = [&](int l) { return (eplanes[(l >> 5) * 64 + lane] >> (l & 31)) & 1u; };
   double e = 0.0;
-  for (int j = 0; j < d.n; ++j) {
-    if (!((x[(size_t)(j >> 5) * Rp] >> (j & 31)) & 1u)) continue;
+  for (int j = j0; j < j1; ++j) {
+    // s gains +0.0 for a neighbour that is not set or not above j, and e
+    // gains +0.0 for an unset spin: the same sums as skipping them
     double s = dg[j];
-    for (int k = rp[j]; k < rp[j + 1]; ++k) {
-      const int l = cl[k];
-      if (l > j && ((x[(size_t)(l >> 5) * Rp] >> (l & 31)) & 1u)) s += vl[k];
+    int k = rp[j];
+    const int k1 = rp[j + 1];
+    for (; k + 4 <= k1; k += 4) {
+      int l[4];
+      uint32_t b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) l[q] = cl[k + q];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = l[q] > j ? bit(l[q]) : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s += b[q] ? vl[k + q] : 0.0;
     }
-    e += s;
+    for (; k < k1; ++k) {
+      const int l = cl[k];
+      s += (l > j && bit(l)) ? vl[k] : 0.0;
+    }
+    e += bit(j) ? s : 0.0;
   }
-  out[d.rep_off + r] = e;
+  part[w][lane] = e;
+  __syncthreads();
+  if (w == 0) {
+    double t = part[0][lane];
+#pragma unroll
+    for (int q = 1; q < kEnergyWaves; ++q) t += part[q][lane];
+    out[d.rep_off + r] = t;
+  }
 }
 
 // Population annealing: thread = (item, lane) = replica r of the item's
@@ -821,11 +859,14 @@ void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_word
 }
 
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
-                       int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
-                       const double *diag64, const uint32_t *bits, double *out) {
+                       int Rp, int max_words, const int32_t *row_ptr, const int32_t *col,
+                       const double *val, const double *diag64, const uint32_t *bits,
+                       double *out) {
   if (num_items == 0) return;
-  csr_energy_kernel<<<num_items, 64, 0, st>>>(inst, items, num_items, Rp, row_ptr, col, val,
-                                             diag64, bits, out);
+  const size_t lds = (size_t)max_words * 64 * sizeof(uint32_t);
+  csr_energy_kernel<<<num_items, 64 * kEnergyWaves, lds, st>>>(inst, items, num_items, Rp,
+                                                                row_ptr, col, val, diag64, bits,
+                                                                out);
   OS_HIP_CHECK(hipGetLastError());
 }
 

@@ -72,6 +72,7 @@ struct CsrBatch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int g_waves = 0, g_groups = 0, g_Rp = 0, g_mode = 0, g_uniform = 0, g_limit = 0;
+  bool g_energy = false;  // the graph ends with the FP64 energies of the best states
   bool g_reg = false;
   const void *g_tscale = nullptr, *g_items_c = nullptr;
 
@@ -111,7 +112,7 @@ struct CsrBatch {
                         init.as<uint32_t>(), best.as<uint32_t>(), fin.as<uint32_t>(),
                         eb.as<double>(), ef.as<double>(), queue.as<int>());
     if (energy)
-      launch_csr_energy(st, d_inst.as<CsrInst>(), items.as<int2>(), num_items, Rp,
+      launch_csr_energy(st, d_inst.as<CsrInst>(), items.as<int2>(), num_items, Rp, max_words,
                         row_ptr.as<int32_t>(), col.as<int32_t>(), val64.as<double>(),
                         diag64.as<double>(), best.as<uint32_t>(), e64.as<double>());
   }
@@ -385,16 +386,16 @@ Shape select_shape(CsrBatch &b, const CsrRun &run, int site_order, bool tune_it)
 
 // Captures (reset, sweeps, FP64 energies of the best states) for `sh` unless
 // the graph already holds that shape.
-void ensure_graph(CsrBatch &b, const Shape &sh) {
+void ensure_graph(CsrBatch &b, const Shape &sh, bool energy) {
   hipStream_t s = b.ctx->stream;
   if (b.gexec && b.g_mode == b.mode && b.g_reg == sh.rg && b.g_waves == sh.waves &&
       b.g_groups == sh.groups && b.g_uniform == sh.uniform && b.g_limit == sh.limit &&
-      b.g_items_c == b.items_c.ptr && b.g_Rp == b.Rp)
+      b.g_items_c == b.items_c.ptr && b.g_Rp == b.Rp && b.g_energy == energy)
     return;
   b.drop_graph();
   OS_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   try {
-    b.enqueue(sh.rg, sh.waves, sh.groups, true);
+    b.enqueue(sh.rg, sh.waves, sh.groups, energy);
   } catch (...) {
     hipGraph_t g = nullptr;
     (void)hipStreamEndCapture(s, &g);
@@ -411,6 +412,7 @@ void ensure_graph(CsrBatch &b, const Shape &sh) {
   b.g_limit = sh.limit;
   b.g_items_c = b.items_c.ptr;
   b.g_Rp = b.Rp;
+  b.g_energy = energy;
   b.g_tscale = b.tscale.ptr;
 }
 
@@ -676,7 +678,7 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     }
     const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
     os::put_run(b, run);
-    if (p->use_graph) os::ensure_graph(b, sh);
+    if (p->use_graph) os::ensure_graph(b, sh, true);
     OS_HIP_CHECK(hipEventRecord(c.ev_start, s));
     if (p->use_graph)
       OS_HIP_CHECK(hipGraphLaunch(b.gexec, s));
@@ -739,7 +741,7 @@ int os_csr_batch_population(os_csr_batch *b_, const os_csr_population_params *p,
     run.k1 = (uint32_t)(p->seed >> 32);
     run.instance_base = p->instance_base;
     const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
-    if (p->use_graph) os::ensure_graph(b, sh);
+    if (p->use_graph) os::ensure_graph(b, sh, false);
     const size_t planes = sizeof(uint32_t) * (size_t)std::max<int64_t>(b.total_words * Rp, 1);
     b.pop_best.ensure(planes);
     b.pop_eb.ensure(sizeof(double) * (size_t)b.I * Rp);
@@ -783,7 +785,7 @@ int os_csr_batch_population(os_csr_batch *b_, const os_csr_population_params *p,
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
     // FP64 energies of the best states of every replica index
     os::launch_csr_energy(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
-                          b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
+                          b.max_words, b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
                           b.diag64.as<double>(), b.pop_best.as<uint32_t>(), b.e64.as<double>());
     os::collect(b, R, b.pop_best, b.fin, nullptr, best_energies, nullptr, best_states,
                 final_states, inst_best_energy, inst_best_replica);
@@ -839,7 +841,7 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
     run.instance_base = p->instance_base;
     run.num_sweeps = std::min(S, iv);
     const os::Shape sh = os::select_shape(b, run, p->site_order, p->tune != 0);
-    if (p->use_graph) os::ensure_graph(b, sh);
+    if (p->use_graph) os::ensure_graph(b, sh, false);
     const size_t planes = sizeof(uint32_t) * (size_t)std::max<int64_t>(b.total_words * Rp, 1);
     b.pop_best.ensure(planes);
     b.pop_eb.ensure(sizeof(double) * (size_t)b.I * Rp);
@@ -896,7 +898,7 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
     }
     OS_HIP_CHECK(hipEventRecord(c.ev_stop, s));
     os::launch_csr_energy(s, b.d_inst.as<os::CsrInst>(), b.items.as<int2>(), b.num_items, Rp,
-                          b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
+                          b.max_words, b.row_ptr.as<int32_t>(), b.col.as<int32_t>(), b.val64.as<double>(),
                           b.diag64.as<double>(), b.pop_best.as<uint32_t>(), b.e64.as<double>());
     os::collect(b, R, b.pop_best, b.fin, nullptr, best_energies, nullptr, best_states,
                 final_states, inst_best_energy, inst_best_replica);

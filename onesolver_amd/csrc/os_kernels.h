@@ -160,8 +160,9 @@ void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_word
                            const float *tsr, const uint32_t *init_bits, uint32_t *best_bits,
                            uint32_t *final_bits, double *e_best, double *e_final, int *queue);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
-                       int Rp, const int32_t *row_ptr, const int32_t *col, const double *val,
-                       const double *diag64, const uint32_t *bits, double *out);
+                       int Rp, int max_words, const int32_t *row_ptr, const int32_t *col,
+                       const double *val, const double *diag64, const uint32_t *bits,
+                       double *out);
 // population annealing over the 64-replica items: replica r < R of every
 // instance keeps the step's best planes and energy (eb, best) where they are
 // below its best so far (or all of them when `first`); the gather gives
