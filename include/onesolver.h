@@ -234,6 +234,18 @@ int os_dense_destroy(os_dense *dense);
 int os_dense_anneal(os_dense *dense, const os_sweep_params *params, double *best_energies,
                     uint8_t *best_states, uint8_t *final_states, uint8_t *best_state,
                     double *best_energy, os_sweep_stats *stats);
+/* Autotuner of the streaming path (n > 1024).  Times a probe run of
+ * min(num_sweeps, 4) sweeps (the params' replicas, schedule and seed, no
+ * outputs) for every candidate: blocks per field pass (3, 4, 5, 6, 8) x
+ * persistent pass work-groups per CU (1, 2), keeps the fastest on the handle
+ * for its later runs and returns it (best_ms: its summed sweep-kernel time).
+ * Neither knob changes results: every field is the same fp32 sum in the same
+ * block order.  The OS_STREAM_GROUP / OS_STREAM_PASS_WGS_PER_CU environment
+ * variables still take precedence.  For n <= 1024 nothing is tuned and the
+ * outputs are 0: the register-resident kernel fixes k = 16 and 64 replicas
+ * per work-group by its register layout.  No reference counterpart. */
+int os_dense_tune(os_dense *dense, const os_sweep_params *params, int32_t *group,
+                  int32_t *pass_wgs_per_cu, double *best_ms);
 /* Replica exchange on `dense` (see os_tempering_params).  Outputs as in
  * os_dense_anneal, indexed by replica; final_slots (optional, [I][R]) holds
  * the slot of every replica at the end. */

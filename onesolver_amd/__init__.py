@@ -211,6 +211,17 @@ class Context:
         return st, e.value
 
 
+def _dense_tune(h, num_replicas, tscale, seed, instance):
+    """os_dense_tune: the streaming path's blocks per field pass and pass
+    work-groups per CU, timed on probe runs (zeros for n <= 1024)."""
+    ts = np.ascontiguousarray(tscale, dtype=np.float32)
+    prm = SweepParams(int(num_replicas), len(ts), ts.ctypes.data, seed, instance, 0, None, None)
+    g, w, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_double(0.0)
+    _check(lib().os_dense_tune(h, ctypes.byref(prm), ctypes.byref(g), ctypes.byref(w),
+                               ctypes.byref(ms)))
+    return {"group": g.value, "pass_wgs_per_cu": w.value, "probe_ms": ms.value}
+
+
 class DenseProblem:
     """A dense QUBO resident on the GPU for sweep-mode annealing (os_dense)."""
 
@@ -254,6 +265,10 @@ class DenseProblem:
                                      ctypes.byref(st)))
         return {"best_energies": be, "best_states": bs, "final_states": fs, "best_state": best,
                 "best_energy": e.value, "stats": st.as_dict()}
+
+    def tune(self, num_replicas, tscale, seed=1, instance=0):
+        """Autotune the streaming path for these replicas (os_dense_tune)."""
+        return _dense_tune(self._h, num_replicas, tscale, seed, instance)
 
     def tempering(self, ladders, tscale, num_sweeps, exchange_interval, seed=1, instance=0,
                   refresh_interval=0, best_states=True, final_states=False):
@@ -397,6 +412,10 @@ class DenseBatch:
         [I, steps - 1, R]."""
         return _population(self._h, self.I, self.n, num_replicas, tscale, sweeps_per_step, seed,
                            instance, refresh_interval, best_states, final_states)
+
+    def tune(self, num_replicas, tscale, seed=1, instance=0):
+        """Autotune the streaming path for these replicas (os_dense_tune)."""
+        return _dense_tune(self._h, num_replicas, tscale, seed, instance)
 
     def anneal(self, num_replicas, tscale, seed=1, instance=0, refresh_interval=0,
                init_states=None, init_marginals=None, best_states=True, final_states=False):

@@ -91,6 +91,91 @@ __global__ __launch_bounds__(64) void ref_anneal_kernel(int n, const double *__r
   if (lane == 0) best_run[replica] = e_best;
 }
 
+// n <= 64: the same chain with the replica in registers.  Lane j holds h_j;
+// the state and best state are wave-uniform 64-bit masks; Q is staged in
+// LDS.  Per attempt, h_s is one readlane and row s of Q is requested before
+// the acceptance test, so the chain waits on no memory but that LDS read.
+// Every FP64 operation, and the order of the initial field and energy sums,
+// is the one of ref_anneal_kernel, so the chains are the same.
+constexpr int kRefSmallN = 64;
+
+__global__ __launch_bounds__(64) void ref_anneal_small_kernel(int n, const double *__restrict__ q,
+                                                              const double *__restrict__ beta,
+                                                              uint64_t total_attempts, int spb,
+                                                              uint32_t k0, uint32_t k1,
+                                                              uint32_t *__restrict__ best_bits,
+                                                              int W,
+                                                              double *__restrict__ best_run) {
+  extern __shared__ __align__(16) double qs[];  // n x n
+  const uint32_t replica = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n * n; i += 64) qs[i] = q[i];
+  const bool live = lane < n;
+  uint32_t b0 = 0;
+  {
+    const uint4 o = philox4x32_10(make_uint4(0u, replica, 0u, TAG_REF_INIT), k0, k1);
+    if (live) b0 = (pick(o, (lane >> 5) & 3) >> (lane & 31)) & 1u;
+  }
+  uint64_t x = __ballot(b0 != 0u);
+  uint64_t best = x;
+  __syncthreads();
+  const double qjj = live ? qs[lane * n + lane] : 0.0;
+  double h = qjj;
+  for (int l = 0; l < n; ++l)
+    if ((x >> l) & 1u) {
+      const double v = live ? qs[l * n + lane] : 0.0;
+      if (live && lane != l) h += v;
+    }
+  double part = 0.0;
+  if (live && ((x >> lane) & 1u)) part += 0.5 * (h + qjj);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  double e_cur = part;
+  double e_best = part;
+
+  // beta[t / spb] is kept in b and reloaded when the index moves; the draw of
+  // attempt t + 1 is computed during attempt t (it depends on no state)
+  int bi = 0, bc = 0;
+  double b = total_attempts > 0 ? beta[0] : 0.0;
+  uint4 o = philox4x32_10(make_uint4(0u, replica, 0u, TAG_REF_STEP), k0, k1);
+  for (uint64_t t = 0; t < total_attempts; ++t) {
+    const int s = __builtin_amdgcn_readfirstlane((int)__umulhi(o.x, (uint32_t)n));
+    const double u = (double)o.y * 0x1p-32;
+    const uint64_t t1 = t + 1;
+    o = philox4x32_10(make_uint4((uint32_t)t1, replica, (uint32_t)(t1 >> 32), TAG_REF_STEP), k0,
+                      k1);
+    const double qv = live ? qs[s * n + lane] : 0.0;
+    const uint64_t hb = (uint64_t)__double_as_longlong(h);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hb, s);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hb >> 32), s);
+    const double hs = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    const bool xs = (x >> s) & 1u;
+    const double de = xs ? -hs : hs;
+    const bool accept = (de < 0.0) || (exp(-de / b) > u);
+    if (accept) {
+      const double d = xs ? -1.0 : 1.0;
+      if (live && lane != s) h = fma(qv, d, h);
+      x ^= 1ull << s;
+      e_cur += de;
+    }
+    if (e_cur < e_best) {
+      e_best = e_cur;
+      best = x;
+    }
+    if (++bc == spb) {
+      bc = 0;
+      ++bi;
+      if (t1 < total_attempts) b = beta[bi];
+    }
+  }
+  if (lane == 0) {
+    uint32_t *dst = best_bits + (size_t)replica * W;
+    dst[0] = (uint32_t)best;
+    if (W > 1) dst[1] = (uint32_t)(best >> 32);
+    best_run[replica] = e_best;
+  }
+}
+
 int ref_lds_limit() { return 4096; }
 
 size_t ref_lds_bytes(int n) {
@@ -104,7 +189,10 @@ void launch_ref_anneal(hipStream_t st, int n, const double *q, const double *bet
   const uint64_t total = (uint64_t)num_iter * (uint64_t)spb;
   const int W = (n + 31) / 32;
   const size_t lds = (ref_lds_bytes(n) + 15) & ~(size_t)15;
-  if (n <= ref_lds_limit()) {
+  if (n <= kRefSmallN) {
+    ref_anneal_small_kernel<<<num_tries, 64, sizeof(double) * (size_t)n * n, st>>>(
+        n, q, beta, total, spb, k0, k1, best_bits, W, best_run);
+  } else if (n <= ref_lds_limit()) {
     ref_anneal_kernel<true><<<num_tries, 64, lds, st>>>(n, q, beta, total, spb, k0, k1, nullptr,
                                                          best_bits, W, best_run);
   } else {

@@ -338,7 +338,8 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
     auto fbuf = [&](int i) { return d.fields.as<float>() + (size_t)(i & 1) * T * np; };
     // blocks per field pass: OS_STREAM_GROUP overrides the default (tuning)
     const char *genv = std::getenv("OS_STREAM_GROUP");
-    d.group = genv ? std::min(16, std::max(1, std::atoi(genv))) : os::Dense::kGroup;
+    d.group = genv ? std::min(16, std::max(1, std::atoi(genv)))
+                   : (d.tuned_group ? d.tuned_group : os::Dense::kGroup);
     const int m = d.group, ring = 3 * d.group;
     // decide launches: operand chunking (OS_STREAM_DECIDE_VARIANT overrides
     // the default, for tuning)
@@ -352,9 +353,11 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
     // overrides the default); the grid leaves room for the decide launches
     const char *wenv = std::getenv("OS_STREAM_PASS_WGS_PER_CU");
     const char *grenv = std::getenv("OS_STREAM_PASS_GRID");
-    const int pgrid = grenv ? std::max(1, std::atoi(grenv))
-                           : c.num_cus * (wenv ? std::max(1, std::atoi(wenv))
-                                               : os::Dense::kPassWgsPerCu);
+    const int pgrid =
+        grenv ? std::max(1, std::atoi(grenv))
+              : c.num_cus * (wenv ? std::max(1, std::atoi(wenv))
+                                  : (d.tuned_pass_wgs ? d.tuned_pass_wgs
+                                                      : os::Dense::kPassWgsPerCu));
     // stream path, with stats: HIP events around every pass_stride-th pass
     // (at most kPassSamples of them)
     constexpr int kPassSamples = 256;
@@ -557,6 +560,52 @@ int os_dense_anneal(os_dense *d_, const os_sweep_params *p, double *best_energie
                     double *best_energy, os_sweep_stats *stats) {
   return dense_run(d_, p, nullptr, nullptr, best_energies, best_states, final_states, best_state,
                    best_energy, stats);
+}
+
+int os_dense_tune(os_dense *d_, const os_sweep_params *p, int32_t *group,
+                  int32_t *pass_wgs_per_cu, double *best_ms) {
+  if (group) *group = 0;
+  if (pass_wgs_per_cu) *pass_wgs_per_cu = 0;
+  if (best_ms) *best_ms = 0.0;
+  if (!d_ || !p) return dguard([&] { OS_REQUIRE(false, "null argument"); });
+  if (p->num_sweeps < 1 || !p->tscale)
+    return dguard([&] { OS_REQUIRE(false, "tuning needs num_sweeps >= 1 and tscale"); });
+  os::Dense &d = *reinterpret_cast<os::Dense *>(d_);
+  if (!d.stream) return OS_OK;  // n <= 1024: nothing to tune
+  // probe runs: the caller's replicas, schedule and seed, at most 4 sweeps,
+  // no outputs; each run starts from fresh states, so the probes leave the
+  // caller's later runs unchanged
+  os_sweep_params probe = *p;
+  probe.num_sweeps = std::min(p->num_sweeps, 4);
+  probe.refresh_interval = 0;
+  static constexpr int kGroups[] = {3, 4, 5, 6, 8};
+  static constexpr int kWgs[] = {1, 2};
+  int bg = 0, bw = 0;
+  double bms = 0.0;
+  for (int wg : kWgs)
+    for (int m : kGroups) {
+      d.tuned_group = m;
+      d.tuned_pass_wgs = wg;
+      os_sweep_stats st{};
+      const int rc = dense_run(d_, &probe, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, &st);
+      if (rc != OS_OK) {
+        d.tuned_group = bg;
+        d.tuned_pass_wgs = bw;
+        return rc;
+      }
+      if (bg == 0 || st.kernel_ms < bms) {
+        bg = m;
+        bw = wg;
+        bms = st.kernel_ms;
+      }
+    }
+  d.tuned_group = bg;
+  d.tuned_pass_wgs = bw;
+  if (group) *group = bg;
+  if (pass_wgs_per_cu) *pass_wgs_per_cu = bw;
+  if (best_ms) *best_ms = bms;
+  return OS_OK;
 }
 
 int os_dense_tempering(os_dense *d_, const os_tempering_params *t, double *best_energies,

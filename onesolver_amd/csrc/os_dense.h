@@ -34,6 +34,8 @@ struct Dense {
   static constexpr int kPassWgsPerCu = 1;  // default persistent pass work-groups per CU (room for decides)
   static constexpr int kDecideVariant = 20;  // default decide operand chunking (launch_stream_decide)
   int group = kGroup;
+  // streaming path: the autotuner's choice (0: not tuned, defaults apply)
+  int tuned_group = 0, tuned_pass_wgs = 0;
   hipStream_t side = nullptr;
   hipEvent_t ev_seg = nullptr, ev_dec = nullptr, ev_pass[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> pass_ev;  // timing pairs around sampled passes

@@ -154,6 +154,42 @@ def test_dense_streaming_decide_chunking(ctx, monkeypatch, variant):
     assert (out["best_states"] == bs).all()
 
 
+def test_dense_streaming_autotuner(ctx, monkeypatch):
+    """os_dense_tune picks blocks per field pass and pass work-groups per CU
+    from probe runs and keeps them on the handle.  Neither changes a result
+    (every field is the same fp32 sum in the same block order), so runs after
+    tuning equal the runs before it on a real-valued instance, and the oracle's
+    chains on an integer one.  For n <= 1024 nothing is tuned."""
+    for k in ("OS_STREAM_GROUP", "OS_STREAM_PASS_WGS_PER_CU", "OS_STREAM_PASS_GRID"):
+        monkeypatch.delenv(k, raising=False)
+    n, R, S = 2048, 64, 3
+    ts = _tscale(S)
+    print("[gpu] dense streaming autotuner, SK n=2048", flush=True)
+    b = onesolver_amd.DenseBatch(ctx, sk=(1, n, 9, 0))
+    before = b.anneal(R, ts, seed=5, final_states=True)
+    t = b.tune(R, ts, seed=5)
+    assert t["group"] in (3, 4, 5, 6, 8)
+    assert t["pass_wgs_per_cu"] in (1, 2)
+    assert t["probe_ms"] > 0.0
+    after = b.anneal(R, ts, seed=5, final_states=True)
+    assert after["stats"]["pass_launches"] == -(-(S * (n // 64)) // t["group"])
+    for k in ("final_states", "best_states", "best_energies"):
+        assert (after[k] == before[k]).all(), k
+    b.close()
+    print("[gpu] dense streaming autotuner, integer n=1100", flush=True)
+    q = _int_instance(1100, 12)
+    prob = onesolver_amd.DenseProblem(ctx, q)
+    prob.tune(R, ts, seed=7)
+    out = prob.anneal(R, ts, seed=7, final_states=True)
+    prob.close()
+    fin, bs, be = oracle.sweep_dense(q, 1100, ts, R, seed=7, instance=0)
+    assert (out["final_states"] == fin).all()
+    assert (out["best_states"] == bs).all()
+    small = onesolver_amd.DenseProblem(ctx, _int_instance(100, 1))
+    assert small.tune(R, ts) == {"group": 0, "pass_wgs_per_cu": 0, "probe_ms": 0.0}
+    small.close()
+
+
 def test_dense_streaming_sk_energies_and_drift(ctx):
     n, R, S = 2048, 64, 6
     b = onesolver_amd.DenseBatch(ctx, sk=(1, n, 4, 0))
