@@ -91,13 +91,26 @@ __global__ __launch_bounds__(64) void ref_anneal_kernel(int n, const double *__r
   if (lane == 0) best_run[replica] = e_best;
 }
 
-// n <= 64: the same chain with the replica in registers.  Lane j holds h_j;
-// the state and best state are wave-uniform 64-bit masks; Q is staged in
-// LDS.  Per attempt, h_s is one readlane and row s of Q is requested before
-// the acceptance test, so the chain waits on no memory but that LDS read.
-// Every FP64 operation, and the order of the initial field and energy sums,
-// is the one of ref_anneal_kernel, so the chains are the same.
+// n <= 64: the same chain with the replica in registers, kLook attempts per
+// step by speculation.  Lane j holds h_j; the state and best state are
+// wave-uniform 64-bit masks; Q is staged in LDS.  The draws of an attempt
+// do not depend on the state, so lane 2^m - 1 + p (m < kLook, p < 2^m)
+// evaluates attempt t + m as if attempts t .. t+m-1 had been accepted where
+// bit i of p is set: it replays their field updates on h at its own site, in
+// the chain's order and with the chain's fma, and takes the acceptance test.
+// A ballot of the 63 answers, walked from m = 0 along the decisions made,
+// gives the chain's kLook decisions; they are then applied in order.  Every
+// FP64 operation that reaches the state, energy or best state is the one of
+// ref_anneal_kernel, so the chains are the same.
 constexpr int kRefSmallN = 64;
+constexpr int kLook = 6;
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 
 __global__ __launch_bounds__(64) void ref_anneal_small_kernel(int n, const double *__restrict__ q,
                                                               const double *__restrict__ beta,
@@ -133,40 +146,93 @@ __global__ __launch_bounds__(64) void ref_anneal_small_kernel(int n, const doubl
   double e_cur = part;
   double e_best = part;
 
-  // beta[t / spb] is kept in b and reloaded when the index moves; the draw of
-  // attempt t + 1 is computed during attempt t (it depends on no state)
-  int bi = 0, bc = 0;
-  double b = total_attempts > 0 ? beta[0] : 0.0;
-  uint4 o = philox4x32_10(make_uint4(0u, replica, 0u, TAG_REF_STEP), k0, k1);
-  for (uint64_t t = 0; t < total_attempts; ++t) {
-    const int s = __builtin_amdgcn_readfirstlane((int)__umulhi(o.x, (uint32_t)n));
-    const double u = (double)o.y * 0x1p-32;
-    const uint64_t t1 = t + 1;
-    o = philox4x32_10(make_uint4((uint32_t)t1, replica, (uint32_t)(t1 >> 32), TAG_REF_STEP), k0,
-                      k1);
-    const double qv = live ? qs[s * n + lane] : 0.0;
+  // this lane's candidate: attempt offset m_l, assumed decisions p_l (lane
+  // 63 evaluates nothing); its draw is computed one step ahead
+  const int m_l = 31 - __clz(lane + 1);
+  const uint32_t p_l = (uint32_t)(lane + 1 - (1 << m_l));
+  const int m_c = min(m_l, kLook - 1);
+  auto draw = [&](uint64_t t) {
+    const uint64_t tm = t + (uint64_t)m_c;
+    return philox4x32_10(make_uint4((uint32_t)tm, replica, (uint32_t)(tm >> 32), TAG_REF_STEP),
+                         k0, k1);
+  };
+  uint4 o = draw(0);
+  int bi = 0, bc = 0;  // beta index of attempt t and its remainder: t = bi * spb + bc
+  for (uint64_t t = 0; t < total_attempts; t += kLook) {
+    const int kk = (int)min((uint64_t)kLook, total_attempts - t);
+    const int s_l = (int)__umulhi(o.x, (uint32_t)n);
+    const double u_l = (double)o.y * 0x1p-32;
+    o = draw(t + kLook);
+    // sites and temperatures of the kLook attempts (uniform)
+    int sm[kLook];
+    double bm[kLook];
+    int ci = bi, cc = bc, bi_next = bi, bc_next = bc;
+#pragma unroll
+    for (int m = 0; m < kLook; ++m) {
+      sm[m] = __builtin_amdgcn_readlane(s_l, (1 << m) - 1);
+      bm[m] = m < kk ? beta[ci] : 1.0;
+      if (++cc == spb) {
+        cc = 0;
+        ++ci;
+      }
+      if (m == kk - 1) {
+        bi_next = ci;
+        bc_next = cc;
+      }
+    }
+    // rows of Q at the kLook sites, for the updates applied below
+    double qrow[kLook];
+#pragma unroll
+    for (int m = 0; m < kLook; ++m) qrow[m] = live ? qs[sm[m] * n + lane] : 0.0;
+    // this lane's field h_{s_l} at time t (from lane s_l) and temperature
     const uint64_t hb = (uint64_t)__double_as_longlong(h);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hb, s);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hb >> 32), s);
-    const double hs = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-    const bool xs = (x >> s) & 1u;
-    const double de = xs ? -hs : hs;
-    const bool accept = (de < 0.0) || (exp(-de / b) > u);
-    if (accept) {
-      const double d = xs ? -1.0 : 1.0;
-      if (live && lane != s) h = fma(qv, d, h);
-      x ^= 1ull << s;
-      e_cur += de;
+    const uint32_t hlo = (uint32_t)__builtin_amdgcn_ds_bpermute(s_l << 2, (int)(uint32_t)hb);
+    const uint32_t hhi =
+        (uint32_t)__builtin_amdgcn_ds_bpermute(s_l << 2, (int)(uint32_t)(hb >> 32));
+    double h_l = __longlong_as_double((long long)(((uint64_t)hhi << 32) | hlo));
+    double b_l = bm[0];
+#pragma unroll
+    for (int m = 1; m < kLook; ++m) b_l = m_c == m ? bm[m] : b_l;
+    // replay the assumed decisions of attempts t .. t + m_l - 1 (selects,
+    // not branches: the fma is the chain's whether or not it is kept)
+    uint64_t fm = 0;  // spins flipped on the assumed path
+#pragma unroll
+    for (int i = 0; i < kLook - 1; ++i) {
+      const int si = sm[i];
+      const double qv = qs[si * n + s_l];
+      const bool on = i < m_l && ((p_l >> i) & 1u);
+      const bool xi = ((x ^ fm) >> si) & 1u;
+      const double hn = fma(qv, xi ? -1.0 : 1.0, h_l);
+      h_l = on && si != s_l ? hn : h_l;
+      fm ^= on ? 1ull << si : 0ull;
     }
-    if (e_cur < e_best) {
-      e_best = e_cur;
-      best = x;
+    const bool xl = ((x ^ fm) >> s_l) & 1u;
+    const double de = xl ? -h_l : h_l;
+    const bool acc = m_l < kk && ((de < 0.0) || (exp(-de / b_l) > u_l));
+    const uint64_t A = __ballot(acc);
+    // the chain's decisions, applied in order
+    uint32_t p = 0;
+#pragma unroll
+    for (int m = 0; m < kLook; ++m) {
+      if (m < kk) {
+        const int L = (1 << m) - 1 + (int)p;
+        if ((A >> L) & 1u) {
+          const int s = sm[m];
+          const double d = ((x >> s) & 1u) ? -1.0 : 1.0;
+          const double hn = fma(qrow[m], d, h);
+          h = live && lane != s ? hn : h;
+          x ^= 1ull << s;
+          e_cur += readlane_f64(de, L);
+          p |= 1u << m;
+        }
+        if (e_cur < e_best) {
+          e_best = e_cur;
+          best = x;
+        }
+      }
     }
-    if (++bc == spb) {
-      bc = 0;
-      ++bi;
-      if (t1 < total_attempts) b = beta[bi];
-    }
+    bi = bi_next;
+    bc = bc_next;
   }
   if (lane == 0) {
     uint32_t *dst = best_bits + (size_t)replica * W;
