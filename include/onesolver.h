@@ -313,6 +313,11 @@ typedef struct os_csr_params {
                                   wave updates several independent sites of its replicas at
                                   once.  The update at position p of the visiting order draws
                                   word p&3 of counter (p>>2, replica, sweep, tag) either way */
+  const float *init_marginals; /* optional warm start, per instance [n[i]] (instance-major):
+                                  x_j = (u < p_j), u = (w >> 8) 2^-24 with w word j&3 of
+                                  counter (j>>2, replica, 0, ((instance_base + i) << 8) | 0x12),
+                                  the rule of os_sweep_params::init_marginals; sampled on the
+                                  GPU.  Not together with init_states */
 } os_csr_params;
 
 /* Population annealing on a sparse batch (os_csr_batch_population): every

@@ -605,7 +605,7 @@ class CsrParams(ctypes.Structure):
     _fields_ = [("num_replicas", ctypes.c_uint32), ("num_sweeps", ctypes.c_int32),
                 ("tscale", _vp), ("seed", ctypes.c_uint64), ("instance_base", ctypes.c_uint32),
                 ("init_states", _vp), ("use_graph", ctypes.c_int32), ("tune", ctypes.c_int32),
-                ("site_order", ctypes.c_int32)]
+                ("site_order", ctypes.c_int32), ("init_marginals", _vp)]
 
 
 class CsrPopulationParams(ctypes.Structure):
@@ -714,10 +714,12 @@ class CsrBatch:
         return order, cs
 
     def anneal(self, num_replicas, tscale, seed=1, instance_base=0, init_states=None,
-               best_states=True, final_states=False, use_graph=True, tune=True, site_order=0):
+               best_states=True, final_states=False, use_graph=True, tune=True, site_order=0,
+               init_marginals=None):
         """Returns per-instance lists of best/final states ([R, n_i] uint8) and
         [I, R] arrays of FP64 best energies and running-best energies.
-        site_order 1: sweeps visit the sites class by class (site_order())."""
+        site_order 1: sweeps visit the sites class by class (site_order()).
+        init_marginals: per-instance [n_i] probabilities, sampled on the GPU."""
         ts = np.ascontiguousarray(tscale, dtype=np.float32)
         I, R = len(self.ns), int(num_replicas)
         ini = None
@@ -725,9 +727,14 @@ class CsrBatch:
             ini = np.ascontiguousarray(np.concatenate(
                 [np.asarray(s, dtype=np.uint8).reshape(R, n).ravel()
                  for s, n in zip(init_states, self.ns)]))
+        mar = None
+        if init_marginals is not None:
+            mar = np.ascontiguousarray(np.concatenate(
+                [np.asarray(m, dtype=np.float32).reshape(n) for m, n in zip(init_marginals,
+                                                                            self.ns)]))
         prm = CsrParams(R, len(ts), ts.ctypes.data, seed, instance_base,
                         None if ini is None else ini.ctypes.data, int(use_graph), int(tune),
-                        int(site_order))
+                        int(site_order), None if mar is None else mar.ctypes.data)
         tot = int(self.ns.sum()) * R
         be = np.zeros((I, R), dtype=np.float64)
         rb = np.zeros((I, R), dtype=np.float64)

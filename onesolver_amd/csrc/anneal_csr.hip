@@ -858,6 +858,50 @@ void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_word
   OS_HIP_CHECK(hipGetLastError());
 }
 
+// Warm start of a batch from per-instance marginals p (instance i at
+// p + spin_off): x_j = (u < p_j), u = (w >> 8) 2^-24, w word j&3 of Philox
+// counter (j>>2, r, 0, ((instance_base + i) << 8) | TAG_WARM), the dense
+// path's rule.  Work-group = instance; replicas r >= R and bit n (the ELL
+// padding target) stay zero.
+__global__ __launch_bounds__(256) void csr_warm_init_kernel(const CsrInst *__restrict__ inst,
+                                                            int R, int Rp, uint32_t k0,
+                                                            uint32_t k1, uint32_t instance_base,
+                                                            const float *__restrict__ p,
+                                                            uint32_t *__restrict__ planes) {
+  const CsrInst d = inst[blockIdx.x];
+  const uint32_t tag = ((instance_base + blockIdx.x) << 8) | TAG_WARM;
+  const float *pi = p + d.spin_off;
+  uint32_t *pl = planes + d.state_off;
+  for (int idx = threadIdx.x; idx < d.words * Rp; idx += blockDim.x) {
+    const int w = idx / Rp, r = idx % Rp;
+    uint32_t v = 0;
+    if (r < R) {
+      for (int q = 0; q < 8; ++q) {
+        const int j0 = w * 32 + 4 * q;
+        if (j0 >= d.n) break;
+        const uint4 o =
+            philox4x32_10(make_uint4((uint32_t)(j0 >> 2), (uint32_t)r, 0u, tag), k0, k1);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int j = j0 + jj;
+          if (j < d.n && (float)(pick(o, jj) >> 8) * 0x1p-24f < pi[j]) v |= 1u << (4 * q + jj);
+        }
+      }
+    }
+    pl[(size_t)w * Rp + r] = v;
+  }
+}
+
+void launch_csr_warm_init(hipStream_t st, const CsrInst *inst, int num_instances, int R, int Rp,
+                          uint64_t seed, uint32_t instance_base, const float *p,
+                          uint32_t *planes) {
+  if (num_instances == 0) return;
+  csr_warm_init_kernel<<<num_instances, 256, 0, st>>>(inst, R, Rp, (uint32_t)seed,
+                                                      (uint32_t)(seed >> 32), instance_base, p,
+                                                      planes);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, int max_words, const int32_t *row_ptr, const int32_t *col,
                        const double *val, const double *diag64, const uint32_t *bits,

@@ -43,6 +43,7 @@ struct CsrBatch {
   // per run shape
   int Rp = 0;
   DevBuf items, run, tscale, init, best, fin, eb, ef, e64, queue;
+  DevBuf marg;  // warm-start marginals (total_spins floats)
   // population annealing: best planes and energies of every replica index
   // over the steps, ancestors of a resampling round
   DevBuf pop_best, pop_eb, pop_anc;
@@ -85,7 +86,7 @@ struct CsrBatch {
   ~CsrBatch() {
     drop_graph();
     for (auto *b : {&d_inst, &ell_col, &ell_val, &diag, &diag64, &row_ptr, &col, &val64, &items,
-                    &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &corder,
+                    &run, &tscale, &init, &best, &fin, &eb, &ef, &e64, &queue, &marg, &corder,
                     &cstart, &items_c, &pell_col, &pell_val, &pdiag, &pop_best, &pop_eb,
                     &pop_anc, &tsr, &pt_slots})
       b->release();
@@ -659,10 +660,21 @@ int os_csr_batch_anneal(os_csr_batch *b_, const os_csr_params *p, double *best_e
     run.num_items = b.num_items;
     run.num_sweeps = S;
     run.Rp = Rp;
-    run.init_mode = p->init_states ? 1 : 0;
+    OS_REQUIRE(!(p->init_states && p->init_marginals),
+               "give init_states or init_marginals, not both");
+    run.init_mode = p->init_states || p->init_marginals ? 1 : 0;
     run.k0 = (uint32_t)p->seed;
     run.k1 = (uint32_t)(p->seed >> 32);
     run.instance_base = p->instance_base;
+    if (p->init_marginals) {
+      // start planes sampled on the GPU from the marginals
+      b.marg.ensure(sizeof(float) * (size_t)std::max<int64_t>(b.total_spins, 1));
+      OS_HIP_CHECK(hipMemcpyAsync(b.marg.ptr, p->init_marginals,
+                                  sizeof(float) * (size_t)b.total_spins, hipMemcpyHostToDevice,
+                                  s));
+      os::launch_csr_warm_init(s, b.d_inst.as<os::CsrInst>(), b.I, R, Rp, p->seed,
+                               p->instance_base, b.marg.as<float>(), b.init.as<uint32_t>());
+    }
     if (p->init_states) {
       // pack the caller's states (instance-major, [R x n_i]) into planes
       std::vector<uint32_t> planes((size_t)b.total_words * Rp, 0u);

@@ -159,6 +159,11 @@ void launch_csr_col_sweeps(hipStream_t st, int waves, int groups, int plane_word
                            const float4 *pell_val, const float *pdiag, const float *tscale,
                            const float *tsr, const uint32_t *init_bits, uint32_t *best_bits,
                            uint32_t *final_bits, double *e_best, double *e_final, int *queue);
+// start planes of every instance from marginals p (Σn floats, instance i at
+// spin_off): the dense warm-start rule, stream instance_base + i
+void launch_csr_warm_init(hipStream_t st, const CsrInst *inst, int num_instances, int R, int Rp,
+                          uint64_t seed, uint32_t instance_base, const float *p,
+                          uint32_t *planes);
 void launch_csr_energy(hipStream_t st, const CsrInst *inst, const int2 *items, int num_items,
                        int Rp, int max_words, const int32_t *row_ptr, const int32_t *col,
                        const double *val, const double *diag64, const uint32_t *bits,

@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from . import Context, DenseBatch, WarmStartNet, tscale_from_beta
+from . import Context, CsrBatch, DenseBatch, WarmStartNet, instances as _inst, tscale_from_beta
 
 
 def geometric(beta0: float, beta1: float, sweeps: int) -> np.ndarray:
@@ -156,3 +156,99 @@ def evaluate(ctx: Context, net: WarmStartNet, n: int = 1024, instances: int = 32
                     else "-> not reached")
             log(f"{k:22s} {rungs} {tail}")
     return res
+
+
+# ---- mixed-sparse instances ------------------------------------------------
+# The network reads dense instances.  A sparse instance (CSR) is densified
+# with one extra spin in front: spin 0 is isolated and labelled 1, so the
+# network's canonical labels (spin 0 pinned to 1) are the instance's own
+# states, which is what instances with a diagonal field need (their
+# complement is not a symmetry).  Instances are zero-padded to 128-multiples
+# and batched by that size; padding spins are isolated and labelled 0.
+
+
+def pinned_size(n: int) -> int:
+    return (n + 1 + 127) // 128 * 128
+
+
+def pinned_dense(inst, size: int) -> np.ndarray:
+    """[size, size] FP64: spin j of the instance at j + 1, spin 0 isolated."""
+    rp, cl, vl, dg = inst
+    n = len(dg)
+    q = np.zeros((size, size))
+    q[1:n + 1, 1:n + 1] = _inst.csr_to_dense(rp, cl, vl, dg)
+    return q
+
+
+def _by_size(insts, chunk: int):
+    groups = {}
+    for i, inst in enumerate(insts):
+        groups.setdefault(pinned_size(len(inst[3])), []).append(i)
+    for size in sorted(groups):
+        ids = groups[size]
+        for c in range(0, len(ids), chunk):
+            yield size, ids[c:c + chunk]
+
+
+def sparse_labels(ctx: Context, insts, replicas: int = 64, sweeps: int = 512, seed: int = 7,
+                  instance_base: int = 0, site_order: int = 1):
+    """The best state (lowest FP64 energy over the replicas) of a plain
+    anneal of every instance, and its energy."""
+    b = CsrBatch(ctx, insts)
+    out = b.anneal(replicas, geometric(0.1, 3.0, sweeps), seed=seed, instance_base=instance_base,
+                   best_states=True, site_order=site_order)
+    b.close()
+    best = [bs[int(np.argmin(be))] for bs, be in zip(out["best_states"], out["best_energies"])]
+    return best, out["inst_best_energy"]
+
+
+def train_sparse(ctx: Context, net: WarmStartNet, insts, labels, epochs: int = 2,
+                 chunk: int = 32, minibatch: int = 8, lr: float = 1e-3, log=None):
+    """Trains `net` on sparse instances with their label states; returns the
+    per-step losses."""
+    losses = []
+    for ep in range(epochs):
+        for size, ids in _by_size(insts, chunk):
+            d = DenseBatch(ctx, np.stack([pinned_dense(insts[i], size) for i in ids]))
+            y = np.zeros((len(ids), size), dtype=np.uint8)
+            y[:, 0] = 1
+            for k, i in enumerate(ids):
+                y[k, 1:len(insts[i][3]) + 1] = labels[i]
+            for m in range(0, len(ids), minibatch):
+                losses.append(net.train_step(d, y[m:m + minibatch], lr, first=m))
+            d.close()
+        if log:
+            log(f"sparse epoch {ep}: mean loss {np.mean(losses):.4f}")
+    return losses
+
+
+def sparse_marginals(ctx: Context, net: WarmStartNet, insts, chunk: int = 32):
+    """Per-instance marginals [n_i] of the network."""
+    out = [None] * len(insts)
+    for size, ids in _by_size(insts, chunk):
+        d = DenseBatch(ctx, np.stack([pinned_dense(insts[i], size) for i in ids]))
+        p = net.predict(d)
+        d.close()
+        for k, i in enumerate(ids):
+            out[i] = np.ascontiguousarray(p[k, 1:len(insts[i][3]) + 1])
+    return out
+
+
+def sparse_time_to_target(batch: CsrBatch, targets, replicas: int, sweeps_ladder, beta, seed,
+                          marginals=None, t_infer: float = 0.0, site_order: int = 1):
+    """Per rung: (sweeps, wall seconds incl. t_infer, fraction of instances
+    at target, sum of best energies); and the first rung where all reached
+    it."""
+    rows, hit = [], None
+    for S in sweeps_ladder:
+        t0 = time.perf_counter()
+        out = batch.anneal(replicas, geometric(beta[0], beta[1], S), seed=seed,
+                           init_marginals=marginals, best_states=False, site_order=site_order)
+        wall = time.perf_counter() - t0 + t_infer
+        e = out["inst_best_energy"]
+        frac = float(np.mean(e <= targets))
+        rows.append({"sweeps": S, "wall_s": wall, "frac_at_target": frac,
+                     "sum_best": float(np.sum(e))})
+        if hit is None and frac == 1.0:
+            hit = rows[-1]
+    return rows, hit

@@ -70,3 +70,69 @@ def test_colder_schedule_with_learned_start_reaches_target_no_later_than_plain(c
     print(f"[gpu] same schedule: learned {learned['sweeps']} sweeps, random {same['sweeps']}",
           flush=True)
     assert learned["sweeps"] <= 2 * same["sweeps"]
+
+
+# ---- mixed-sparse instances ------------------------------------------------
+
+
+def test_csr_init_marginals_follow_the_dense_warm_start_rule(ctx):
+    """os_csr_params::init_marginals samples the start planes on the GPU with
+    the dense path's rule: the run equals the one started from the oracle's
+    warm_init states, bit for bit."""
+    import oracle
+    from onesolver_amd import instances
+
+    insts = [instances.regular_csr(n, d, seed=3, instance=i)
+             for i, (n, d) in enumerate([(40, 3), (97 + 1, 4), (130, 5), (33 + 1, 6)])]
+    rng = np.random.default_rng(4)
+    marg = [rng.random(len(dg)).astype(np.float32) for (_, _, _, dg) in insts]
+    R, seed, base = 64, 21, 5
+    ts = onesolver_amd.tscale_from_beta(np.geomspace(0.5, 3.0, 3))
+    init = [oracle.warm_init(p, R, seed=seed, instance=base + i) for i, p in enumerate(marg)]
+    print("[gpu] csr init_marginals vs oracle warm_init", flush=True)
+    b = onesolver_amd.CsrBatch(ctx, insts)
+    for order in (0, 1):
+        a = b.anneal(R, ts, seed=seed, instance_base=base, init_marginals=marg, final_states=True,
+                     site_order=order)
+        c = b.anneal(R, ts, seed=seed, instance_base=base, init_states=init, final_states=True,
+                     site_order=order)
+        for i in range(len(insts)):
+            assert (a["final_states"][i] == c["final_states"][i]).all(), (order, i)
+            assert (a["best_states"][i] == c["best_states"][i]).all(), (order, i)
+        assert (a["best_energies"] == c["best_energies"]).all()
+    b.close()
+
+
+def test_sparse_warm_start_gains_at_a_fixed_budget(ctx):
+    """The network on mixed-sparse (config-3 distribution) instances: trained
+    on 256 instances labelled by the sparse annealer, its marginals seed
+    unseen instances.  On the same schedule and sampler, starts from its
+    marginals put more instances at 99% of the best known energy after a few
+    cold sweeps than starts from p = 1/2."""
+    from onesolver_amd import instances
+
+    train = instances.batch_c3(256, seed=201, n_max=256)
+    test = instances.batch_c3(64, seed=202, n_max=256)
+    print("[gpu] sparse warm start: labels and training (256 instances)", flush=True)
+    labels, _ = ws.sparse_labels(ctx, train, replicas=64, sweeps=512)
+    net = onesolver_amd.WarmStartNet(ctx, seed=1)
+    losses = ws.train_sparse(ctx, net, train, labels, epochs=3)
+    assert np.mean(losses[-10:]) < np.mean(losses[:10])
+    p = ws.sparse_marginals(ctx, net, test)
+    net.close()
+    b = onesolver_amd.CsrBatch(ctx, test)
+    R = 128
+    ref = b.anneal(R, ws.geometric(0.1, 3.0, 2048), seed=99, best_states=False, site_order=1)
+    targets = 0.99 * ref["inst_best_energy"]
+    half = [np.full(len(q), 0.5, dtype=np.float32) for q in p]
+    ladder = [2, 4, 8]
+    learned, _ = ws.sparse_time_to_target(b, targets, R, ladder, (3.0, 3.0), seed=11, marginals=p)
+    blind, _ = ws.sparse_time_to_target(b, targets, R, ladder, (3.0, 3.0), seed=11,
+                                        marginals=half)
+    b.close()
+    for x, y in zip(learned, blind):
+        print(f"[gpu] {x['sweeps']} sweeps at beta 3: learned {x['frac_at_target']:.3f} "
+              f"({x['sum_best']:.1f}), p=1/2 {y['frac_at_target']:.3f} ({y['sum_best']:.1f})",
+              flush=True)
+    assert learned[0]["sum_best"] < blind[0]["sum_best"]
+    assert learned[1]["frac_at_target"] >= blind[1]["frac_at_target"] + 0.1
