@@ -472,9 +472,9 @@ def run_warmstart_c4_sparse(args):
     R = 256
     ref = b.anneal(R, ws.geometric(0.1, 3.0, 4096), seed=99, best_states=False, site_order=1)
     targets = 0.99 * ref["inst_best_energy"]
-    ws.sparse_marginals(ctx, net, ev[:8])  # warm-up
+    ws.sparse_marginals(ctx, net, ev, batch=b)  # warm-up
     t0 = time.perf_counter()
-    p = ws.sparse_marginals(ctx, net, ev)
+    p = ws.sparse_marginals(ctx, net, ev, batch=b)
     t_inf = time.perf_counter() - t0
     half = [np.full(len(q), 0.5, dtype=np.float32) for q in p]
     lad = ws.ladder(512)

@@ -357,6 +357,16 @@ int os_csr_batch_anneal(os_csr_batch *batch, const os_csr_params *params,
                         double *best_energies, double *run_best_energies, uint8_t *best_states,
                         uint8_t *final_states, double *inst_best_energy,
                         int32_t *inst_best_replica, os_sweep_stats *stats);
+/* Marginals of the warm-start network on every spin of a sparse batch
+ * (marginals: sum of n[i] floats, instance-major), the input of
+ * os_csr_params::init_marginals.  The network reads each instance as it
+ * reads the instance's dense matrix (features (Q_jj, sum_l Q_jl) / scale, 0, 1;
+ * A = bf16(Q_jl / scale)), with a sparse aggregation; scale is the dense one
+ * over scale_rows[i] rows (the instance's size in the dense layout the
+ * network was trained on; NULL: n[i]).  The batch and the network must share
+ * a context.  No reference counterpart. */
+int os_mpnn_predict_csr(os_mpnn *net, os_csr_batch *batch, const int32_t *scale_rows,
+                        float *marginals);
 /* Replica exchange on a sparse batch (os_csr_batch_tempering): every
  * instance runs num_ladders ladders of num_temperatures slots; replica
  * r = l * num_temperatures + k starts in slot k of ladder l and sweeps at

@@ -822,6 +822,7 @@ def _mpnn_lib():
         L.os_mpnn_get_params.argtypes = [_vp, _f32p]
         L.os_mpnn_set_params.argtypes = [_vp, _f32p]
         L.os_mpnn_predict.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _f32p]
+        L.os_mpnn_predict_csr.argtypes = [_vp, _vp, _vp, _f32p]
         L.os_mpnn_loss_grad.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _u8p,
                                         ctypes.POINTER(ctypes.c_double), _f32p]
         L.os_mpnn_train_step.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, _u8p,
@@ -868,6 +869,25 @@ class WarmStartNet:
         count = data.I - first if count is None else count
         out = np.zeros((count, data.n), dtype=np.float32)
         _check(_mpnn_lib().os_mpnn_predict(self._h, data._h, first, count, _p(out, _f32p)))
+        return out
+
+    def predict_csr(self, batch: "CsrBatch", scale_rows=None):
+        """Per-instance marginals [n_i] on a sparse batch (os_mpnn_predict_csr);
+        scale_rows: per-instance row count of the scale (the dense layout's
+        size), None for n_i."""
+        flat = np.zeros(int(batch.ns.sum()), dtype=np.float32)
+        rows = None
+        if scale_rows is not None:
+            rows = np.ascontiguousarray(scale_rows, dtype=np.int32)
+            if rows.size != len(batch.ns):
+                raise ValueError("scale_rows needs one entry per instance")
+        _check(_mpnn_lib().os_mpnn_predict_csr(self._h, batch._h,
+                                               None if rows is None else rows.ctypes.data,
+                                               _p(flat, _f32p)))
+        out, at = [], 0
+        for n in batch.ns:
+            out.append(flat[at:at + n].copy())
+            at += n
         return out
 
     def loss_grad(self, data: "DenseBatch", labels, first=0):

@@ -933,6 +933,36 @@ int os_csr_batch_tempering(os_csr_batch *b_, const os_csr_tempering_params *p,
   });
 }
 
+}  // extern "C"
+
+namespace os {
+// The batch's device arrays and the context it lives on (for the network).
+CsrView csr_view(os_csr_batch *b_, Ctx **ctx) {
+  OS_REQUIRE(b_ != nullptr, "null batch");
+  CsrBatch &b = *reinterpret_cast<CsrBatch *>(b_);
+  // the descriptors reach the device with a batch's first run (shape_for);
+  // a batch that has not run yet gets them here (n and the CSR offsets are
+  // fixed at creation)
+  b.ctx->activate();
+  b.d_inst.ensure(sizeof(CsrInst) * std::max(b.I, 1));
+  OS_HIP_CHECK(hipMemcpyAsync(b.d_inst.ptr, b.inst.data(), sizeof(CsrInst) * b.I,
+                              hipMemcpyHostToDevice, b.ctx->stream));
+  CsrView v;
+  v.num_instances = b.I;
+  v.total_spins = b.total_spins;
+  for (const auto &d : b.inst) v.max_n = std::max(v.max_n, d.n);
+  v.inst = b.d_inst.as<CsrInst>();
+  v.row_ptr = b.row_ptr.as<int32_t>();
+  v.col = b.col.as<int32_t>();
+  v.val = b.val64.as<double>();
+  v.diag = b.diag64.as<double>();
+  *ctx = b.ctx;
+  return v;
+}
+}  // namespace os
+
+extern "C" {
+
 int os_csr_batch_launch_shape(os_csr_batch *b_, int32_t *waves_per_group, int32_t *groups,
                               int32_t *prefetch) {
   return os::guarded([&] {

@@ -30,6 +30,13 @@ void mp_launch_loss_sum(hipStream_t st, size_t count, const double *terms, doubl
 void mp_launch_adam(hipStream_t st, int P, const float *grad, const uint8_t *kind, float *w,
                     float *m, float *v, float *wc, float lr, float b1, float b2, float eps,
                     int step);
+void mp_launch_csr_features(hipStream_t st, int I, int max_n, const CsrInst *inst,
+                            const int32_t *row_ptr, const double *val, const double *diag,
+                            const int32_t *scale_rows, double *scale, float *X0);
+void mp_launch_csr_agg(hipStream_t st, int I, int max_n, const CsrInst *inst,
+                       const int32_t *row_ptr, const int32_t *col, const double *val,
+                       const double *scale, const float *Hin, float *M);
+CsrView csr_view(os_csr_batch *b, Ctx **ctx);
 void mp_launch_compute_copy(hipStream_t st, int P, const uint8_t *kind, const float *w,
                             float *wc);
 
@@ -68,9 +75,10 @@ struct Mpnn {
   int B = 0, Np = 0;
   DevBuf rowstat, scale, A, X0, XT0, HT[kLayers], M[kLayers], Z[kLayers], H[kLayers];
   DevBuf p, lossterm, dlogit, dH, dHprev, dZ, dMsgT, partial, lossout, labels;
+  DevBuf srows;  // sparse batches: the rows of each instance's scale
   ~Mpnn() {
     for (auto *b : {&w, &wc, &grad, &m, &v, &kind, &rowstat, &scale, &A, &X0, &XT0, &p, &lossterm,
-                    &dlogit, &dH, &dHprev, &dZ, &dMsgT, &partial, &lossout, &labels})
+                    &dlogit, &dH, &dHprev, &dZ, &dMsgT, &partial, &lossout, &labels, &srows})
       b->release();
     for (int l = 0; l < kLayers; ++l)
       for (auto *b : {&HT[l], &M[l], &Z[l], &H[l]}) b->release();
@@ -122,6 +130,42 @@ struct Mpnn {
     mp_launch_head(st, count, n, np, H[kLayers - 1].as<float>(), wc.as<float>() + kWo, dev_labels,
                    1.0f / ((float)count * (float)n), p.as<float>(), lossterm.as<double>(),
                    dlogit.as<float>(), dH.as<float>());
+  }
+
+  // forward pass over a sparse batch (marginals in p, one row per spin);
+  // buffers for the rows only (no dense adjacency)
+  void forward_csr(const CsrView &v, const int32_t *dev_scale_rows) {
+    const size_t rows = (size_t)(v.total_spins + 63) / 64 * 64;
+    hipStream_t st = ctx->stream;
+    scale.ensure(sizeof(double) * std::max(v.num_instances, 1));
+    X0.ensure(sizeof(float) * rows * 64);
+    for (int l = 0; l < kLayers; ++l) {
+      M[l].ensure(sizeof(float) * rows * 64);
+      Z[l].ensure(sizeof(float) * rows * 64);
+      H[l].ensure(sizeof(float) * rows * 64);
+    }
+    p.ensure(sizeof(float) * rows);
+    // padding rows: zero features and messages, so every row the node
+    // kernels read is defined
+    OS_HIP_CHECK(hipMemsetAsync(X0.ptr, 0, sizeof(float) * rows * 64, st));
+    for (int l = 0; l < kLayers; ++l)
+      OS_HIP_CHECK(hipMemsetAsync(M[l].ptr, 0, sizeof(float) * rows * 64, st));
+    mp_launch_csr_features(st, v.num_instances, v.max_n, v.inst, v.row_ptr, v.val, v.diag,
+                           dev_scale_rows, scale.as<double>(), X0.as<float>());
+    for (int l = 0; l < kLayers; ++l) {
+      const float *hin = l == 0 ? X0.as<float>() : H[l - 1].as<float>();
+      mp_launch_csr_agg(st, v.num_instances, v.max_n, v.inst, v.row_ptr, v.col, v.val,
+                        scale.as<double>(), hin, M[l].as<float>());
+      mp_launch_node_fwd(st, 1, (int)rows, hin, M[l].as<float>(),
+                         wc.as<float>() + l * kLayerStride, Z[l].as<float>(), H[l].as<float>(),
+                         nullptr);
+    }
+    mp_launch_head(st, 1, (int)rows, (int)rows, H[kLayers - 1].as<float>(),
+                   wc.as<float>() + kWo, nullptr, 0.0f, p.as<float>(), nullptr, nullptr,
+                   nullptr);
+    // the next dense forward re-shapes its buffers from scratch
+    B = 0;
+    Np = 0;
   }
 
   // gradient of the mean loss into `grad` (padded layout); forward() with
@@ -284,6 +328,33 @@ int os_mpnn_predict(os_mpnn *h, os_dense *data, int first, int count, float *mar
     for (int b = 0; b < count; ++b)
       std::memcpy(marginals + (size_t)b * d.n, p.data() + (size_t)b * net.Np,
                   sizeof(float) * d.n);
+  });
+}
+
+int os_mpnn_predict_csr(os_mpnn *h, os_csr_batch *batch, const int32_t *scale_rows,
+                        float *marginals) {
+  return os::guarded([&] {
+    Mpnn &net = net_of(h);
+    OS_REQUIRE(batch && marginals, "null argument");
+    Ctx *bctx = nullptr;
+    const os::CsrView v = os::csr_view(batch, &bctx);
+    OS_REQUIRE(bctx == net.ctx, "the network and the batch belong to different contexts");
+    OS_REQUIRE(v.inst && v.row_ptr && v.diag && (v.col && v.val || v.total_spins == 0),
+               "the batch's device arrays are missing");
+    net.ctx->activate();
+    const int32_t *dev_rows = nullptr;
+    if (scale_rows) {
+      for (int i = 0; i < v.num_instances; ++i)
+        OS_REQUIRE(scale_rows[i] >= 1, "scale_rows must be positive");
+      net.srows.ensure(sizeof(int32_t) * std::max(v.num_instances, 1));
+      OS_HIP_CHECK(hipMemcpyAsync(net.srows.ptr, scale_rows, sizeof(int32_t) * v.num_instances,
+                                  hipMemcpyHostToDevice, net.ctx->stream));
+      dev_rows = net.srows.as<int32_t>();
+    }
+    net.forward_csr(v, dev_rows);
+    OS_HIP_CHECK(hipMemcpyAsync(marginals, net.p.ptr, sizeof(float) * (size_t)v.total_spins,
+                                hipMemcpyDeviceToHost, net.ctx->stream));
+    net.ctx->sync();
   });
 }
 

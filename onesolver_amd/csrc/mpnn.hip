@@ -97,6 +97,100 @@ __global__ __launch_bounds__(256) void mp_fill_kernel(int n, int Np, const doubl
   }
 }
 
+// ---- sparse instances (CSR batch) -------------------------------------------
+// The network on a batch of CSR instances, as on each instance's dense
+// matrix: one row per spin, instance i at rows spin_off.. (instance-major,
+// padded to a multiple of 64 with zero rows).  The scale of instance i is
+// the dense one over scale_rows[i] rows (its size in the dense layout the
+// network was trained on; n when null): rows without couplings add nothing
+// but the row count.  A_jl = bf16((float)(Q_jl / scale)) as in mp_fill_kernel.
+
+// scale[i] = sqrt(sum_j (sum_{l != j} Q_jl^2) / rows_i), the row sums first
+__global__ void mp_csr_scale_kernel(int I, const CsrInst *__restrict__ inst,
+                                    const int32_t *__restrict__ row_ptr,
+                                    const double *__restrict__ val,
+                                    const int32_t *__restrict__ scale_rows,
+                                    double *__restrict__ scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= I) return;
+  const CsrInst d = inst[i];
+  const int32_t *rp = row_ptr + d.row_off;
+  const double *vl = val + d.csr_off;
+  double s = 0.0;
+  for (int j = 0; j < d.n; ++j) {
+    double s2 = 0.0;
+    for (int k = rp[j]; k < rp[j + 1]; ++k) s2 += vl[k] * vl[k];
+    s += s2;
+  }
+  const int rows = scale_rows ? scale_rows[i] : d.n;
+  s = sqrt(s / rows);
+  scale[i] = s > 0.0 ? s : 1.0;
+}
+
+// features of spin j: (Q_jj, sum_l Q_jl) / scale, 0, 1 (grid: rows / 256, I)
+__global__ __launch_bounds__(256) void mp_csr_features_kernel(
+    const CsrInst *__restrict__ inst, const int32_t *__restrict__ row_ptr,
+    const double *__restrict__ val, const double *__restrict__ diag,
+    const double *__restrict__ scale, float *__restrict__ X0) {
+  const CsrInst d = inst[blockIdx.y];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= d.n) return;
+  const int32_t *rp = row_ptr + d.row_off;
+  const double *vl = val + d.csr_off;
+  const double sc = scale[blockIdx.y];
+  double s1 = 0.0;
+  for (int k = rp[j]; k < rp[j + 1]; ++k) s1 += vl[k];
+  float *x = X0 + (size_t)(d.spin_off + j) * 64;
+  x[0] = (float)(diag[d.spin_off + j] / sc);
+  x[1] = (float)(s1 / sc);
+  x[2] = 0.0f;
+  x[3] = 1.0f;
+  for (int f = 4; f < 64; ++f) x[f] = 0.0f;
+}
+
+// M[row j] = sum_k A_{j col_k} bf16(Hin[col_k]) in fp32 (one wave per spin,
+// lane = feature; grid: rows / 4, I)
+__global__ __launch_bounds__(256) void mp_csr_agg_kernel(
+    const CsrInst *__restrict__ inst, const int32_t *__restrict__ row_ptr,
+    const int32_t *__restrict__ col, const double *__restrict__ val,
+    const double *__restrict__ scale, const float *__restrict__ Hin, float *__restrict__ M) {
+  const CsrInst d = inst[blockIdx.y];
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int f = threadIdx.x & 63;
+  if (j >= d.n) return;
+  const int32_t *rp = row_ptr + d.row_off;
+  const int32_t *cl = col + d.csr_off;
+  const double *vl = val + d.csr_off;
+  const double sc = scale[blockIdx.y];
+  const float *h = Hin + (size_t)d.spin_off * 64 + f;
+  float acc = 0.0f;
+  for (int k = rp[j]; k < rp[j + 1]; ++k) {
+    const float a = bf16_round((float)(vl[k] / sc));
+    acc = fmaf(a, bf16_round(h[(size_t)cl[k] * 64]), acc);
+  }
+  M[(size_t)(d.spin_off + j) * 64 + f] = acc;
+}
+
+void mp_launch_csr_features(hipStream_t st, int I, int max_n, const CsrInst *inst,
+                            const int32_t *row_ptr, const double *val, const double *diag,
+                            const int32_t *scale_rows, double *scale, float *X0) {
+  if (I == 0 || max_n == 0) return;
+  mp_csr_scale_kernel<<<(I + 63) / 64, 64, 0, st>>>(I, inst, row_ptr, val, scale_rows, scale);
+  OS_HIP_CHECK(hipGetLastError());
+  mp_csr_features_kernel<<<dim3((max_n + 255) / 256, I), 256, 0, st>>>(inst, row_ptr, val, diag,
+                                                                       scale, X0);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
+void mp_launch_csr_agg(hipStream_t st, int I, int max_n, const CsrInst *inst,
+                       const int32_t *row_ptr, const int32_t *col, const double *val,
+                       const double *scale, const float *Hin, float *M) {
+  if (I == 0 || max_n == 0) return;
+  mp_csr_agg_kernel<<<dim3((max_n + 3) / 4, I), 256, 0, st>>>(inst, row_ptr, col, val, scale,
+                                                              Hin, M);
+  OS_HIP_CHECK(hipGetLastError());
+}
+
 // ---- neighbour aggregation: Y[b] (+)= A[b] . X[b], X given feature-major bf16
 template <bool kAcc>
 __global__ __launch_bounds__(256) void mp_agg_kernel(int Np, const uint16_t *__restrict__ A,

@@ -119,6 +119,19 @@ struct CsrInst {
   int32_t sites;      // colour-class kernel: sites per wave step (4, 8 or 16)
   int32_t pad_;
 };
+// The device arrays of a sparse batch, for kernels outside the annealer (the
+// warm-start network): instance descriptors, CSR (both directions, no
+// diagonal) and the diagonal, with spin j of instance i at spin_off + j.
+struct CsrView {
+  int num_instances = 0;
+  int max_n = 0;
+  int64_t total_spins = 0;
+  const CsrInst *inst = nullptr;
+  const int32_t *row_ptr = nullptr;
+  const int32_t *col = nullptr;
+  const double *val = nullptr;
+  const double *diag = nullptr;
+};
 // Parameters of one batched run, read by the kernel from device memory so a
 // captured graph replays with new values.
 struct CsrRun {

@@ -222,8 +222,23 @@ def train_sparse(ctx: Context, net: WarmStartNet, insts, labels, epochs: int = 2
     return losses
 
 
-def sparse_marginals(ctx: Context, net: WarmStartNet, insts, chunk: int = 32):
-    """Per-instance marginals [n_i] of the network."""
+def sparse_marginals(ctx: Context, net: WarmStartNet, insts, batch: CsrBatch = None):
+    """Per-instance marginals [n_i] of the network, from the sparse batch
+    (os_mpnn_predict_csr; `batch` holds `insts`, or one is made): the
+    features and couplings the network saw in training, the scale over the
+    padded dense size."""
+    own = batch is None
+    if own:
+        batch = CsrBatch(ctx, insts)
+    p = net.predict_csr(batch, [pinned_size(len(inst[3])) for inst in insts])
+    if own:
+        batch.close()
+    return p
+
+
+def sparse_marginals_dense(ctx: Context, net: WarmStartNet, insts, chunk: int = 32):
+    """The same marginals through the densified instances (the training
+    layout), for checking the sparse inference."""
     out = [None] * len(insts)
     for size, ids in _by_size(insts, chunk):
         d = DenseBatch(ctx, np.stack([pinned_dense(insts[i], size) for i in ids]))

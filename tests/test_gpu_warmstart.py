@@ -136,3 +136,21 @@ def test_sparse_warm_start_gains_at_a_fixed_budget(ctx):
               flush=True)
     assert learned[0]["sum_best"] < blind[0]["sum_best"]
     assert learned[1]["frac_at_target"] >= blind[1]["frac_at_target"] + 0.1
+
+
+def test_sparse_inference_matches_the_densified_instances(ctx):
+    """os_mpnn_predict_csr (sparse aggregation on the CSR batch) computes the
+    network's marginals as on the densified instances it was trained on; the
+    two differ only by the fp32 summation order of the aggregation."""
+    from onesolver_amd import instances
+
+    insts = instances.batch_c3(24, seed=203, n_max=600)
+    net = onesolver_amd.WarmStartNet(ctx, seed=7)
+    print("[gpu] sparse inference vs densified", flush=True)
+    a = ws.sparse_marginals(ctx, net, insts)
+    b = ws.sparse_marginals_dense(ctx, net, insts)
+    net.close()
+    diff = np.concatenate([np.abs(x - y) for x, y in zip(a, b)])
+    print(f"[gpu] max |dp| {diff.max():.2e}, mean {diff.mean():.2e}", flush=True)
+    assert diff.max() < 2e-2
+    assert diff.mean() < 1e-3
