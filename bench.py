@@ -514,6 +514,81 @@ def run_warmstart_c4_sparse(args):
     }
 
 
+def run_warmstart_c5(args):
+    """Config 5's learned vs plain init: the network trained on dense uniform
+    instances (N=1024, Q U[-1,1), in the pinned dense layout since the
+    diagonal breaks the complement symmetry; labels from the dense annealer),
+    applied to the N=8192 instance of config 5; time-to-99% of the best known
+    FP64 energy with 4096 replicas: plain init on the config-5 schedule
+    (beta 0.02 -> 2.0), and random, p = 1/2 and learned starts on a colder
+    schedule (beta 0.5 -> 2.0), inference included."""
+    import onesolver_amd
+    from onesolver_amd import warmstart as ws
+
+    ctx = onesolver_amd.Context(0)
+    log(f"device {ctx.device_name()}")
+    t0 = time.perf_counter()
+    qs, labels = [], []
+    for c in range(0, args.c4_train, 64):
+        k = min(64, args.c4_train - c)
+        d = onesolver_amd.DenseBatch(ctx, uniform=(k, 1024, 5, 1000 + c))
+        bs, _ = ws.best_states(d, 64, 256, seed=17, instance=1000 + c, beta=(0.02, 2.0))
+        qs.extend(list(d.q()))
+        labels.extend(list(bs))
+        d.close()
+    t_labels = time.perf_counter() - t0
+    net = onesolver_amd.WarmStartNet(ctx, seed=1)
+    t0 = time.perf_counter()
+    losses = ws.train_sparse(ctx, net, qs, labels, epochs=args.c4_epochs, chunk=16, log=log)
+    t_train = time.perf_counter() - t0
+    del qs
+    log(f"trained on {len(labels)} instances in {t_train:.1f} s (labels {t_labels:.1f} s)")
+    n, R = 8192, args.c5_replicas
+    b = onesolver_amd.DenseBatch(ctx, uniform=(1, n, 0, 0))
+    q = b.q()[0]
+    ws.sparse_marginals_dense(ctx, net, [q[:1024, :1024]])  # warm-up
+    t0 = time.perf_counter()
+    p = ws.sparse_marginals_dense(ctx, net, [q])[0]
+    t_inf = time.perf_counter() - t0
+    del q
+    net.close()
+    ref = b.anneal(R, ws.geometric(0.02, 2.0, 2048), seed=99, instance=0, best_states=False)
+    targets = 0.99 * np.asarray(ref["best_energy"])
+    lad = ws.ladder(1024)[2:]
+    half = np.full((1, n), 0.5, dtype=np.float32)
+    res = {
+        "plain": ws.time_to_target(b, targets, R, lad, (0.02, 2.0), 11, 0),
+        "random_warm": ws.time_to_target(b, targets, R, lad, (0.5, 2.0), 11, 0),
+        "half_warm": ws.time_to_target(b, targets, R, lad, (0.5, 2.0), 11, 0, marginals=half),
+        "learned_warm": ws.time_to_target(b, targets, R, lad, (0.5, 2.0), 11, 0,
+                                          marginals=p[None, :], t_infer=t_inf),
+    }
+    b.close()
+    ctx.close()
+    for k, (rows, hit) in res.items():
+        rungs = " ".join(f"{r['sweeps']}:{r['mean_best']:.0f}" for r in rows)
+        log(f"{k:13s} {rungs} -> " + (f"{hit['wall_s']:.3f} s at {hit['sweeps']} sweeps"
+                                      if hit else "not reached"))
+    hit = {k: v[1] for k, v in res.items()}
+    return {
+        "metric": "time-to-99%-best energy (learned warm start, config 5)",
+        "value": hit["learned_warm"]["wall_s"] if hit["learned_warm"] else None, "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 weights fp32 acc (network); bf16 hi+lo MFMA (SA)",
+        "data": "synthetic (uniform instances generated on the GPU)",
+        "config": {"workload": f"dense uniform N={n}, {R} replicas; network trained on "
+                               f"{len(labels)} uniform N=1024 instances x {args.c4_epochs} "
+                               f"epochs; sweep ladder {lad[0]}..{lad[-1]}", "path": "c5"},
+        "e_ref": float(np.asarray(ref["best_energy"])[0]),
+        "time_to_99_s": {k: (v["wall_s"] if v else None) for k, v in hit.items()},
+        "sweeps_to_99": {k: (v["sweeps"] if v else None) for k, v in hit.items()},
+        "best_by_rung": {k: [r["mean_best"] for r in v[0]] for k, v in res.items()},
+        "infer_s": t_inf, "marginal_confidence": float(np.mean(np.abs(p - 0.5))),
+        "train_s": t_train, "label_s": t_labels,
+        "loss_first_last": [float(np.mean(losses[:10])), float(np.mean(losses[-10:]))],
+    }
+
+
 def run_copy_calibration(args):
     """A plain device-to-device copy of known size (torch's copy kernel), whose
     FETCH_SIZE / WRITE_SIZE counters calibrate the traffic figures of the
@@ -993,8 +1068,9 @@ def main():
     ap.add_argument("--c4-train", type=int, default=4096)
     ap.add_argument("--c4-epochs", type=int, default=2)
     ap.add_argument("--c4-eval", type=int, default=32)
-    ap.add_argument("--c4-path", choices=["dense", "sparse"], default="dense",
-                    help="sparse: the warm start on the config-3 (mixed-sparse) batch")
+    ap.add_argument("--c4-path", choices=["dense", "sparse", "c5"], default="dense",
+                    help="sparse: the warm start on the config-3 (mixed-sparse) batch; c5: "
+                         "learned vs plain init on the config-5 instance")
     ap.add_argument("--sweeps-c3", type=int, default=2048)
     ap.add_argument("--c3-order", choices=["colour", "sequential"], default="colour")
     ap.add_argument("--c3-no-tune", action="store_true",
@@ -1013,8 +1089,9 @@ def main():
         print(json.dumps(run_copy_calibration(args)))
         return
     if args.workload == "c4":
-        print(json.dumps(run_warmstart_c4_sparse(args) if args.c4_path == "sparse"
-                         else run_warmstart_c4(args)))
+        run = {"sparse": run_warmstart_c4_sparse, "c5": run_warmstart_c5}.get(
+            args.c4_path, run_warmstart_c4)
+        print(json.dumps(run(args)))
         return
     if args.workload == "batch":
         print(json.dumps(run_batch_cli(args)))

@@ -180,10 +180,24 @@ def pinned_dense(inst, size: int) -> np.ndarray:
     return q
 
 
+def _spins(item) -> int:
+    """n of a CSR tuple (row_ptr, col, val, diag) or of a dense [n, n] matrix."""
+    return item.shape[0] if isinstance(item, np.ndarray) else len(item[3])
+
+
+def _pinned(item, size: int) -> np.ndarray:
+    if isinstance(item, np.ndarray):
+        q = np.zeros((size, size))
+        n = item.shape[0]
+        q[1:n + 1, 1:n + 1] = item
+        return q
+    return pinned_dense(item, size)
+
+
 def _by_size(insts, chunk: int):
     groups = {}
     for i, inst in enumerate(insts):
-        groups.setdefault(pinned_size(len(inst[3])), []).append(i)
+        groups.setdefault(pinned_size(_spins(inst)), []).append(i)
     for size in sorted(groups):
         ids = groups[size]
         for c in range(0, len(ids), chunk):
@@ -204,16 +218,17 @@ def sparse_labels(ctx: Context, insts, replicas: int = 64, sweeps: int = 512, se
 
 def train_sparse(ctx: Context, net: WarmStartNet, insts, labels, epochs: int = 2,
                  chunk: int = 32, minibatch: int = 8, lr: float = 1e-3, log=None):
-    """Trains `net` on sparse instances with their label states; returns the
-    per-step losses."""
+    """Trains `net` on instances with their label states, in the pinned dense
+    layout; an instance is a CSR tuple (row_ptr, col, val, diag) or a dense
+    [n, n] matrix with a diagonal field.  Returns the per-step losses."""
     losses = []
     for ep in range(epochs):
         for size, ids in _by_size(insts, chunk):
-            d = DenseBatch(ctx, np.stack([pinned_dense(insts[i], size) for i in ids]))
+            d = DenseBatch(ctx, np.stack([_pinned(insts[i], size) for i in ids]))
             y = np.zeros((len(ids), size), dtype=np.uint8)
             y[:, 0] = 1
             for k, i in enumerate(ids):
-                y[k, 1:len(insts[i][3]) + 1] = labels[i]
+                y[k, 1:_spins(insts[i]) + 1] = labels[i]
             for m in range(0, len(ids), minibatch):
                 losses.append(net.train_step(d, y[m:m + minibatch], lr, first=m))
             d.close()
@@ -237,15 +252,16 @@ def sparse_marginals(ctx: Context, net: WarmStartNet, insts, batch: CsrBatch = N
 
 
 def sparse_marginals_dense(ctx: Context, net: WarmStartNet, insts, chunk: int = 32):
-    """The same marginals through the densified instances (the training
-    layout), for checking the sparse inference."""
+    """The same marginals through the pinned dense layout (the training
+    layout): for checking the sparse inference, and for dense instances with
+    a diagonal field (`insts` of [n, n] matrices)."""
     out = [None] * len(insts)
     for size, ids in _by_size(insts, chunk):
-        d = DenseBatch(ctx, np.stack([pinned_dense(insts[i], size) for i in ids]))
+        d = DenseBatch(ctx, np.stack([_pinned(insts[i], size) for i in ids]))
         p = net.predict(d)
         d.close()
         for k, i in enumerate(ids):
-            out[i] = np.ascontiguousarray(p[k, 1:len(insts[i][3]) + 1])
+            out[i] = np.ascontiguousarray(p[k, 1:_spins(insts[i]) + 1])
     return out
 
 
