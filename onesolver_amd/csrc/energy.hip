@@ -2,7 +2,8 @@
 //
 // For states X (R x N, bit-packed rows of W = ceil(N/32) words) and the
 // symmetric FP64 matrix Q, Y = X Q is accumulated in FP64 over LDS tiles
-// (64 states x 64 columns x 16 k), and
+// (128 states x 128 columns x 16 k; 8 x 8 per thread, so every FMA pair
+// costs one LDS read), and
 //   E_r   = 1/2 * sum_j x_rj (Y_rj + Q_jj)           (= sum_{i<=j} Q_ij x_ri x_rj,
 //                                                      annealing.hpp:31-40)
 //   h_rj  = Q_jj + Y_rj - Q_jj x_rj                  (local field of spin j)
@@ -13,9 +14,10 @@
 namespace os {
 
 namespace {
-constexpr int kTR = 64;  // states per tile
-constexpr int kTN = 64;  // columns per tile
-constexpr int kTK = 16;  // k per LDS stage
+constexpr int kTR = 128;  // states per tile
+constexpr int kTN = 128;  // columns per tile
+constexpr int kTK = 16;   // k per LDS stage
+constexpr int kPT = 8;    // rows and columns per thread (16 x 16 threads)
 }  // namespace
 
 __global__ __launch_bounds__(256) void pack_bytes_kernel(const uint8_t *__restrict__ bytes, int n,
@@ -70,11 +72,12 @@ __global__ __launch_bounds__(256) void xq_tile_kernel(int n, int W, const double
     const int c = c0 + tid;
     qdiag[tid] = c < n ? q[(size_t)c * n + c] : 0.0;
   }
-  double acc[4][4];
+  // acc[i][j]: state r0 + ty + 16i, column c0 + tx + 16j
+  double acc[kPT][kPT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < kPT; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < kPT; ++j) acc[i][j] = 0.0;
 
   for (int k0 = 0; k0 < n; k0 += kTK) {
     for (int e = tid; e < kTK * kTN; e += 256) {
@@ -82,37 +85,41 @@ __global__ __launch_bounds__(256) void xq_tile_kernel(int n, int W, const double
       const int gk = k0 + kk, gc = c0 + cc;
       qs[kk][cc] = (gk < n && gc < n) ? q[(size_t)gk * n + gc] : 0.0;
     }
-    for (int e = tid; e < kTK * kTR; e += 256) {
-      const int kk = e % kTK, rr = e / kTK;
-      const int gr = r0 + rr, gk = k0 + kk;
-      double v = 0.0;
-      if (gr < R && gk < n) v = (double)((bits[(size_t)gr * W + (gk >> 5)] >> (gk & 31)) & 1u);
-      xs[kk][rr] = v;
+    {
+      // the 16 bits k0 .. k0+15 of a state are in one word (k0 is a multiple
+      // of 16): two threads per state, one load and 8 bits each
+      static_assert(kTK == 16 && 2 * kTR == 256, "x staging: two threads per state row");
+      const int rr = tid >> 1, k8 = 8 * (tid & 1);
+      const int gr = r0 + rr;
+      const uint32_t wd = gr < R ? bits[(size_t)gr * W + (k0 >> 5)] >> (k0 & 31) : 0u;
+#pragma unroll
+      for (int kk = 0; kk < 8; ++kk)
+        xs[k8 + kk][rr] = k0 + k8 + kk < n ? (double)((wd >> (k8 + kk)) & 1u) : 0.0;
     }
     __syncthreads();
-#pragma unroll 4
+#pragma unroll 2
     for (int kk = 0; kk < kTK; ++kk) {
-      double a[4], b[4];
+      double a[kPT], b[kPT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = xs[kk][ty + 16 * i];
+      for (int i = 0; i < kPT; ++i) a[i] = xs[kk][ty + 16 * i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = qs[kk][tx + 16 * j];
+      for (int j = 0; j < kPT; ++j) b[j] = qs[kk][tx + 16 * j];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < kPT; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < kPT; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
     }
     __syncthreads();
   }
 
   float dmax = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < kPT; ++i) {
     const int r = r0 + ty + 16 * i;
     double s = 0.0;
     if (r < R) {
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < kPT; ++j) {
         const int c = c0 + tx + 16 * j;
         if (c < n) {
           const uint32_t xb = (bits[(size_t)r * W + (c >> 5)] >> (c & 31)) & 1u;
