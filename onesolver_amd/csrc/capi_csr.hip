@@ -367,12 +367,18 @@ Shape select_shape(CsrBatch &b, const CsrRun &run, int site_order, bool tune_it)
     if (tune_it) {
       tune_col(b, run);
     } else {
+      // untuned: the shape the tuner picks for the mixed config-3 batch
+      // (at most 256 steps per class sweep, 2 waves per group);
+      // OS_CSR_COLOUR_LIMIT overrides the step limit
+      const char *lenv = std::getenv("OS_CSR_COLOUR_LIMIT");
+      const int lim = lenv ? std::atoi(lenv) : 256;
+      OS_REQUIRE(!lenv || lim > 0, "OS_CSR_COLOUR_LIMIT must be a positive step count");
       b.c_uniform = 0;
-      b.c_limit = 128;
-      b.c_waves = 1;
+      b.c_limit = lim;
+      b.c_waves = 2;
       col_items(b, b.c_uniform, b.c_limit);
       b.c_groups = std::max(
-          1, std::min(c.num_cus * csr_col_max_groups_per_cu(1, b.c_plane_words), b.num_items_c));
+          1, std::min(c.num_cus * csr_col_max_groups_per_cu(2, b.c_plane_words), b.num_items_c));
     }
   }
   if (b.mode == 1) col_items(b, b.c_uniform, b.c_limit);

@@ -229,10 +229,12 @@ def test_csr_colour_classes_match_oracle(ctx, monkeypatch, sites):
     b.close()
 
 
-def test_csr_colour_mixed_sites_per_step_match_oracle(ctx):
-    """The default policy (no tuning: at most 128 steps per class sweep)
-    gives small instances 4 sites per step, mid-size 8 and large 16, in one
-    launch; every instance still follows the oracle's class-order chains."""
+def test_csr_colour_mixed_sites_per_step_match_oracle(ctx, monkeypatch):
+    """The untuned policy with at most 128 steps per class sweep
+    (OS_CSR_COLOUR_LIMIT) gives small instances 4 sites per step, mid-size 8
+    and large 16, in one launch; every instance still follows the oracle's
+    class-order chains."""
+    monkeypatch.setenv("OS_CSR_COLOUR_LIMIT", "128")
     insts = [instances.regular_csr(n, d, seed=3, instance=i)
              for i, (n, d) in enumerate([(64, 3), (700, 5), (1500, 8), (300, 4), (2048, 3)])]
     insts.append(_irregular(70, 1))
