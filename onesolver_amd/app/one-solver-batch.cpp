@@ -30,6 +30,7 @@
  * device), as one-solver-anneal.
  */
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <dirent.h>
@@ -40,6 +41,7 @@
 #include <stdexcept>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
 #include <vector>
 
 #include "onesolver.h"
@@ -161,38 +163,62 @@ void flatten(const Instance &in, double *q) {
   }
 }
 
-// Symmetric off-diagonal rows sorted by column, and the diagonal.
+// Symmetric off-diagonal rows sorted by column, and the diagonal.  The
+// entries are bucketed by row (counting sort), then each row is sorted by
+// column, stably: O(nnz) for rows of a few entries.
 void to_csr(const Instance &in, std::vector<int32_t> &rp, std::vector<int32_t> &col,
             std::vector<double> &val, std::vector<double> &diag) {
   const int32_t n = in.n;
-  std::vector<int32_t> deg(n, 0);
+  std::vector<int32_t> start(n + 1, 0);
   std::vector<double> d(n, 0.0);
-  std::vector<std::pair<int64_t, double>> e;
-  e.reserve(2 * (size_t)in.nq);
+  for (size_t k = 0; k < in.ti.size(); ++k)
+    if (in.ti[k] != in.tj[k]) {
+      ++start[in.ti[k] + 1];
+      ++start[in.tj[k] + 1];
+    }
+  for (int32_t i = 0; i < n; ++i) start[i + 1] += start[i];
+  const size_t base = col.size();
+  col.resize(base + start[n]);
+  val.resize(base + start[n]);
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
   for (size_t k = 0; k < in.ti.size(); ++k) {
     const int32_t i = in.ti[k], j = in.tj[k];
     if (i == j) {
       d[i] = in.tv[k];
     } else {
-      e.push_back({(int64_t)i * n + j, in.tv[k]});
-      e.push_back({(int64_t)j * n + i, in.tv[k]});
+      const size_t a = base + fill[i]++, b = base + fill[j]++;
+      col[a] = j;
+      val[a] = in.tv[k];
+      col[b] = i;
+      val[b] = in.tv[k];
     }
   }
-  std::sort(e.begin(), e.end(),
-            [](const std::pair<int64_t, double> &a, const std::pair<int64_t, double> &b) {
-              return a.first < b.first;
-            });
-  for (const auto &x : e) ++deg[x.first / n];
-  int32_t run = 0;
+  std::vector<std::pair<int32_t, double>> row;
   for (int32_t i = 0; i < n; ++i) {
-    rp.push_back(run);
-    run += deg[i];
+    rp.push_back(start[i]);
+    int32_t *c = col.data() + base + start[i];
+    double *v = val.data() + base + start[i];
+    const int32_t len = start[i + 1] - start[i];
+    if (len <= 16) {
+      for (int32_t x = 1; x < len; ++x)
+        for (int32_t y = x; y > 0 && c[y - 1] > c[y]; --y) {
+          std::swap(c[y - 1], c[y]);
+          std::swap(v[y - 1], v[y]);
+        }
+    } else {
+      row.clear();
+      for (int32_t x = 0; x < len; ++x) row.push_back({c[x], v[x]});
+      std::stable_sort(row.begin(), row.end(),
+                       [](const std::pair<int32_t, double> &p, const std::pair<int32_t, double> &q) {
+                         return p.first < q.first;
+                       });
+      for (int32_t x = 0; x < len; ++x) {
+        c[x] = row[x].first;
+        v[x] = row[x].second;
+      }
+    }
   }
-  rp.push_back(run);
-  for (const auto &x : e) {
-    col.push_back((int32_t)(x.first % n));
-    val.push_back(x.second);
-  }
+  rp.push_back(start[n]);
   diag.insert(diag.end(), d.begin(), d.end());
 }
 
@@ -270,8 +296,40 @@ int main(int argc, char *argv[]) {
       return 1;
     }
     auto t0 = clock::now();
-    std::vector<Instance> insts;
-    for (const auto &f : files) insts.push_back(load(ctx, f));
+    // Files are parsed by up to kParseThreads host threads, each on a
+    // context (stream) of its own, so that one file's launches and
+    // read-backs overlap another's.  A failure stops further files from
+    // being taken; every file before it was taken already, so the error
+    // reported is the first in file order, as in a serial run.
+    std::vector<Instance> insts(files.size());
+    {
+      constexpr size_t kParseThreads = 4;
+      std::vector<std::string> err(files.size());
+      std::atomic<size_t> next{0};
+      std::atomic<bool> failed{false};
+      auto work = [&](os_ctx *c) {
+        for (size_t k; !failed && (k = next++) < files.size();) {
+          try {
+            insts[k] = load(c, files[k]);
+          } catch (std::exception &e) {
+            err[k] = e.what();
+            failed = true;
+          }
+        }
+      };
+      std::vector<std::thread> pool;
+      for (size_t t = 1; t < std::min(kParseThreads, files.size()); ++t)
+        pool.emplace_back([&] {
+          os_ctx *c = nullptr;
+          if (os_ctx_create(0, &c) != OS_OK) return;  // the other threads take its files
+          work(c);
+          os_ctx_destroy(c);
+        });
+      work(ctx);
+      for (auto &th : pool) th.join();
+      for (const auto &e : err)
+        if (!e.empty()) throw std::runtime_error(e);
+    }
     const double parse_ms = ms_since(t0);
     t0 = clock::now();
 
