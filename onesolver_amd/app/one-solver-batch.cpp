@@ -37,6 +37,7 @@
 #include <fstream>
 #include <iostream>
 #include <iterator>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -408,17 +409,17 @@ int main(int argc, char *argv[]) {
       size_t tot = 0;
       for (int32_t n : ns) tot += (size_t)n * R;
       std::vector<double> be((size_t)I * R);
-      std::vector<uint8_t> bs(tot);
+      std::unique_ptr<uint8_t[]> bs(new uint8_t[tot]);  // filled by the call
       std::vector<double> ib(I);
       std::vector<int32_t> ir(I);
-      const int rc = os_csr_batch_anneal(b, &p, be.data(), nullptr, bs.data(), nullptr, ib.data(),
+      const int rc = os_csr_batch_anneal(b, &p, be.data(), nullptr, bs.get(), nullptr, ib.data(),
                                          ir.data(), nullptr);
       os_csr_batch_destroy(b);
       check(rc, "sparse anneal");
       size_t off = 0;
       for (int32_t k = 0; k < I; ++k) {
         best_e[k] = ib[k];
-        const uint8_t *x = bs.data() + off + (size_t)ir[k] * ns[k];
+        const uint8_t *x = bs.get() + off + (size_t)ir[k] * ns[k];
         best_x[k].assign(x, x + ns[k]);
         off += (size_t)ns[k] * R;
       }

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "../../include/onesolver.h"
@@ -452,7 +453,6 @@ void collect(CsrBatch &b, int R, DevBuf &best, DevBuf &fin, DevBuf *eb, double *
                                 hipMemcpyDeviceToHost, s));
   }
   OS_HIP_CHECK(hipStreamSynchronize(s));
-  uint8_t *bs = best_states, *fs = final_states;
   for (int i = 0; i < b.I; ++i) {
     const auto &d = b.inst[i];
     int arg = 0;
@@ -464,16 +464,49 @@ void collect(CsrBatch &b, int R, DevBuf &best, DevBuf &fin, DevBuf *eb, double *
     }
     if (inst_best_energy) inst_best_energy[i] = e64[(size_t)d.rep_off + arg];
     if (inst_best_replica) inst_best_replica[i] = arg;
-    if (bs || fs)
-      for (int r = 0; r < R; ++r)
-        for (int j = 0; j < d.n; ++j) {
-          const size_t at = (size_t)d.state_off + (size_t)(j >> 5) * Rp + r;
-          if (bs) bs[(size_t)r * d.n + j] = (uint8_t)((bp[at] >> (j & 31)) & 1u);
-          if (fs) fs[(size_t)r * d.n + j] = (uint8_t)((fp[at] >> (j & 31)) & 1u);
-        }
-    if (bs) bs += (size_t)R * d.n;
-    if (fs) fs += (size_t)R * d.n;
   }
+  if (!best_states && !final_states) return;
+  // planes [words][Rp] -> bytes [R][n] per instance: word-major, so the
+  // planes are read in order and each word fills a run of up to 32 bytes;
+  // instances are split over a few host threads for large batches
+  std::vector<size_t> out_off(b.I + 1, 0);
+  for (int i = 0; i < b.I; ++i) out_off[i + 1] = out_off[i] + (size_t)R * b.inst[i].n;
+  auto unpack = [&](const std::vector<uint32_t> &pl, uint8_t *dst, int i0, int i1) {
+    for (int i = i0; i < i1; ++i) {
+      const auto &d = b.inst[i];
+      uint8_t *o = dst + out_off[i];
+      for (int w = 0; w * 32 < d.n; ++w) {
+        const uint32_t *src = pl.data() + (size_t)d.state_off + (size_t)w * Rp;
+        const int j0 = w * 32, len = std::min(32, d.n - j0);
+        for (int r = 0; r < R; ++r) {
+          const uint32_t x = src[r];
+          uint8_t *row = o + (size_t)r * d.n + j0;
+          for (int k = 0; k < len; ++k) row[k] = (uint8_t)((x >> k) & 1u);
+        }
+      }
+    }
+  };
+  auto run = [&](int i0, int i1) {
+    if (best_states) unpack(bp, best_states, i0, i1);
+    if (final_states) unpack(fp, final_states, i0, i1);
+  };
+  const size_t bytes = out_off[b.I] * ((best_states ? 1 : 0) + (final_states ? 1 : 0));
+  const int T = bytes < ((size_t)8 << 20) ? 1 : std::min(8, std::max(1, b.I));
+  if (T == 1) {
+    run(0, b.I);
+    return;
+  }
+  // contiguous instance ranges of about equal output size
+  std::vector<int> cut(T + 1, b.I);
+  cut[0] = 0;
+  for (int t = 1, i = 0; t < T; ++t) {
+    while (i < b.I && out_off[i] < out_off[b.I] * t / T) ++i;
+    cut[t] = i;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(run, cut[t], cut[t + 1]);
+  run(cut[0], cut[1]);
+  for (auto &th : pool) th.join();
 }
 
 }  // namespace
