@@ -858,10 +858,10 @@ def run_batch_cli(args):
     from onesolver_amd import instances
     insts = instances.batch_c3(512)
     R, S = 256, args.sweeps_c3
-    tmp = tempfile.mkdtemp(prefix="osbatch_")
+    tmp = args.batch_dir or tempfile.mkdtemp(prefix="osbatch_")
     inp, out = os.path.join(tmp, "in"), os.path.join(tmp, "out")
-    os.makedirs(inp)
-    os.makedirs(out)
+    os.makedirs(inp, exist_ok=True)
+    os.makedirs(out, exist_ok=True)
     nbytes = 0
     for k, (rp, cl, vl, dg) in enumerate(insts):
         n = len(dg)
@@ -1073,6 +1073,9 @@ def main():
                          "learned vs plain init on the config-5 instance")
     ap.add_argument("--sweeps-c3", type=int, default=2048)
     ap.add_argument("--c3-order", choices=["colour", "sequential"], default="colour")
+    ap.add_argument("--batch-dir", default="",
+                    help="batch workload: write the .qubo files to DIR/in and the CSVs to "
+                         "DIR/out and keep them (default: a temporary directory)")
     ap.add_argument("--c3-no-tune", action="store_true",
                     help="skip the launch-shape tuner (library default shape)")
     ap.add_argument("--n", type=int, default=1024)
