@@ -249,7 +249,12 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       d_rep = d_slot + slot_of.size();
       d_ts = reinterpret_cast<float *>(d_rep + rep_at.size());
     }
-    d.fields.ensure(sizeof(float) * T * np * (d.stream ? 2 : 1));
+    // stream path: pipeline depth P, P field copies (OS_STREAM_DEPTH
+    // overrides the default, for tuning)
+    const char *denv = std::getenv("OS_STREAM_DEPTH");
+    const int P = denv ? std::min(os::Dense::kMaxDepth, std::max(2, std::atoi(denv)))
+                       : os::Dense::kDepth;
+    d.fields.ensure(sizeof(float) * T * np * (d.stream ? P : 1));
     d.xbits.ensure(sizeof(uint32_t) * T * npw);
     d.bestbits.ensure(sizeof(uint32_t) * T * npw);
     d.energy.ensure(sizeof(double) * T);
@@ -333,14 +338,14 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
                                 hipMemcpyDeviceToDevice, st));
     OS_HIP_CHECK(hipMemsetAsync(d.drift.ptr, 0, sizeof(unsigned int), st));
     int launches = 0;
-    // stream path: the up-to-date field copy (0 or 1)
+    // stream path: the up-to-date field copy (0 .. P-1)
     int cur = 0;
-    auto fbuf = [&](int i) { return d.fields.as<float>() + (size_t)(i & 1) * T * np; };
+    auto fbuf = [&](int i) { return d.fields.as<float>() + (size_t)(i % P) * T * np; };
     // blocks per field pass: OS_STREAM_GROUP overrides the default (tuning)
     const char *genv = std::getenv("OS_STREAM_GROUP");
     d.group = genv ? std::min(16, std::max(1, std::atoi(genv)))
                    : (d.tuned_group ? d.tuned_group : os::Dense::kGroup);
-    const int m = d.group, ring = 3 * d.group;
+    const int m = d.group, ring = (P + 1) * d.group;
     // decide launches: operand chunking (OS_STREAM_DECIDE_VARIANT overrides
     // the default, for tuning)
     const char *venv = std::getenv("OS_STREAM_DECIDE_VARIANT");
@@ -376,8 +381,10 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       d.dblk.ensure(sizeof(uint4) * ring * T);
       if (!d.side) {
         OS_HIP_CHECK(hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking));
-        for (hipEvent_t *e : {&d.ev_seg, &d.ev_dec, &d.ev_pass[0], &d.ev_pass[1]})
+        for (hipEvent_t *e : {&d.ev_seg, &d.ev_dec})
           OS_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        for (hipEvent_t &e : d.ev_pass)
+          OS_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       }
     }
     for (int s0 = 0; s0 < S; s0 += seg) {
@@ -390,13 +397,17 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
       OS_HIP_CHECK(hipEventRecord(a, st));
       if (d.stream) {
         // Block g (0 .. steps-1) of group H = g / m is decided on d.side from
-        // the fields after pass H-2 plus the flips of blocks m(H-1) .. g-1;
-        // pass H (context's stream) adds the flips of group H to the fields
-        // after pass H-1, in the other copy, once the group is decided.  So
-        // the passes run beside the decisions of the next group.  Pass H
-        // lands in copy cur + H + 1; the flip masks of block g stay in the
-        // ring slot g % 3m until every reader (the decides of groups H and
-        // H+1, pass H) is done: block g + 3m is decided after pass H+1.
+        // the fields after pass H-P plus the flips of blocks m(H-P+1) .. g-1
+        // (at most Pm - 1 of them); pass H (context's stream) adds the flips
+        // of group H to the fields after pass H-1, in the next copy, once the
+        // group is decided.  So pass H runs beside the decisions of groups
+        // H+1 .. H+P-1 (with P >= 3 the two cross-stream hand-offs of a pass
+        // leave the decisions' path, but the decides get slower by more than
+        // they save).  Pass H lands in copy
+        // cur + H + 1 (mod P), the copy the decides of group H read: it waits
+        // for them.  The flip masks of block g stay in ring slot
+        // g % ((P+1)m) until every reader (the decides of groups H .. H+P-1,
+        // pass H) is done: block g + (P+1)m is decided after pass H+1.
         const int nb = np / os::stream_block();
         uint4 *ring_d = d.dblk.as<uint4>();
         const int steps = ns * nb;
@@ -404,11 +415,12 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
         OS_HIP_CHECK(hipStreamWaitEvent(d.side, d.ev_seg, 0));
         for (int g = 0; g < steps; ++g) {
           const int H = g / m, sw = s0 + g / nb, blk = g % nb;
-          if (g % m == 0 && H >= 2) OS_HIP_CHECK(hipStreamWaitEvent(d.side, d.ev_pass[H & 1], 0));
-          const int gb = std::max(0, m * (H - 1));
+          if (g % m == 0 && H >= P)
+            OS_HIP_CHECK(hipStreamWaitEvent(d.side, d.ev_pass[(H - P) % P], 0));
+          const int gb = std::max(0, m * (H - P + 1));
           os::launch_stream_decide(d.side, I, n, np, Rp, g, gb, ring, sw, p->tscale[sw], p->seed,
                                    p->instance, blk == nb - 1, d.qpack.as<uint4>(),
-                                   d.qdb.as<float>(), fbuf(cur + std::max(0, H - 1)),
+                                   d.qdb.as<float>(), fbuf(cur + std::max(0, H - P + 1)),
                                    d.xbits.as<uint32_t>(), ring_d, d.energy.as<double>(),
                                    d.ebest.as<double>(), d.bestbits.as<uint32_t>(), dvar);
           ++launches;
@@ -422,12 +434,12 @@ int dense_run(os_dense *d_, const os_sweep_params *p, const Exchange *xc, const 
                                    ring_d);
             if (timing) OS_HIP_CHECK(hipEventRecord(d.pass_ev[2 * timed++ + 1], st));
             ++passes;
-            OS_HIP_CHECK(hipEventRecord(d.ev_pass[H & 1], st));
+            OS_HIP_CHECK(hipEventRecord(d.ev_pass[H % P], st));
           }
         }
         // the last pass waited for the last decide: the context's stream
         // holds every launch of the segment again
-        if (steps > 0) cur = (cur + (steps - 1) / m + 1) & 1;
+        if (steps > 0) cur = (cur + (steps - 1) / m + 1) % P;
       } else {
         os::launch_dense_sweeps(st, n, np, I, d.qpack.as<uint4>(), d.qdb.as<float>(),
                                 d.tscale.as<float>() + s0, xc ? d.tsr.as<float>() : nullptr, s0,

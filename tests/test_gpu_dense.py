@@ -135,11 +135,42 @@ def test_dense_streaming_pass_grouping(ctx, monkeypatch, group, refresh):
         np.testing.assert_allclose(out["best_energies"][i], oracle.energy(qs[i], bs), rtol=1e-12)
 
 
+@pytest.mark.parametrize("depth,group", [(2, 1), (2, 5), (3, 1), (3, 4), (4, 3)])
+def test_dense_streaming_pipeline_depth(ctx, monkeypatch, depth, group):
+    """Pipeline depth P (OS_STREAM_DEPTH): the decides of group H read the
+    fields after pass H-P plus up to P*group - 1 pending blocks, from P field
+    copies.  Every depth gives the oracle's chains on a two-instance integer
+    batch, and the same bits on a real-valued SK instance (every field is the
+    same fp32 sum in the same block order, whichever launch added a block)."""
+    n, R, S = 1100, 64, 3
+    qs = np.stack([_int_instance(n, 15), _int_instance(n, 16)])
+    ts = _tscale(S)
+    monkeypatch.setenv("OS_STREAM_GROUP", str(group))
+    monkeypatch.setenv("OS_STREAM_DEPTH", str(depth))
+    print(f"[gpu] dense streaming depth={depth} group={group}", flush=True)
+    b = onesolver_amd.DenseBatch(ctx, qs)
+    out = b.anneal(R, ts, seed=2, instance=6, final_states=True)
+    b.close()
+    for i in range(2):
+        fin, bs, be = oracle.sweep_dense(qs[i], n, ts, R, seed=2, instance=6 + i)
+        assert (out["final_states"][i] == fin).all()
+        assert (out["best_states"][i] == bs).all()
+    sk = onesolver_amd.DenseBatch(ctx, sk=(1, 1536, 4, 0))
+    got = sk.anneal(R, ts, seed=8, final_states=True)
+    monkeypatch.setenv("OS_STREAM_DEPTH", "2")
+    monkeypatch.setenv("OS_STREAM_GROUP", "1")
+    ref = sk.anneal(R, ts, seed=8, final_states=True)
+    sk.close()
+    for k in ("final_states", "best_states", "best_energies"):
+        assert (got[k] == ref[k]).all(), k
+
+
 @pytest.mark.parametrize("variant", [20, 21, 30, 31, 40, 50])
 def test_dense_streaming_decide_chunking(ctx, monkeypatch, variant):
     """Every operand chunking of the decide launches (OS_STREAM_DECIDE_VARIANT:
     blocks per round trip, next chunk requested early or not) gives the
-    oracle's chains; 7 blocks per pass leave up to 13 pending blocks."""
+    oracle's chains; 7 blocks per pass leave up to 13 pending blocks at the
+    default depth of 2."""
     n, R, S = 1100, 64, 3
     q = _int_instance(n, 8)
     ts = _tscale(S)
