@@ -1,10 +1,11 @@
 // anneal_stream.hip -- sweep-mode annealing of large dense QUBOs (n > 1024).
 //
 // When R x N fp32 fields no longer fit the register file (N=8192, R=4096:
-// 128 MiB), they live in HBM, in two copies.  Spins are decided in blocks of
-// k = 64; block g of a segment covers spins 64*(g mod NB).. of sweep g / NB
-// (NB = N/64), and its flips are kept as +/- bit masks in a ring of 3m
-// slots.  Two kinds of launch run side by side on two streams:
+// 128 MiB), they live in HBM, in P copies (P = 2 by default).  Spins are
+// decided in blocks of k = 64; block g of a segment covers spins
+// 64*(g mod NB).. of sweep g / NB (NB = N/64), and its flips are kept as +/-
+// bit masks in a ring of (P+1)m slots.  Two kinds of launch run side by side
+// on two streams:
 //  - decide (one work-group per 64 replicas): reads the 64x64 field tile of
 //    block g from the last field copy, adds the flips of the blocks not yet
 //    in that copy (bf16 hi + lo v_mfma_f32_32x32x16_bf16, fp32 accumulation),
@@ -14,9 +15,9 @@
 //  - pass (persistent, one work-group per CU by default): adds the flips of a
 //    group of m blocks to every tile of the field array, from one copy into
 //    the other -- the field array moves through HBM once per m blocks.
-// Decide of block g in group H reads the copy after pass H-2, so pass H runs
-// while group H+1 is decided (the host side, capi_dense.hip, orders them
-// with events).  Thresholds are the sweep-mode ones of the register-resident
+// Decide of block g in group H reads the copy after pass H-P, so pass H runs
+// while groups H+1 .. H+P-1 are decided (the host side, capi_dense.hip,
+// orders them with events).  Thresholds are the sweep-mode ones of the register-resident
 // kernel: flip iff dE < neg_ln_u(w) * tscale[t], w = Philox word of counter
 // (j>>2, replica, sweep, tag).
 #include <algorithm>
@@ -100,6 +101,62 @@ __device__ __forceinline__ void ops_apply(sf32x16 &C, const TileOps<K> &o, int c
         const uint4 a = spins8(pw, mw, 16 * (ks & 1) + 8 * kh);
         C = smfma(a, o.B[t][ks], C);
         C = smfma(a, o.B[t][4 + ks], C);
+      }
+    }
+  }
+}
+
+// Operands of up to K blocks for both 32-replica halves of one column half
+// (shared-operand decide): the flip masks of replicas lc and 32 + lc and the
+// Q rows of the wave's columns, loaded once for the two tiles.
+template <int K>
+struct TileOps2 {
+  uint4 dd[K][2];
+  uint4 B[K][8];
+};
+
+template <int K>
+__device__ __forceinline__ void ops_load2(TileOps2<K> &o, int g0, int cnt, int NB, int ring,
+                                          size_t T, size_t drow, const uint4 *qcol, int np,
+                                          const uint4 *__restrict__ dring, int kh) {
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    if (t < cnt) {
+      const int g = g0 + t;
+      const uint4 *dr = dring + (size_t)(g % ring) * T + drow;
+      o.dd[t][0] = dr[0];
+      o.dd[t][1] = dr[32];
+      const uint4 *qb = qcol + (size_t)(g % NB) * np * 16;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        o.B[t][k] = qb[2 * k + kh];
+        o.B[t][4 + k] = qb[8 + 2 * k + kh];
+      }
+    }
+  }
+}
+
+// C0 / C1 += D_g * Qhi + D_g * Qlo for the (up to K) blocks in `o`, in order:
+// per tile the same sequence of accumulations as ops_apply.
+template <int K>
+__device__ __forceinline__ void ops_apply2(sf32x16 &C0, sf32x16 &C1, const TileOps2<K> &o,
+                                           int cnt, int kh) {
+#pragma unroll
+  for (int t = 0; t < K; ++t) {
+    if (t < cnt) {
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        uint4 a[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t pw = ks < 2 ? o.dd[t][h].x : o.dd[t][h].y;
+          const uint32_t mw = ks < 2 ? o.dd[t][h].z : o.dd[t][h].w;
+          a[h] = spins8(pw, mw, 16 * (ks & 1) + 8 * kh);
+        }
+        C0 = smfma(a[0], o.B[t][ks], C0);
+        C1 = smfma(a[1], o.B[t][ks], C1);
+        C0 = smfma(a[0], o.B[t][4 + ks], C0);
+        C1 = smfma(a[1], o.B[t][4 + ks], C1);
       }
     }
   }
@@ -241,8 +298,11 @@ __global__ __launch_bounds__(256) void stream_pass_kernel(int np, int Rp, int I,
 // launch and is on the critical path.
 //
 // KC: pending blocks whose operands load in one round trip; PIPE: the next
-// chunk's operands are requested before the current chunk is applied.
-template <int KC, bool PIPE>
+// chunk's operands are requested before the current chunk is applied; SH:
+// waves 0 and 1 each add the pending flips to both replica halves of their
+// column half (one operand load for the two tiles), while waves 2 and 3 make
+// all the thresholds.
+template <int KC, bool PIPE, bool SH>
 __global__ __launch_bounds__(256) void stream_decide_kernel(
     int n, int np, int Rp, int g, int gb, int ring, size_t T, int sweep, float ts, uint32_t k0,
     uint32_t k1, uint32_t inst_base, int last_of_sweep, const uint4 *__restrict__ qpack,
@@ -269,18 +329,15 @@ __global__ __launch_bounds__(256) void stream_decide_kernel(
   for (int i = 0; i < 4; ++i) qv[i] = qsrc[tid + 256 * i];
   const size_t off = (row0 + 32 * mt + 4 * kh) * np + b * kSB + 32 * nt + lc;
   const uint4 *qcol = qpack + ((size_t)s * NB * np + b * kSB + 32 * nt + lc) * 16;
-  sf32x16 C = tile_load(Fin + off, np);
-  TileOps<KC> o;
-  ops_load(o, gb, cnt, np, NB, ring, T, drow, qcol, dring, kh);
   const int rep = 16 * w + (lane >> 2), r4 = lane & 3;
   const size_t rr = row0 + rep;
   const uint2 xw0 = *reinterpret_cast<const uint2 *>(xbits + rr * npw + 2 * b);
   const double e_old = energy[rr];
   const double e_best = last_of_sweep ? ebest[rr] : 0.0;
-  // thresholds: 64 replicas x 16 Philox calls (4 spins each), 4 per thread
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int it = tid + 256 * i, r = it >> 4, q4 = it & 15;
+  // thresholds: 64 replicas x 16 Philox calls (4 spins each); item it
+  // (0 .. 1023) is replica it / 16, spins 4 (it % 16) ..
+  auto threshold4 = [&](int it) {
+    const int r = it >> 4, q4 = it & 15;
     const int j0 = b * kSB + 4 * q4;
     const uint4 ph = philox4x32_10(make_uint4((uint32_t)(j0 >> 2), (uint32_t)(ry * 64 + r),
                                               (uint32_t)sweep, tag), k0, k1);
@@ -289,26 +346,53 @@ __global__ __launch_bounds__(256) void stream_decide_kernel(
       const float x = neg_ln_u(pick(ph, jj)) * ts;
       thr[r][4 * q4 + jj] = j0 + jj < n ? x : -INFINITY;
     }
-  }
-  if (PIPE) {
-    TileOps<KC> o2;
-    for (int c0 = 0; c0 < cnt; c0 += KC) {
-      if (c0 + KC < cnt) ops_load(o2, gb + c0 + KC, cnt - c0 - KC, np, NB, ring, T, drow, qcol,
-                                  dring, kh);
-      ops_apply(C, o, cnt - c0, kh);
-      o = o2;
+  };
+  if (SH) {
+    if (mt == 0) {
+      // both replica halves of column half nt (offset 32 rows for the second)
+      sf32x16 C0 = tile_load(Fin + off, np), C1 = tile_load(Fin + off + (size_t)32 * np, np);
+      TileOps2<KC> o;
+      for (int c0 = 0; c0 < cnt; c0 += KC) {
+        ops_load2(o, gb + c0, cnt - c0, NB, ring, T, row0 + lc, qcol, np, dring, kh);
+        ops_apply2(C0, C1, o, cnt - c0, kh);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        *reinterpret_cast<float4 *>(&fl[32 * nt + lc][8 * q + 4 * kh]) =
+            make_float4(C0[4 * q], C0[4 * q + 1], C0[4 * q + 2], C0[4 * q + 3]);
+        *reinterpret_cast<float4 *>(&fl[32 * nt + lc][32 + 8 * q + 4 * kh]) =
+            make_float4(C1[4 * q], C1[4 * q + 1], C1[4 * q + 2], C1[4 * q + 3]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) threshold4(tid - 128 + 128 * i);
     }
   } else {
-    ops_apply(C, o, cnt, kh);
-    for (int c0 = KC; c0 < cnt; c0 += KC) {
-      ops_load(o, gb + c0, cnt - c0, np, NB, ring, T, drow, qcol, dring, kh);
-      ops_apply(C, o, cnt - c0, kh);
-    }
-  }
+    sf32x16 C = tile_load(Fin + off, np);
+    TileOps<KC> o;
+    ops_load(o, gb, cnt, np, NB, ring, T, drow, qcol, dring, kh);
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
-    *reinterpret_cast<float4 *>(&fl[32 * nt + lc][32 * mt + 8 * q + 4 * kh]) =
-        make_float4(C[4 * q], C[4 * q + 1], C[4 * q + 2], C[4 * q + 3]);
+    for (int i = 0; i < 4; ++i) threshold4(tid + 256 * i);
+    if (PIPE) {
+      TileOps<KC> o2;
+      for (int c0 = 0; c0 < cnt; c0 += KC) {
+        if (c0 + KC < cnt) ops_load(o2, gb + c0 + KC, cnt - c0 - KC, np, NB, ring, T, drow, qcol,
+                                    dring, kh);
+        ops_apply(C, o, cnt - c0, kh);
+        o = o2;
+      }
+    } else {
+      ops_apply(C, o, cnt, kh);
+      for (int c0 = KC; c0 < cnt; c0 += KC) {
+        ops_load(o, gb + c0, cnt - c0, np, NB, ring, T, drow, qcol, dring, kh);
+        ops_apply(C, o, cnt - c0, kh);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<float4 *>(&fl[32 * nt + lc][32 * mt + 8 * q + 4 * kh]) =
+          make_float4(C[4 * q], C[4 * q + 1], C[4 * q + 2], C[4 * q + 3]);
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int idx = tid + 256 * i, j = idx >> 4, c4 = idx & 15;  // Q[j][4*c4 .. 4*c4+3]
@@ -468,17 +552,20 @@ void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, i
   const size_t T = (size_t)I * Rp;
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   const int lo = last_of_sweep ? 1 : 0;
-#define OS_DECIDE(KC, PIPE)                                                                   \
-  stream_decide_kernel<KC, PIPE><<<grid, 256, 0, st>>>(n, np, Rp, g, gb, ring, T, sweep, ts, k0, \
-                                                       k1, inst_base, lo, qpack, qdb, Fin, xbits, \
-                                                       dring, energy, ebest, bestbits)
+#define OS_DECIDE(KC, PIPE, SH)                                                             \
+  stream_decide_kernel<KC, PIPE, SH><<<grid, 256, 0, st>>>(                                  \
+      n, np, Rp, g, gb, ring, T, sweep, ts, k0, k1, inst_base, lo, qpack, qdb, Fin, xbits,   \
+      dring, energy, ebest, bestbits)
   switch (variant) {
-    case 21: OS_DECIDE(2, true); break;
-    case 30: OS_DECIDE(3, false); break;
-    case 31: OS_DECIDE(3, true); break;
-    case 40: OS_DECIDE(4, false); break;
-    case 50: OS_DECIDE(5, false); break;
-    default: OS_DECIDE(2, false); break;
+    case 21: OS_DECIDE(2, true, false); break;
+    case 30: OS_DECIDE(3, false, false); break;
+    case 31: OS_DECIDE(3, true, false); break;
+    case 40: OS_DECIDE(4, false, false); break;
+    case 50: OS_DECIDE(5, false, false); break;
+    case 60: OS_DECIDE(2, false, true); break;
+    case 61: OS_DECIDE(1, false, true); break;
+    case 62: OS_DECIDE(3, false, true); break;
+    default: OS_DECIDE(2, false, false); break;
   }
 #undef OS_DECIDE
   OS_HIP_CHECK(hipGetLastError());

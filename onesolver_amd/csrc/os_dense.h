@@ -33,7 +33,7 @@ struct Dense {
   static constexpr int kGroup = 5;      // default blocks per field pass
   static constexpr int kPassChunk = 2;  // default operand blocks per round trip (pass)
   static constexpr int kPassWgsPerCu = 1;  // default persistent pass work-groups per CU (room for decides)
-  static constexpr int kDecideVariant = 20;  // default decide operand chunking (launch_stream_decide)
+  static constexpr int kDecideVariant = 61;  // default decide operand chunking (launch_stream_decide)
   // pipeline depth P: the decides of group G read the fields after pass G - P
   // (so pass G - P + 1 .. G - 1 run beside them); 2 .. kMaxDepth.  Deeper
   // pipelines take the pass hand-offs off the decisions' path but give each

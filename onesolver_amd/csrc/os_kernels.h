@@ -85,7 +85,10 @@ void launch_stream_pass(hipStream_t st, int I, int np, int Rp, int g0, int cnt, 
 // decide launch: block g, from the fields Fin (after blocks .. gb-1) plus the
 // flips of blocks gb .. g-1
 // variant: 10 * (pending blocks loaded per round trip) + (1 if the next
-// chunk's loads are issued before the current chunk is applied); 20 default
+// chunk's loads are issued before the current chunk is applied), or 60 + c
+// for the shared-operand form (waves 0 and 1 add the pending flips to both
+// replica halves of a column half, c = 0, 1, 2: 2, 1, 3 blocks per round
+// trip; waves 2 and 3 make the thresholds); 61 default
 void launch_stream_decide(hipStream_t st, int I, int n, int np, int Rp, int g, int gb, int ring,
                           int sweep, float ts, uint64_t seed, uint32_t inst_base,
                           bool last_of_sweep, const uint4 *qpack, const float *qdb,

@@ -165,12 +165,13 @@ def test_dense_streaming_pipeline_depth(ctx, monkeypatch, depth, group):
         assert (got[k] == ref[k]).all(), k
 
 
-@pytest.mark.parametrize("variant", [20, 21, 30, 31, 40, 50])
+@pytest.mark.parametrize("variant", [20, 21, 30, 31, 40, 50, 60, 61, 62])
 def test_dense_streaming_decide_chunking(ctx, monkeypatch, variant):
     """Every operand chunking of the decide launches (OS_STREAM_DECIDE_VARIANT:
-    blocks per round trip, next chunk requested early or not) gives the
-    oracle's chains; 7 blocks per pass leave up to 13 pending blocks at the
-    default depth of 2."""
+    blocks per round trip, next chunk requested early or not, 6x: one operand
+    load shared by both replica halves of a column half) gives the oracle's
+    chains; 7 blocks per pass leave up to 13 pending blocks at the default
+    depth of 2."""
     n, R, S = 1100, 64, 3
     q = _int_instance(n, 8)
     ts = _tscale(S)
