@@ -65,6 +65,7 @@ Ctx::~Ctx() {
     if (pinned[k]) (void)hipHostFree(pinned[k]);
     if (pinned_ev[k]) (void)hipEventDestroy(pinned_ev[k]);
   }
+  if (stage) (void)hipHostFree(stage);
   for (hipEvent_t e : ingest_ev)
     if (e) (void)hipEventDestroy(e);
   if (ev_start) (void)hipEventDestroy(ev_start);
@@ -73,6 +74,15 @@ Ctx::~Ctx() {
 }
 
 void Ctx::activate() const { OS_HIP_CHECK(hipSetDevice(device)); }
+
+void *Ctx::staging(size_t bytes) {
+  OS_REQUIRE(bytes <= kStageMax, "staging request above its limit");
+  if (!stage) {
+    OS_HIP_CHECK(hipHostMalloc(&stage, kStageMax, hipHostMallocDefault));
+    stage_bytes = kStageMax;
+  }
+  return stage;
+}
 
 void Ctx::sync() const { OS_HIP_CHECK(hipStreamSynchronize(stream)); }
 
@@ -202,9 +212,21 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
     c.b_energy.ensure(sizeof(double) * num_tries);
     c.b_scratch.ensure(os::energy_scratch_bytes(n, num_tries));
     if (n > os::ref_lds_limit()) c.b_fields.ensure(sizeof(double) * (size_t)num_tries * n);
-    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_sym, qbytes, hipMemcpyHostToDevice, c.stream));
+    // small runs move Q and beta up and the energies and every replica's
+    // state words down through the context's pinned staging buffer: no
+    // pageable copies, one synchronisation per call
+    const size_t bbytes = sizeof(double) * (size_t)num_iter, ebytes = sizeof(double) * num_tries,
+                 wbytes = sizeof(uint32_t) * (size_t)num_tries * W;
+    const bool staged = qbytes + bbytes + ebytes + wbytes <= Ctx::kStageMax;
+    char *h = staged ? static_cast<char *>(c.staging(qbytes + bbytes + ebytes + wbytes)) : nullptr;
+    if (staged) {
+      std::memcpy(h, q_sym, qbytes);
+      if (num_iter > 0) std::memcpy(h + qbytes, beta, bbytes);
+    }
+    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, staged ? h : (const char *)q_sym, qbytes,
+                                hipMemcpyHostToDevice, c.stream));
     if (num_iter > 0)
-      OS_HIP_CHECK(hipMemcpyAsync(c.b_beta.ptr, beta, sizeof(double) * num_iter,
+      OS_HIP_CHECK(hipMemcpyAsync(c.b_beta.ptr, staged ? h + qbytes : (const char *)beta, bbytes,
                                   hipMemcpyHostToDevice, c.stream));
     os::launch_ref_anneal(c.stream, n, c.b_q.as<double>(), c.b_beta.as<double>(), num_iter,
                           sweeps_per_beta, num_tries, seed,
@@ -213,10 +235,14 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
     os::launch_energy_fp64(c.stream, n, W, c.b_q.as<double>(), (int)num_tries,
                            c.b_bits.as<uint32_t>(), c.b_energy.as<double>(),
                            c.b_scratch.as<double>(), nullptr, 0, nullptr);
-    std::vector<double> e(num_tries);
-    OS_HIP_CHECK(hipMemcpyAsync(e.data(), c.b_energy.ptr, sizeof(double) * num_tries,
-                                hipMemcpyDeviceToHost, c.stream));
-    std::vector<uint8_t> bytes;
+    std::vector<double> ev(staged ? 0 : num_tries);
+    double *e = staged ? reinterpret_cast<double *>(h + qbytes + bbytes) : ev.data();
+    const uint32_t *words =
+        staged ? reinterpret_cast<const uint32_t *>(h + qbytes + bbytes + ebytes) : nullptr;
+    OS_HIP_CHECK(hipMemcpyAsync(e, c.b_energy.ptr, ebytes, hipMemcpyDeviceToHost, c.stream));
+    if (staged)
+      OS_HIP_CHECK(hipMemcpyAsync(const_cast<uint32_t *>(words), c.b_bits.ptr, wbytes,
+                                  hipMemcpyDeviceToHost, c.stream));
     if (best_states) {
       c.b_bytes.ensure((size_t)num_tries * n);
       os::launch_unpack_bits(c.stream, c.b_bits.as<uint32_t>(), n, num_tries, W,
@@ -226,13 +252,16 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
     }
     c.sync();
     // argmin over replicas, first index on ties (annealing.hpp:134-139)
-    const size_t best_idx = (size_t)(std::min_element(e.begin(), e.end()) - e.begin());
-    if (best_energies) std::memcpy(best_energies, e.data(), sizeof(double) * num_tries);
+    const size_t best_idx = (size_t)(std::min_element(e, e + num_tries) - e);
+    if (best_energies) std::memcpy(best_energies, e, ebytes);
     if (best_energy) *best_energy = e[best_idx];
     if (best_state) {
       std::vector<uint32_t> w(W);
-      OS_HIP_CHECK(hipMemcpy(w.data(), c.b_bits.as<uint32_t>() + best_idx * W,
-                             sizeof(uint32_t) * W, hipMemcpyDeviceToHost));
+      if (staged)
+        std::memcpy(w.data(), words + best_idx * W, sizeof(uint32_t) * W);
+      else
+        OS_HIP_CHECK(hipMemcpy(w.data(), c.b_bits.as<uint32_t>() + best_idx * W,
+                               sizeof(uint32_t) * W, hipMemcpyDeviceToHost));
       for (int j = 0; j < n; ++j) best_state[j] = (uint8_t)((w[j >> 5] >> (j & 31)) & 1u);
     }
   });

@@ -34,6 +34,12 @@ struct Ctx {
   DevBuf ingest[16];
   void *pinned[2] = {nullptr, nullptr};
   hipEvent_t pinned_ev[2] = {nullptr, nullptr};
+  // pinned staging of the one-shot entry points' small transfers (up to
+  // kStageMax bytes; idle whenever no call is running)
+  static constexpr size_t kStageMax = 1u << 20;
+  void *stage = nullptr;
+  size_t stage_bytes = 0;
+  void *staging(size_t bytes);
   hipEvent_t ingest_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // parse phase timing
 
   explicit Ctx(int dev);
