@@ -128,6 +128,30 @@ def test_batch_cli_reports_parse_errors(tmp_path):
     assert "b.qubo: Incorrect file, encountered coefficient from lower triangle" in r.stderr
 
 
+def test_batch_cli_reports_the_first_failing_file_in_order(tmp_path):
+    """The files are parsed by several host threads; with two bad files the
+    run still stops with the message of the first in name order, as a serial
+    run would, and writes no CSV."""
+    good = _sparse_text(20, 3, 1)
+    for k in range(12):
+        text = good
+        if k == 3:
+            text = "p qubo 0 100 2 2\n0 0 -0.5\n1 0 2.0\n"
+        if k == 8:
+            text = "p qubo 0 4 1 0\nnot a coefficient line\n"
+        (tmp_path / f"f{k:02d}.qubo").write_text(text)
+    out = tmp_path / "out"
+    out.mkdir()
+    r = _run(["--input", str(tmp_path), "--output", str(out)])
+    assert r.returncode == 1
+    assert "f03.qubo: Incorrect file, encountered coefficient from lower triangle" in r.stderr
+    assert "f08.qubo" not in r.stderr
+    assert not list(out.iterdir())
+    (tmp_path / "f03.qubo").write_text(good)
+    r = _run(["--input", str(tmp_path), "--output", str(out)])
+    assert r.returncode == 1 and "f08.qubo: " in r.stderr
+
+
 def test_batch_cli_forced_dense_path_needs_one_size(tmp_path):
     (tmp_path / "a.qubo").write_text(_sparse_text(20, 3, 1))
     (tmp_path / "b.qubo").write_text(_sparse_text(30, 3, 2))
