@@ -212,37 +212,48 @@ int os_anneal_reference(os_ctx *ctx_, int n, const double *q_sym, const double *
     c.b_energy.ensure(sizeof(double) * num_tries);
     c.b_scratch.ensure(os::energy_scratch_bytes(n, num_tries));
     if (n > os::ref_lds_limit()) c.b_fields.ensure(sizeof(double) * (size_t)num_tries * n);
-    // small runs move Q and beta up and the energies and every replica's
-    // state words down through the context's pinned staging buffer: no
-    // pageable copies, one synchronisation per call
+    // Small runs move Q and beta up in one copy and every replica's state
+    // words and energies down in one copy, through the context's pinned
+    // staging buffer: beta sits behind Q in b_q, the energies behind the
+    // words in b_bits.  No pageable copies, one synchronisation per call.
     const size_t bbytes = sizeof(double) * (size_t)num_iter, ebytes = sizeof(double) * num_tries,
-                 wbytes = sizeof(uint32_t) * (size_t)num_tries * W;
-    const bool staged = qbytes + bbytes + ebytes + wbytes <= Ctx::kStageMax;
-    char *h = staged ? static_cast<char *>(c.staging(qbytes + bbytes + ebytes + wbytes)) : nullptr;
+                 wbytes = sizeof(uint32_t) * (size_t)num_tries * W,
+                 wpad = (wbytes + 255) & ~(size_t)255;
+    const bool staged = qbytes + bbytes + wpad + ebytes <= Ctx::kStageMax;
+    char *h = staged ? static_cast<char *>(c.staging(qbytes + bbytes + wpad + ebytes)) : nullptr;
+    const double *beta_dev = c.b_beta.as<double>();
+    double *e_dev = c.b_energy.as<double>();
     if (staged) {
+      c.b_q.ensure(qbytes + bbytes);
+      c.b_bits.ensure(wpad + ebytes);
+      beta_dev = reinterpret_cast<const double *>(c.b_q.as<char>() + qbytes);
+      e_dev = reinterpret_cast<double *>(c.b_bits.as<char>() + wpad);
       std::memcpy(h, q_sym, qbytes);
       if (num_iter > 0) std::memcpy(h + qbytes, beta, bbytes);
+      OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, h, qbytes + bbytes, hipMemcpyHostToDevice,
+                                  c.stream));
+    } else {
+      OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, q_sym, qbytes, hipMemcpyHostToDevice, c.stream));
+      if (num_iter > 0)
+        OS_HIP_CHECK(hipMemcpyAsync(c.b_beta.ptr, beta, bbytes, hipMemcpyHostToDevice,
+                                    c.stream));
     }
-    OS_HIP_CHECK(hipMemcpyAsync(c.b_q.ptr, staged ? h : (const char *)q_sym, qbytes,
-                                hipMemcpyHostToDevice, c.stream));
-    if (num_iter > 0)
-      OS_HIP_CHECK(hipMemcpyAsync(c.b_beta.ptr, staged ? h + qbytes : (const char *)beta, bbytes,
-                                  hipMemcpyHostToDevice, c.stream));
-    os::launch_ref_anneal(c.stream, n, c.b_q.as<double>(), c.b_beta.as<double>(), num_iter,
-                          sweeps_per_beta, num_tries, seed,
+    os::launch_ref_anneal(c.stream, n, c.b_q.as<double>(), beta_dev, num_iter, sweeps_per_beta,
+                          num_tries, seed,
                           n > os::ref_lds_limit() ? c.b_fields.as<double>() : nullptr,
                           c.b_bits.as<uint32_t>(), c.b_run.as<double>());
     os::launch_energy_fp64(c.stream, n, W, c.b_q.as<double>(), (int)num_tries,
-                           c.b_bits.as<uint32_t>(), c.b_energy.as<double>(),
-                           c.b_scratch.as<double>(), nullptr, 0, nullptr);
+                           c.b_bits.as<uint32_t>(), e_dev, c.b_scratch.as<double>(), nullptr, 0,
+                           nullptr);
     std::vector<double> ev(staged ? 0 : num_tries);
-    double *e = staged ? reinterpret_cast<double *>(h + qbytes + bbytes) : ev.data();
-    const uint32_t *words =
-        staged ? reinterpret_cast<const uint32_t *>(h + qbytes + bbytes + ebytes) : nullptr;
-    OS_HIP_CHECK(hipMemcpyAsync(e, c.b_energy.ptr, ebytes, hipMemcpyDeviceToHost, c.stream));
+    double *e = staged ? reinterpret_cast<double *>(h + qbytes + bbytes + wpad) : ev.data();
+    const uint32_t *words = staged ? reinterpret_cast<const uint32_t *>(h + qbytes + bbytes)
+                                   : nullptr;
     if (staged)
-      OS_HIP_CHECK(hipMemcpyAsync(const_cast<uint32_t *>(words), c.b_bits.ptr, wbytes,
+      OS_HIP_CHECK(hipMemcpyAsync(h + qbytes + bbytes, c.b_bits.ptr, wpad + ebytes,
                                   hipMemcpyDeviceToHost, c.stream));
+    else
+      OS_HIP_CHECK(hipMemcpyAsync(e, e_dev, ebytes, hipMemcpyDeviceToHost, c.stream));
     if (best_states) {
       c.b_bytes.ensure((size_t)num_tries * n);
       os::launch_unpack_bits(c.stream, c.b_bits.as<uint32_t>(), n, num_tries, W,

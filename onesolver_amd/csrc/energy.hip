@@ -178,10 +178,38 @@ size_t energy_scratch_bytes(int n, int count) {
   return sizeof(double) * (size_t)count * ncb;
 }
 
+// n <= 64, energies only: one thread per state, in the reference's order
+// (sa::energy, annealing.hpp:31-40): one FP64 accumulator over the upper
+// triangle row by row, q[i][j] * x_i * x_j for every j >= i (zero terms
+// included, as the reference's loop has them).  Q is read through the cache
+// by every thread alike.
+__global__ __launch_bounds__(256) void energy_small_kernel(int n, int W, const double *__restrict__ q,
+                                                          int count,
+                                                          const uint32_t *__restrict__ bits,
+                                                          double *__restrict__ out) {
+  const int r = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
+  if (r >= count) return;
+  const uint32_t *x = bits + ((size_t)s * count + r) * W;
+  const uint64_t m = (uint64_t)x[0] | (W > 1 ? (uint64_t)x[1] << 32 : 0ull);
+  const double *qs = q + (size_t)s * n * n;
+  double e = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double xi = (double)((m >> i) & 1ull);
+    for (int j = i; j < n; ++j) e += qs[i * n + j] * xi * (double)((m >> j) & 1ull);
+  }
+  out[(size_t)s * count + r] = e;
+}
+
 void launch_energy_fp64(hipStream_t st, int n, int W, const double *q, int count,
                         const uint32_t *bits, double *out, double *scratch, float *fields, int ldf,
                         unsigned int *drift_max, int num_instances) {
   if (count <= 0 || num_instances <= 0) return;
+  if (!fields && n <= 64) {
+    energy_small_kernel<<<dim3((count + 255) / 256, num_instances), 256, 0, st>>>(n, W, q, count,
+                                                                                bits, out);
+    OS_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int ncb = (n + kTN - 1) / kTN;
   dim3 grid(ncb, (count + kTR - 1) / kTR, num_instances);
   if (fields)
