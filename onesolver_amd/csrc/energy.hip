@@ -181,21 +181,27 @@ size_t energy_scratch_bytes(int n, int count) {
 // n <= 64, energies only: one thread per state, in the reference's order
 // (sa::energy, annealing.hpp:31-40): one FP64 accumulator over the upper
 // triangle row by row, q[i][j] * x_i * x_j for every j >= i (zero terms
-// included, as the reference's loop has them).  Q is read through the cache
-// by every thread alike.
+// included, as the reference's loop has them).  The work-group stages Q in
+// LDS first; every thread then reads the same addresses (broadcasts).
 __global__ __launch_bounds__(256) void energy_small_kernel(int n, int W, const double *__restrict__ q,
                                                           int count,
                                                           const uint32_t *__restrict__ bits,
                                                           double *__restrict__ out) {
-  const int r = blockIdx.x * 256 + threadIdx.x, s = blockIdx.y;
+  __shared__ double ql[64 * 64];
+  const int s = blockIdx.y;
+  const double *qs = q + (size_t)s * n * n;
+  for (int k = threadIdx.x; k < n * n; k += 256) ql[k] = qs[k];
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
   if (r >= count) return;
   const uint32_t *x = bits + ((size_t)s * count + r) * W;
   const uint64_t m = (uint64_t)x[0] | (W > 1 ? (uint64_t)x[1] << 32 : 0ull);
-  const double *qs = q + (size_t)s * n * n;
   double e = 0.0;
   for (int i = 0; i < n; ++i) {
     const double xi = (double)((m >> i) & 1ull);
-    for (int j = i; j < n; ++j) e += qs[i * n + j] * xi * (double)((m >> j) & 1ull);
+    const double *row = ql + i * n;
+#pragma unroll 8
+    for (int j = i; j < n; ++j) e += row[j] * xi * (double)((m >> j) & 1ull);
   }
   out[(size_t)s * count + r] = e;
 }
