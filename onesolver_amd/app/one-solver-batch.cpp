@@ -164,6 +164,24 @@ void flatten(const Instance &in, double *q) {
   }
 }
 
+// fn(0) .. fn(count - 1) on up to `threads` host threads (fn must not throw).
+template <class F>
+void parallel_for(size_t count, size_t threads, F fn) {
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t k; (k = next++) < count;) fn(k);
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < std::min(threads, count); ++t) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
+size_t host_threads() {
+  const unsigned hw = std::thread::hardware_concurrency();
+  return std::max<size_t>(1, std::min<size_t>(8, hw));
+}
+
 // Symmetric off-diagonal rows sorted by column, and the diagonal.  The
 // entries are bucketed by row (counting sort), then each row is sorted by
 // column, stably: O(nnz) for rows of a few entries.
@@ -387,11 +405,37 @@ int main(int argc, char *argv[]) {
         best_x[k].assign(x, x + n);
       }
     } else {
+      // each instance's rows on a host thread, then the batch arrays in order
+      std::vector<std::vector<int32_t>> rps(I), cols(I);
+      std::vector<std::vector<double>> vals(I), diags(I);
+      std::vector<std::string> cerr_k(I);
+      parallel_for((size_t)I, host_threads(), [&](size_t k) {
+        try {
+          to_csr(insts[k], rps[k], cols[k], vals[k], diags[k]);
+        } catch (std::exception &e) {
+          cerr_k[k] = e.what();
+        }
+      });
+      for (const auto &e : cerr_k)
+        if (!e.empty()) throw std::runtime_error(e);
       std::vector<int32_t> ns, rp, col;
       std::vector<double> val, diag;
-      for (const auto &in : insts) {
-        ns.push_back(in.n);
-        to_csr(in, rp, col, val, diag);
+      size_t nrp = 0, nnz = 0, nsp = 0;
+      for (int32_t k = 0; k < I; ++k) {
+        nrp += rps[k].size();
+        nnz += cols[k].size();
+        nsp += diags[k].size();
+      }
+      rp.reserve(nrp);
+      col.reserve(nnz);
+      val.reserve(nnz);
+      diag.reserve(nsp);
+      for (int32_t k = 0; k < I; ++k) {
+        ns.push_back(insts[k].n);
+        rp.insert(rp.end(), rps[k].begin(), rps[k].end());
+        col.insert(col.end(), cols[k].begin(), cols[k].end());
+        val.insert(val.end(), vals[k].begin(), vals[k].end());
+        diag.insert(diag.end(), diags[k].begin(), diags[k].end());
       }
       os_csr_batch *b = nullptr;
       check(os_csr_batch_create(ctx, I, ns.data(), rp.data(), col.data(), val.data(), diag.data(),
@@ -428,14 +472,31 @@ int main(int argc, char *argv[]) {
     const double anneal_ms = ms_since(t0);
     t0 = clock::now();
 
-    for (int32_t k = 0; k < I; ++k) {
+    // the CSVs on host threads; the log lines, and an error for the first
+    // file in order that could not be written, as a serial loop gives them
+    std::vector<std::string> werr(I);
+    parallel_for((size_t)I, host_threads(), [&](size_t k) {
       const std::string out = output + "/" + stem(insts[k].file) + ".csv";
       std::ofstream f(out);
-      if (!f) throw std::runtime_error("can not open output file: " + out);
-      std::vector<char> x(best_x[k].begin(), best_x[k].end());
-      qubo::Solution(x.begin(), x.end(), best_e[k]).save(f);
-      std::cout << insts[k].file << ": n=" << insts[k].n << " energy=" << best_e[k] << std::endl;
+      if (!f) {
+        werr[k] = "can not open output file: " + out;
+        return;
+      }
+      try {
+        std::vector<char> x(best_x[k].begin(), best_x[k].end());
+        qubo::Solution(x.begin(), x.end(), best_e[k]).save(f);
+      } catch (std::exception &e) {
+        werr[k] = e.what();
+      }
+    });
+    for (int32_t k = 0; k < I; ++k) {
+      if (!werr[k].empty()) {
+        std::cout.flush();
+        throw std::runtime_error(werr[k]);
+      }
+      std::cout << insts[k].file << ": n=" << insts[k].n << " energy=" << best_e[k] << '\n';
     }
+    std::cout.flush();
     std::cout << "Read and parse: " << parse_ms << " ms" << std::endl;
     std::cout << "Anneal: " << anneal_ms << " ms" << std::endl;
     std::cout << "Write: " << ms_since(t0) << " ms" << std::endl;
