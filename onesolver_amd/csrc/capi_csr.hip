@@ -13,6 +13,7 @@
 // greedily when the batch is created (index order, smallest colour free of
 // lower-index neighbours) for the colour-class site order.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -119,6 +120,24 @@ struct CsrBatch {
                         diag64.as<double>(), best.as<uint32_t>(), e64.as<double>());
   }
 };
+
+// fn(0) .. fn(count - 1) on up to 8 host threads (fn must not throw)
+template <class F>
+void host_parallel_for(int count, F fn) {
+  const int T = std::max(1, std::min({8, count, (int)std::thread::hardware_concurrency()}));
+  if (T == 1) {
+    for (int i = 0; i < count; ++i) fn(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  auto work = [&] {
+    for (int i; (i = next++) < count;) fn(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
 
 namespace {
 
@@ -533,23 +552,31 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
     b->I = num_instances;
     // validate and lay out (host)
     int64_t rows = 0, nnz = 0, spins = 0, ell = 0;
-    std::vector<int32_t> ell_c;
-    std::vector<float> ell_v, dg;
-    for (int i = 0; i < num_instances; ++i) {
+    // Pass 1 (serial, O(n) per instance): sizes, offsets and the row_ptr
+    // checks, up to the first instance that fails one.
+    std::vector<os::CsrInst> ds;
+    const char *bad_msg = nullptr;
+    for (int i = 0; i < num_instances && !bad_msg; ++i) {
       const int ni = n[i];
-      OS_REQUIRE(ni >= 1, "instance size must be positive");
+      if (ni < 1) {
+        bad_msg = "instance size must be positive";
+        break;
+      }
       const int32_t *rp = row_ptr + rows;
-      OS_REQUIRE(rp[0] == 0, "row_ptr of each instance must start at 0");
+      if (rp[0] != 0) {
+        bad_msg = "row_ptr of each instance must start at 0";
+        break;
+      }
       int maxdeg = 0;
-      for (int j = 0; j < ni; ++j) {
-        OS_REQUIRE(rp[j + 1] >= rp[j], "row_ptr must be non-decreasing");
+      for (int j = 0; j < ni && !bad_msg; ++j) {
+        if (rp[j + 1] < rp[j]) bad_msg = "row_ptr must be non-decreasing";
         maxdeg = std::max(maxdeg, rp[j + 1] - rp[j]);
       }
+      if (bad_msg) break;
       const int nz = rp[ni];
-      OS_REQUIRE(nz == 0 || (col && val), "null col/val");
-      for (int k = 0; k < nz; ++k) {
-        const int l = col[nnz + k];
-        OS_REQUIRE(l >= 0 && l < ni, "column index out of range");
+      if (!(nz == 0 || (col && val))) {
+        bad_msg = "null col/val";
+        break;
       }
       os::CsrInst d{};
       d.n = ni;
@@ -559,50 +586,81 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
       d.spin_off = spins;
       d.row_off = rows;
       d.csr_off = nnz;
-      ell_c.resize((size_t)(ell + (int64_t)ni * d.width), ni);
-      ell_v.resize(ell_c.size(), 0.0f);
-      for (int j = 0; j < ni; ++j)
-        for (int k = rp[j]; k < rp[j + 1]; ++k) {
-          const size_t at = (size_t)(ell + (int64_t)j * d.width + (k - rp[j]));
-          ell_c[at] = col[nnz + k];
-          ell_v[at] = (float)val[nnz + k];
-        }
-      for (int j = 0; j < ni; ++j) dg.push_back((float)diag[spins + j]);
-      // colour classes (greedy, index order)
-      {
-        std::vector<int32_t> colour(ni);
-        std::vector<uint8_t> used((size_t)maxdeg + 2, 0);
-        int ncol = 0;
-        for (int j = 0; j < ni; ++j) {
-          for (int k = rp[j]; k < rp[j + 1]; ++k)
-            if (col[nnz + k] < j) used[colour[col[nnz + k]]] = 1;
-          int cc = 0;
-          while (used[cc]) ++cc;
-          colour[j] = cc;
-          ncol = std::max(ncol, cc + 1);
-          for (int k = rp[j]; k < rp[j + 1]; ++k)
-            if (col[nnz + k] < j) used[colour[col[nnz + k]]] = 0;
-        }
-        d.ncol = ncol;
-        d.class_off = (int64_t)b->h_cstart.size();
-        std::vector<int32_t> start(ncol + 1, 0);
-        for (int j = 0; j < ni; ++j) ++start[colour[j] + 1];
-        for (int cc = 0; cc < ncol; ++cc) start[cc + 1] += start[cc];
-        b->h_cstart.insert(b->h_cstart.end(), start.begin(), start.end());
-        std::vector<int32_t> fill(start.begin(), start.end() - 1), ord(ni);
-        for (int j = 0; j < ni; ++j) ord[fill[colour[j]]++] = j;
-        b->h_order.insert(b->h_order.end(), ord.begin(), ord.end());
-      }
-      b->max_words = std::max(b->max_words, d.words);
-      b->max_width = std::max(b->max_width, d.width);
-      b->total_words += d.words;
-      b->inst.push_back(d);
+      ds.push_back(d);
       ell += (int64_t)ni * d.width;
       rows += ni + 1;
       nnz += nz;
       spins += ni;
     }
+    // Pass 2 (host threads, one instance each): column checks, ELL rows
+    // (padded with column n), the fp32 diagonal and the greedy colouring
+    // (index order, smallest colour free of lower-index neighbours).
+    const int nd = (int)ds.size();
+    std::vector<int32_t> ell_c((size_t)ell);
+    std::vector<float> ell_v((size_t)ell, 0.0f), dg((size_t)spins);
+    std::vector<std::vector<int32_t>> cst(nd), cord(nd);
+    std::vector<const char *> err(nd, nullptr);
+    os::host_parallel_for(nd, [&](int i) {
+      os::CsrInst &d = ds[i];
+      const int ni = d.n;
+      const int32_t *rp = row_ptr + d.row_off;
+      const int32_t *cl = col + d.csr_off;
+      const double *vl = val + d.csr_off;
+      for (int k = 0; k < rp[ni]; ++k)
+        if (cl[k] < 0 || cl[k] >= ni) {
+          err[i] = "column index out of range";
+          return;
+        }
+      int32_t *ec = ell_c.data() + d.ell_off;
+      float *ev = ell_v.data() + d.ell_off;
+      std::fill(ec, ec + (size_t)ni * d.width, ni);
+      int maxdeg = 0;
+      for (int j = 0; j < ni; ++j) {
+        maxdeg = std::max(maxdeg, rp[j + 1] - rp[j]);
+        for (int k = rp[j]; k < rp[j + 1]; ++k) {
+          ec[(size_t)j * d.width + (k - rp[j])] = cl[k];
+          ev[(size_t)j * d.width + (k - rp[j])] = (float)vl[k];
+        }
+      }
+      for (int j = 0; j < ni; ++j) dg[(size_t)d.spin_off + j] = (float)diag[d.spin_off + j];
+      std::vector<int32_t> colour(ni);
+      std::vector<uint8_t> used((size_t)maxdeg + 2, 0);
+      int ncol = 0;
+      for (int j = 0; j < ni; ++j) {
+        for (int k = rp[j]; k < rp[j + 1]; ++k)
+          if (cl[k] < j) used[colour[cl[k]]] = 1;
+        int cc = 0;
+        while (used[cc]) ++cc;
+        colour[j] = cc;
+        ncol = std::max(ncol, cc + 1);
+        for (int k = rp[j]; k < rp[j + 1]; ++k)
+          if (cl[k] < j) used[colour[cl[k]]] = 0;
+      }
+      d.ncol = ncol;
+      std::vector<int32_t> &start = cst[i];
+      start.assign(ncol + 1, 0);
+      for (int j = 0; j < ni; ++j) ++start[colour[j] + 1];
+      for (int cc = 0; cc < ncol; ++cc) start[cc + 1] += start[cc];
+      std::vector<int32_t> fill(start.begin(), start.end() - 1);
+      cord[i].resize(ni);
+      for (int j = 0; j < ni; ++j) cord[i][fill[colour[j]]++] = j;
+    });
+    // the first failing instance in order, as a serial pass reports it
+    for (int i = 0; i < nd; ++i) OS_REQUIRE(!err[i], err[i]);
+    OS_REQUIRE(!bad_msg, bad_msg);
+    // Pass 3 (serial): class offsets and the concatenated class arrays
+    for (int i = 0; i < nd; ++i) {
+      os::CsrInst &d = ds[i];
+      d.class_off = (int64_t)b->h_cstart.size();
+      b->h_cstart.insert(b->h_cstart.end(), cst[i].begin(), cst[i].end());
+      b->h_order.insert(b->h_order.end(), cord[i].begin(), cord[i].end());
+      b->max_words = std::max(b->max_words, d.words);
+      b->max_width = std::max(b->max_width, d.width);
+      b->total_words += d.words;
+    }
+    b->inst = ds;
     b->total_spins = spins;
+
     OS_REQUIRE(os::csr_lds_bytes(1, b->max_words) <= 160 * 1024,
                "csr batch: instance too large for the LDS-resident state (n <= 40000)");
     hipStream_t s = c.stream;
@@ -631,7 +689,8 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
     {
       std::vector<int32_t> pc(ell_c.size());
       std::vector<float> pv(ell_v.size()), pd(dg.size());
-      for (const auto &d : b->inst) {
+      os::host_parallel_for((int)b->inst.size(), [&](int i) {
+        const auto &d = b->inst[i];
         const int32_t *ord = b->h_order.data() + d.spin_off;
         for (int pos = 0; pos < d.n; ++pos) {
           const int j = ord[pos];
@@ -641,7 +700,7 @@ int os_csr_batch_create(os_ctx *ctx_, int32_t num_instances, const int32_t *n,
                       &ell_v[(size_t)(d.ell_off + (int64_t)j * d.width)], sizeof(float) * d.width);
           pd[(size_t)(d.spin_off + pos)] = dg[(size_t)(d.spin_off + j)];
         }
-      }
+      });
       b->pell_col.ensure(sizeof(int32_t) * std::max<size_t>(pc.size(), 4));
       b->pell_val.ensure(sizeof(float) * std::max<size_t>(pv.size(), 4));
       b->pdiag.ensure(sizeof(float) * std::max<size_t>(pd.size(), 1));

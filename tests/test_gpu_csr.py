@@ -229,6 +229,40 @@ def test_csr_colour_classes_match_oracle(ctx, monkeypatch, sites):
     b.close()
 
 
+def test_csr_batch_create_reports_the_first_bad_instance(ctx):
+    """Batch creation checks the instances on several host threads; the error
+    is the first bad instance's in order (row_ptr checks before column checks
+    within an instance), as a serial pass reports it, and good batches still
+    build the greedy colouring the oracle gives."""
+    good = [_irregular(40, k) for k in range(6)]
+
+    def bad_col(inst):
+        rp, cl, vl, dg = (np.array(a) for a in inst)
+        cl[len(cl) // 2] = len(dg) + 3
+        return rp, cl, vl, dg
+
+    def bad_rp(inst):
+        rp, cl, vl, dg = (np.array(a) for a in inst)
+        rp[5], rp[6] = rp[6], rp[5] - 1
+        return rp, cl, vl, dg
+
+    cases = [
+        (good[:2] + [bad_col(good[2])] + good[3:4] + [bad_rp(good[4])], "column index out of range"),
+        (good[:1] + [bad_rp(good[1])] + [bad_col(good[2])], "row_ptr must be non-decreasing"),
+        (good[:3] + [(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), np.zeros(0))],
+         "instance size must be positive"),
+    ]
+    for insts, msg in cases:
+        with pytest.raises(onesolver_amd.InvalidArgument, match=msg):
+            onesolver_amd.CsrBatch(ctx, insts)
+    b = onesolver_amd.CsrBatch(ctx, good)
+    for i, (rp, cl, vl, dg) in enumerate(good):
+        order, starts = b.site_order(i)
+        o2, cs2 = oracle.csr_colour_order(rp, cl)
+        assert (order == o2).all() and (starts == cs2).all()
+    b.close()
+
+
 def test_csr_colour_mixed_sites_per_step_match_oracle(ctx, monkeypatch):
     """The untuned policy with at most 128 steps per class sweep
     (OS_CSR_COLOUR_LIMIT) gives small instances 4 sites per step, mid-size 8
